@@ -1,0 +1,518 @@
+// Device-mesh repack: validates a MiniSoA view and builds the HBM layout
+// described in devmesh.hpp. Runs once per mesh, on the host, before any
+// kernel (the reference builds its own cell→face CSR at the same point,
+// CupCfdSoAMesh::updateCellFaceMap, src/geometry/mesh/implementation/component/CupCfdSoAMesh.cpp:549-668).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "devmesh.hpp"
+
+namespace cupcfd_hip {
+
+template <class T> Mesh<T>::~Mesh() {
+    for (void *p : allocs) (void)hipFree(p);
+    for (void *p : ws) (void)hipFree(p);
+    for (int k = 0; k < 4; k++)
+        if (sc_src[k]) (void)hipFree(sc_src[k]);
+}
+
+template <class T> void *Mesh<T>::ws_get(int slot, size_t bytes) {
+    if ((int)ws.size() <= slot) {
+        ws.resize(slot + 1, nullptr);
+        ws_size.resize(slot + 1, 0);
+    }
+    if (ws_size[slot] < bytes) {
+        if (ws[slot]) (void)hipFree(ws[slot]);
+        ws[slot] = nullptr;
+        ws_size[slot] = 0;
+        if (hipMalloc(&ws[slot], bytes) != hipSuccess) return nullptr;
+        ws_size[slot] = bytes;
+    }
+    return ws[slot];
+}
+
+template struct Mesh<double>;
+template struct Mesh<float>;
+
+namespace {
+
+template <class V3> const auto *v3ptr(const V3 &v, int64_t i, int k) {
+    using S = std::remove_pointer_t<decltype(v.base)>;
+    int64_t stride = v.stride ? v.stride : (int64_t)(3 * sizeof(S));
+    return (const S *)((const char *)v.base + i * stride) + k;
+}
+
+struct Uploader {
+    std::vector<void *> &allocs;
+    bool ok = true;
+    template <class X> X *put(const X *src, size_t n) {
+        if (!ok) return nullptr;
+        if (n == 0) n = 1;  // keep every pointer valid
+        void *p = nullptr;
+        if (hipMalloc(&p, n * sizeof(X)) != hipSuccess) {
+            ok = false;
+            return nullptr;
+        }
+        allocs.push_back(p);
+        if (src && hipMemcpy(p, src, n * sizeof(X), hipMemcpyHostToDevice) != hipSuccess) ok = false;
+        return (X *)p;
+    }
+};
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
+    const int C = v.nCells, F = v.nFaces, B = v.nBoundaries, R = v.nRegions;
+    if (C < 0 || F < 0 || B < 0 || R < 0) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if ((int64_t)F >= ((int64_t)1 << 30)) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if ((C && !v.cellVolume) || (F && (!v.faceCell1ID || !v.faceCell2ID || !v.faceBoundaryID || !v.faceLambda ||
+                                      !v.faceArea || !v.faceRLencos || !v.faceNorm.base || !v.faceCenter.base ||
+                                      !v.faceXpac.base || !v.faceXnac.base)) ||
+        (C && !v.cellCenter.base) || (B && (!v.boundaryFaceID || !v.boundaryRegionID || !v.boundaryDistance)) ||
+        (R && !v.regionType))
+        return CUPCFD_E_NULL_PTR;
+
+    // ---- validation -------------------------------------------------------
+    int Fi = 0;
+    for (int f = 0; f < F; f++) {
+        int c1 = v.faceCell1ID[f], c2 = v.faceCell2ID[f], b = v.faceBoundaryID[f];
+        if (c1 < 0 || c1 >= C) return CUPCFD_E_INVALID_INDEX;
+        if (b > -1) {
+            if (b >= B) return CUPCFD_E_INVALID_INDEX;
+        } else {
+            if (c2 < 0 || c2 >= C) return CUPCFD_E_INVALID_INDEX;
+            Fi++;
+        }
+    }
+    for (int b = 0; b < B; b++) {
+        int f = v.boundaryFaceID[b], r = v.boundaryRegionID[b];
+        if (f < 0 || f >= F || r < 0 || r >= R) return CUPCFD_E_INVALID_INDEX;
+    }
+    for (int r = 0; r < R; r++)
+        if (v.regionType[r] < 0 || v.regionType[r] > 4) return CUPCFD_E_CONFIG_INVALID_VALUE;
+
+    // ---- cell order ---------------------------------------------------------
+    M.cell_order.resize(C);
+    for (int c = 0; c < C; c++) M.cell_order[c] = c;
+    if ((flags & CUPCFD_MESH_REORDER) && C > 1) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < C; c++)
+            for (int k = 0; k < 3; k++) {
+                double x = (double)*v3ptr(v.cellCenter, c, k);
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+            }
+        std::vector<uint64_t> key(C);
+        for (int c = 0; c < C; c++) {
+            uint64_t q[3];
+            for (int k = 0; k < 3; k++) {
+                double span = hi[k] - lo[k];
+                double t = span > 0 ? ((double)*v3ptr(v.cellCenter, c, k) - lo[k]) / span : 0.0;
+                q[k] = (uint64_t)std::min(2097151.0, std::max(0.0, std::floor(t * 2097151.0 + 0.5)));
+            }
+            key[c] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+        }
+        std::stable_sort(M.cell_order.begin(), M.cell_order.end(),
+                         [&](int a, int b) { return key[a] < key[b]; });
+    }
+    M.cell_rank.assign(C, 0);
+    for (int k = 0; k < C; k++) M.cell_rank[M.cell_order[k]] = k;
+
+    // ---- face order: (boundary?, owner) buckets, mesh order within ----------
+    std::vector<int> cnt(2 * (size_t)C + 1, 0);
+    auto bucket = [&](int f) {
+        int own = M.cell_rank[v.faceCell1ID[f]];
+        return (v.faceBoundaryID[f] > -1 ? C : 0) + own;
+    };
+    for (int f = 0; f < F; f++) cnt[bucket(f) + 1]++;
+    for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+    M.face_order.assign(F, 0);
+    for (int f = 0; f < F; f++) M.face_order[cnt[bucket(f)]++] = f;
+    M.face_rank.assign(F, 0);
+    for (int k = 0; k < F; k++) M.face_rank[M.face_order[k]] = k;
+
+    // ---- face streams -------------------------------------------------------
+    std::vector<int2> fconn(F);
+    std::vector<V4<T>> fS(F), fX(F), fP(F), fN(F);
+    M.fconn_h.assign(2 * (size_t)F, 0);
+    for (int k = 0; k < F; k++) {
+        int f = M.face_order[k];
+        int b = v.faceBoundaryID[f];
+        int nb = b > -1 ? ~b : M.cell_rank[v.faceCell2ID[f]];
+        fconn[k] = make_int2(M.cell_rank[v.faceCell1ID[f]], nb);
+        M.fconn_h[2 * k] = fconn[k].x;
+        M.fconn_h[2 * k + 1] = nb;
+        fS[k] = {*v3ptr(v.faceNorm, f, 0), *v3ptr(v.faceNorm, f, 1), *v3ptr(v.faceNorm, f, 2), v.faceLambda[f]};
+        fX[k] = {*v3ptr(v.faceCenter, f, 0), *v3ptr(v.faceCenter, f, 1), *v3ptr(v.faceCenter, f, 2),
+                 v.faceRLencos[f]};
+        fP[k] = {*v3ptr(v.faceXpac, f, 0), *v3ptr(v.faceXpac, f, 1), *v3ptr(v.faceXpac, f, 2), v.faceArea[f]};
+        fN[k] = {*v3ptr(v.faceXnac, f, 0), *v3ptr(v.faceXnac, f, 1), *v3ptr(v.faceXnac, f, 2), (T)0};
+    }
+    std::vector<V4<T>> cg(C);
+    std::vector<T> cv(C);
+    for (int k = 0; k < C; k++) {
+        int c = M.cell_order[k];
+        cg[k] = {*v3ptr(v.cellCenter, c, 0), *v3ptr(v.cellCenter, c, 1), *v3ptr(v.cellCenter, c, 2),
+                 v.cellVolume[c]};
+        cv[k] = v.cellVolume[c];
+    }
+
+    // ---- cell→face CSR, rows in mesh face order ----------------------------
+    std::vector<int> row(C + 1, 0);
+    for (int f = 0; f < F; f++) {
+        row[M.cell_rank[v.faceCell1ID[f]] + 1]++;
+        if (v.faceBoundaryID[f] <= -1) row[M.cell_rank[v.faceCell2ID[f]] + 1]++;
+    }
+    for (int c = 0; c < C; c++) row[c + 1] += row[c];
+    const int nnz = row[C];
+    std::vector<int> ent(nnz), fill(row.begin(), row.end() - 1);
+    for (int f = 0; f < F; f++) {
+        int k = M.face_rank[f];
+        ent[fill[M.cell_rank[v.faceCell1ID[f]]]++] = (k << 1);
+        if (v.faceBoundaryID[f] <= -1) ent[fill[M.cell_rank[v.faceCell2ID[f]]]++] = (k << 1) | 1;
+    }
+
+    // ---- boundaries / regions ----------------------------------------------
+    std::vector<int> bface(B), bcell(B);
+    M.btype_h.assign(B, 0);
+    for (int t = 0; t < 5; t++) M.ntype[t] = 0;
+    std::vector<int> blist[5];
+    for (int b = 0; b < B; b++) {
+        int f = v.boundaryFaceID[b];
+        bface[b] = M.face_rank[f];
+        bcell[b] = M.cell_rank[v.faceCell1ID[f]];
+        int t = v.regionType[v.boundaryRegionID[b]];
+        M.btype_h[b] = t;
+        blist[t].push_back(b);
+    }
+    // face-loop counters count boundary faces by type (MassKernels.ipp:248-304)
+    for (int f = 0; f < F; f++) {
+        int b = v.faceBoundaryID[f];
+        if (b > -1) M.ntype[v.regionType[v.boundaryRegionID[b]]]++;
+    }
+    std::vector<int> bcells;
+    {
+        std::vector<char> has(C, 0);
+        for (int k = Fi; k < F; k++) has[fconn[k].x] = 1;
+        for (int c = 0; c < C; c++)
+            if (has[c]) bcells.push_back(c);
+    }
+    // OUTLET-owning cells with their OUTLET boundaries in ascending id
+    std::vector<int> ocells, orow(1, 0), oent;
+    {
+        std::vector<int> ocnt(C, 0);
+        for (int b : blist[CUPCFD_RTYPE_OUTLET]) ocnt[bcell[b]]++;
+        std::vector<int> start(C, -1);
+        for (int c = 0; c < C; c++)
+            if (ocnt[c]) {
+                start[c] = orow.back();
+                ocells.push_back(c);
+                orow.push_back(orow.back() + ocnt[c]);
+            }
+        oent.assign(orow.back(), 0);
+        for (int b : blist[CUPCFD_RTYPE_OUTLET]) oent[start[bcell[b]]++] = b;
+    }
+    bool typed = false;
+    for (int r = 0; r < R; r++)
+        if (v.regionType[r] != CUPCFD_RTYPE_DEFAULT) typed = true;
+    std::vector<int> carry;
+    if (typed) {
+        carry.assign(F, -1);
+        int last_io = -1, last_vis = -1;
+        for (int f = 0; f < F; f++) {
+            int k = M.face_rank[f];
+            int b = v.faceBoundaryID[f];
+            if (b <= -1) {
+                carry[k] = last_io;
+                last_vis = k;
+            } else {
+                int t = v.regionType[v.boundaryRegionID[b]];
+                if (t == CUPCFD_RTYPE_SYMP) carry[k] = last_vis;
+                if (t == CUPCFD_RTYPE_INLET || t == CUPCFD_RTYPE_OUTLET) {
+                    last_io = k;
+                    last_vis = k;
+                }
+            }
+        }
+    }
+
+    std::vector<T> bdist(B), rbuf;
+    std::vector<uint8_t> rstd(R), rflux(R), radiab(R);
+    for (int b = 0; b < B; b++) bdist[b] = v.boundaryDistance[b];
+    M.bregion_h.assign(v.boundaryRegionID, v.boundaryRegionID + B);
+    for (int r = 0; r < R; r++) {
+        rstd[r] = v.regionStd ? v.regionStd[r] : 0;
+        rflux[r] = v.regionFlux ? v.regionFlux[r] : 0;
+        radiab[r] = v.regionAdiab ? v.regionAdiab[r] : 0;
+    }
+
+    // ---- upload -------------------------------------------------------------
+    Uploader up{M.allocs};
+    DevMesh<T> &d = M.d;
+    d.C = C;
+    d.F = F;
+    d.Fi = Fi;
+    d.B = B;
+    d.R = R;
+    d.nnz = nnz;
+    d.fconn = up.put(fconn.data(), F);
+    d.fgeoS = up.put(fS.data(), F);
+    d.fgeoX = up.put(fX.data(), F);
+    d.fgeoP = up.put(fP.data(), F);
+    d.fgeoN = up.put(fN.data(), F);
+    d.cgeo = up.put(cg.data(), C);
+    d.cvol = up.put(cv.data(), C);
+    d.row = up.put(row.data(), C + 1);
+    d.ent = up.put(ent.data(), nnz);
+    d.bface = up.put(bface.data(), B);
+    d.bregion = up.put(v.boundaryRegionID, B);
+    d.bcell = up.put(bcell.data(), B);
+    d.bdist = up.put(bdist.data(), B);
+    d.byplus = up.put<T>(nullptr, B);
+    d.buplus = up.put<T>(nullptr, B);
+    d.bq = up.put<T>(nullptr, B);
+    d.bh = up.put<T>(nullptr, B);
+    d.bt = up.put<T>(nullptr, B);
+    d.bshear = up.put<T>(nullptr, 3 * (size_t)B);
+    d.rtype = up.put(v.regionType, R);
+    d.rstd = up.put(rstd.data(), R);
+    d.rflux = up.put(rflux.data(), R);
+    d.radiab = up.put(radiab.data(), R);
+    d.rylog = up.put<T>(nullptr, R);
+    d.rforcet = up.put<T>(nullptr, 3 * (size_t)R);
+    auto put_r = [&](const T *src) {
+        rbuf.assign(R, (T)0);
+        if (src)
+            for (int r = 0; r < R; r++) rbuf[r] = src[r];
+        return (const T *)up.put(rbuf.data(), R);
+    };
+    d.relog = put_r(v.regionELog);
+    d.rdensity = put_r(v.regionDensity);
+    d.rturbke = put_r(v.regionTurbKE);
+    d.rturbdiss = put_r(v.regionTurbDiss);
+    d.rsplvl = put_r(v.regionSplvl);
+    d.rden = put_r(v.regionDen);
+    d.rr = put_r(v.regionR);
+    d.rt = put_r(v.regionT);
+    {
+        std::vector<T> u(3 * (size_t)R, (T)0);
+        if (v.regionUVW.base)
+            for (int r = 0; r < R; r++)
+                for (int k = 0; k < 3; k++) u[3 * r + k] = *v3ptr(v.regionUVW, r, k);
+        d.ruvw = up.put(u.data(), 3 * (size_t)R);
+    }
+    d.carry_src = typed ? up.put(carry.data(), F) : nullptr;
+    d.typed = typed ? 1 : 0;
+    d.carry_init = (T)0;
+    d.bcells = up.put(bcells.data(), bcells.size());
+    d.nbc = (int)bcells.size();
+    for (int t = 0; t < 5; t++) {
+        d.blist[t] = up.put(blist[t].data(), blist[t].size());
+        d.nblist[t] = (int)blist[t].size();
+    }
+    d.ocells = up.put(ocells.data(), ocells.size());
+    d.orow = up.put(orow.data(), orow.size());
+    d.oent = up.put(oent.data(), oent.size());
+    d.noc = (int)ocells.size();
+    if (!up.ok) return CUPCFD_E_ERROR;
+    M.radiab_h = radiab;
+    return CUPCFD_E_SUCCESS;
+}
+
+}  // namespace
+
+// Copy the view's mutable mesh state to the device (push) or back (pull).
+template <class T, class View> int sync_state(Mesh<T> &M, const View &v, bool push) {
+    const DevMesh<T> &d = M.d;
+    const int B = d.B, R = d.R;
+    std::vector<T> buf;
+    auto xfer = [&](T *dev, T *host, int n) -> bool {
+        if (!host || n == 0) return true;
+        if (push) return hipMemcpy(dev, host, n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+        return hipMemcpy(host, dev, n * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    auto xfer3 = [&](T *dev, const decltype(v.boundaryShear) &hv, int n) -> bool {
+        if (!hv.base || n == 0) return true;
+        int64_t stride = hv.stride ? hv.stride : (int64_t)(3 * sizeof(T));
+        buf.resize(3 * (size_t)n);
+        if (push) {
+            for (int i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++) buf[3 * i + k] = ((T *)((char *)hv.base + i * stride))[k];
+            return hipMemcpy(dev, buf.data(), 3 * n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+        }
+        if (hipMemcpy(buf.data(), dev, 3 * n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return false;
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) ((T *)((char *)hv.base + i * stride))[k] = buf[3 * i + k];
+        return true;
+    };
+    bool ok = xfer(d.byplus, v.boundaryYPlus, B) && xfer(d.buplus, v.boundaryUPlus, B) &&
+              xfer(d.bq, v.boundaryQ, B) && xfer(d.bh, v.boundaryH, B) && xfer(d.bt, v.boundaryT, B) &&
+              xfer3(d.bshear, v.boundaryShear, B) && xfer(d.rylog, v.regionYLog, R) &&
+              xfer3(d.rforcet, v.regionForceTangent, R);
+    return ok ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+}
+
+int sync_state_any(cupcfd_hip_mesh *h, bool push) {
+    if (h->precision == 64) return sync_state(*h->m64, h->m64->view64, push);
+    return sync_state(*h->m32, h->m32->view32, push);
+}
+
+// Scalar-loop PhiFace source per face for one assignment rule
+// (ScalarKernels.ipp:240-447): OUTLET faces always assign PhiFace; INLET faces
+// assign 0 unless ivar is VarT/VarTE/VarED; WALL faces assign it when the
+// enthalpy wall branch runs on a non-adiabatic region. Entry k is the last
+// assigning face before face k in mesh face order, or -1.
+template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns) {
+    int key = (inlet_assigns ? 1 : 0) | (wall_assigns ? 2 : 0);
+    if (M.sc_src[key]) return M.sc_src[key];
+    const int F = M.d.F;
+    std::vector<int> src(F, -1);
+    int last = -1;
+    for (int f = 0; f < F; f++) {
+        int k = M.face_rank[f];
+        int nb = M.fconn_h[2 * k + 1];
+        if (nb >= 0) {
+            src[k] = last;
+            continue;
+        }
+        int b = ~nb;
+        int t = M.btype_h[b];
+        if (t == CUPCFD_RTYPE_INLET) {
+            src[k] = last;
+            if (inlet_assigns) last = k;
+        } else if (t == CUPCFD_RTYPE_OUTLET) {
+            last = k;
+        } else if (t == CUPCFD_RTYPE_WALL) {
+            if (wall_assigns && !M.radiab_h[M.bregion_h[b]]) last = k;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, (F ? F : 1) * sizeof(int)) != hipSuccess) return nullptr;
+    if (F && hipMemcpy(p, src.data(), F * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    M.sc_src[key] = (int *)p;
+    return M.sc_src[key];
+}
+
+template int *scalar_carry_src<double>(Mesh<double> &, bool, bool);
+template int *scalar_carry_src<float>(Mesh<float> &, bool, bool);
+
+}  // namespace cupcfd_hip
+
+using namespace cupcfd_hip;
+
+extern "C" int cupcfd_hip_mesh_create_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out) {
+    if (!view || !out) return CUPCFD_E_NULL_PTR;
+    *out = nullptr;
+    auto *h = new (std::nothrow) cupcfd_hip_mesh{64, new (std::nothrow) Mesh<double>(), nullptr};
+    if (!h || !h->m64) return CUPCFD_E_ERROR;
+    int rc;
+    try {
+        rc = build(*h->m64, *view, flags);
+    } catch (const std::bad_alloc &) {
+        rc = CUPCFD_E_ERROR;
+    }
+    if (rc == CUPCFD_E_SUCCESS) {
+        h->m64->view64 = *view;
+        rc = sync_state(*h->m64, *view, true);
+    }
+    if (rc != CUPCFD_E_SUCCESS) {
+        delete h->m64;
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_create_f32(const cupcfd_mesh_view_f32 *view, int32_t flags, cupcfd_hip_mesh **out) {
+    if (!view || !out) return CUPCFD_E_NULL_PTR;
+    *out = nullptr;
+    auto *h = new (std::nothrow) cupcfd_hip_mesh{32, nullptr, new (std::nothrow) Mesh<float>()};
+    if (!h || !h->m32) return CUPCFD_E_ERROR;
+    int rc;
+    try {
+        rc = build(*h->m32, *view, flags);
+    } catch (const std::bad_alloc &) {
+        rc = CUPCFD_E_ERROR;
+    }
+    if (rc == CUPCFD_E_SUCCESS) {
+        h->m32->view32 = *view;
+        rc = sync_state(*h->m32, *view, true);
+    }
+    if (rc != CUPCFD_E_SUCCESS) {
+        delete h->m32;
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_destroy(cupcfd_hip_mesh *mesh) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    (void)hipDeviceSynchronize();
+    delete mesh->m64;
+    delete mesh->m32;
+    delete mesh;
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t *nFaces,
+                                      int32_t *nBoundaries, int32_t *nRegions) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    int C, F, B, R;
+    if (mesh->precision == 64) {
+        C = mesh->m64->d.C, F = mesh->m64->d.F, B = mesh->m64->d.B, R = mesh->m64->d.R;
+    } else {
+        C = mesh->m32->d.C, F = mesh->m32->d.F, B = mesh->m32->d.B, R = mesh->m32->d.R;
+    }
+    if (nCells) *nCells = C;
+    if (nFaces) *nFaces = F;
+    if (nBoundaries) *nBoundaries = B;
+    if (nRegions) *nRegions = R;
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order) {
+    if (!mesh || !order) return CUPCFD_E_NULL_PTR;
+    const auto &v = mesh->precision == 64 ? mesh->m64->cell_order : mesh->m32->cell_order;
+    std::copy(v.begin(), v.end(), order);
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order) {
+    if (!mesh || !order) return CUPCFD_E_NULL_PTR;
+    const auto &v = mesh->precision == 64 ? mesh->m64->face_order : mesh->m32->face_order;
+    std::copy(v.begin(), v.end(), order);
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    if (mesh->precision == 64) mesh->m64->d.carry_init = value;
+    else mesh->m32->d.carry_init = (float)value;
+    return CUPCFD_E_SUCCESS;
+}
+
+extern "C" int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    return sync_state_any(mesh, true);
+}
+
+extern "C" int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    return sync_state_any(mesh, false);
+}

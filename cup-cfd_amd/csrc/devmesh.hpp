@@ -1,0 +1,98 @@
+// Device mesh: the MiniSoA view repacked into HBM for the gfx950 kernels.
+//
+// Layout (internal numbering):
+//   cells  : Morton order of centres when CUPCFD_MESH_REORDER is set, else the
+//            mesh's own order.
+//   faces  : interior faces first (positions [0, Fi)), then boundary faces;
+//            each group sorted by (internal owner cell, mesh face id).
+//   fconn  : int2 (c1, nb) per face; nb >= 0 is cell2, nb < 0 is ~boundary id.
+//   fgeoS  : (Sx, Sy, Sz, lambda)     fgeoX : (Xf, rlencos)
+//   fgeoP  : (xpac, area)             fgeoN : (xnac, 0)
+//   cgeo   : (Cc, volume) per cell, plus a separate volume stream.
+//   row/ent: cell→face CSR; ent = (internal face << 1) | role, role 1 when
+//            the cell is the face's cell2 ("in"). Each row is sorted by the
+//            mesh's face id, which is the order the reference's sequential
+//            face loop visits them, so a per-cell gather reproduces the
+//            reference's accumulation order exactly without atomics.
+// Boundary and region arrays keep the mesh's numbering.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "cupcfd_hip.h"
+
+struct cupcfd_hip_mesh;
+
+namespace cupcfd_hip {
+
+template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
+
+template <class T> struct DevMesh {
+    int C, F, Fi, B, R, nnz;
+    const int2 *fconn;
+    const V4<T> *fgeoS, *fgeoX, *fgeoP, *fgeoN;
+    const V4<T> *cgeo;
+    const T *cvol;
+    const int *row, *ent;
+    // boundaries (mesh numbering)
+    const int *bface, *bregion, *bcell;
+    const T *bdist;
+    T *byplus, *buplus, *bq, *bh, *bt, *bshear;
+    // regions
+    const int *rtype;
+    const uint8_t *rstd, *rflux, *radiab;
+    T *rylog, *rforcet;
+    const T *relog, *rdensity, *rturbke, *rturbdiss, *rsplvl, *rden, *rr, *rt, *ruvw;
+    // carried-value sources (typed meshes only; null otherwise):
+    // interior face -> last INLET/OUTLET face before it (UVW uFace..wFace),
+    // SYMP face     -> last interior/INLET/OUTLET face before it (UVW visFace)
+    const int *carry_src;
+    int typed;
+    T carry_init;
+    // cells that own at least one boundary face
+    const int *bcells;
+    int nbc;
+    // boundaries of each region type, ascending id
+    const int *blist[5];
+    int nblist[5];
+    // cells owning OUTLET boundaries, with those boundaries in ascending id
+    const int *ocells, *orow, *oent;
+    int noc;
+};
+
+// Host-side owner of a device mesh.
+template <class T> struct Mesh {
+    DevMesh<T> d{};
+    std::vector<void *> allocs;
+    // view kept for drop-in state mirroring
+    cupcfd_mesh_view_f64 view64{};
+    cupcfd_mesh_view_f32 view32{};
+    std::vector<int32_t> cell_order, face_order;      // internal -> mesh
+    std::vector<int32_t> cell_rank, face_rank;        // mesh -> internal
+    std::vector<int32_t> bregion_h;                   // region per boundary
+    std::vector<uint8_t> radiab_h;                    // regionAdiab per region
+    std::vector<int32_t> fconn_h;                     // 2F, host copy for carry precompute
+    std::vector<int32_t> btype_h;                     // type per boundary
+    int32_t ntype[5] = {0, 0, 0, 0, 0};               // boundary faces per region type
+    // scalar carry sources per (inlet assigns, wall assigns) combination
+    int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    // workspace pool for drop-in staging and multi-sweep scratch
+    std::vector<void *> ws;
+    std::vector<size_t> ws_size;
+    void *ws_get(int slot, size_t bytes);
+    ~Mesh();
+};
+
+template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns);
+int sync_state_any(struct ::cupcfd_hip_mesh *h, bool push);
+
+}  // namespace cupcfd_hip
+
+struct cupcfd_hip_mesh {
+    int precision;  // 64 or 32
+    cupcfd_hip::Mesh<double> *m64;
+    cupcfd_hip::Mesh<float> *m32;
+};

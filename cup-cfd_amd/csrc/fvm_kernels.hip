@@ -1,0 +1,1242 @@
+// gfx950 kernels for CUP-CFD's Dolfyn-derived finite-volume loops
+// (include/fvm/implementation/component/*Kernels.ipp) and the resident-mode
+// C ABI (cupcfd_dev_*).
+//
+// Design
+//   * Face loops that scatter into both cells of a face (gradient, scalar,
+//     UVW) run as per-cell gathers over the cell→face CSR, whose rows are in
+//     the reference's face order: each cell applies its faces' updates in
+//     exactly the sequence the sequential loop does, so results are
+//     bit-identical with no atomics and no colouring. Each face is evaluated
+//     by both of its cells (same inputs, same operations, same bits).
+//   * The mass-flux face loop writes only per-face outputs for interior faces
+//     and runs face-parallel; its boundary faces run per owning cell.
+//   * Loop-carried locals of the reference (PhiFace, uFace/vFace/wFace,
+//     visFace) are recomputed from the face that last assigned them, found by
+//     a per-face index built with the mesh.
+//   * Sums over boundaries (mass boundary loops, force tangents) keep the
+//     reference's sequential order with one wavefront: 64 lanes load, every
+//     lane accumulates the 64 values in order through cross-lane reads.
+//   * Arithmetic mirrors the reference expression by expression; the file is
+//     compiled with -ffp-contract=off so no multiply-add is fused.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "devmesh.hpp"
+
+#pragma clang fp contract(off)
+
+namespace cupcfd_hip {
+
+constexpr int kBlock = 256;
+
+template <class T> struct v3 { T x, y, z; };
+
+template <class T> __device__ __forceinline__ v3<T> ld3(const T *__restrict__ p, int i) {
+    return v3<T>{p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]};
+}
+template <class T> __device__ __forceinline__ void st3(T *__restrict__ p, int i, v3<T> v) {
+    p[3 * (size_t)i] = v.x;
+    p[3 * (size_t)i + 1] = v.y;
+    p[3 * (size_t)i + 2] = v.z;
+}
+template <class T> __device__ __forceinline__ v3<T> xyz(const V4<T> &a) { return v3<T>{a.x, a.y, a.z}; }
+
+// EuclideanVector::dotProduct (EuclideanVector.cpp:83-90): from 0.0, in order
+template <class T> __device__ __forceinline__ T dot3(v3<T> a, v3<T> b) {
+    T d = 0.0;
+    d += (a.x * b.x);
+    d += (a.y * b.y);
+    d += (a.z * b.z);
+    return d;
+}
+__device__ __forceinline__ double sqrtw(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrtw(float x) { return sqrtf(x); }
+// EuclideanVector::length (EuclideanVector.cpp:93-99): a double
+template <class T> __device__ __forceinline__ double length3(v3<T> a) {
+    T d = dot3(a, a);
+    double l = sqrtw(d);
+    return l;
+}
+// EuclideanVector::normalise (EuclideanVector.cpp:102-110)
+template <class T> __device__ __forceinline__ v3<T> normalise3(v3<T> v) {
+    T s = (T)length3(v);
+    if (s != (T)0) {
+        s = (T)(1.0) / s;
+        v.x *= s;
+        v.y *= s;
+        v.z *= s;
+    }
+    return v;
+}
+// (a * fa) + (b * fb) componentwise
+template <class T> __device__ __forceinline__ v3<T> lerp3(v3<T> a, T fa, v3<T> b, T fb) {
+    return v3<T>{a.x * fa + b.x * fb, a.y * fa + b.y * fb, a.z * fa + b.z * fb};
+}
+template <class T> __device__ __forceinline__ v3<T> sub3(v3<T> a, v3<T> b) {
+    return v3<T>{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+// std::min / std::max
+template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
+template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// Sequential-order wave reduction: returns acc + v[0] + v[1] + ... + v[n-1]
+// evaluated left to right, as the reference's boundary loops do. Call with
+// one wavefront (64 lanes); every lane returns the same value.
+// ---------------------------------------------------------------------------
+template <class T, class Get> __device__ T wave_seq_sum(T acc, int n, Get get) {
+    const int lane = threadIdx.x;
+    for (int base = 0; base < n; base += 64) {
+        T v = (base + lane < n) ? get(base + lane) : T(0);
+        const int cnt = min(64, n - base);
+        for (int j = 0; j < cnt; j++) acc = acc + __shfl(v, j, 64);
+    }
+    return acc;
+}
+
+// ===========================================================================
+// a1  GradientPhiGaussDolfyn — GradientKernels.ipp:38-136
+// One thread per cell. SWEEP0: dPhidxo is still the zero field of the first
+// sweep, so the face-centre correction dot(0, Xf - xac) adds +0 and is
+// skipped together with its centre and gradient loads.
+// ===========================================================================
+template <class T, bool SWEEP0>
+__global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, const T *__restrict__ phi,
+                                                     const T *__restrict__ phiB, const T *__restrict__ grado,
+                                                     T *__restrict__ out1, T *__restrict__ out2) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T gx = 0, gy = 0, gz = 0;
+    const int e1 = m.row[c + 1];
+    for (int e = m.row[c]; e < e1; e++) {
+        const int f = m.ent[e] >> 1;
+        const int2 cn = m.fconn[f];
+        const V4<T> s = m.fgeoS[f];
+        T phiF;
+        if (cn.y >= 0) {
+            const int ip = cn.x, in = cn.y;
+            const T facn = s.w;
+            const T facp = 1.0 - facn;
+            phiF = (phi[in] * facn) + (phi[ip] * facp);
+            if (!SWEEP0) {
+                const v3<T> xac = lerp3(xyz(m.cgeo[in]), facn, xyz(m.cgeo[ip]), facp);
+                const v3<T> g = lerp3(ld3(grado, in), facn, ld3(grado, ip), facp);
+                phiF += dot3(g, sub3(xyz(m.fgeoX[f]), xac));
+            }
+        } else {
+            phiF = phiB[~cn.y];
+        }
+        gx += phiF * s.x;
+        gy += phiF * s.y;
+        gz += phiF * s.z;
+    }
+    const T vol = m.cvol[c];
+    const T fact = 1.0 / vol;
+    const v3<T> r{gx * fact, gy * fact, gz * fact};
+    st3(out1, c, r);
+    if (out2) st3(out2, c, r);
+}
+
+// ===========================================================================
+// a2  FluxMassDolfynFaceLoop — MassKernels.ipp:36-309
+// ===========================================================================
+template <class T> struct MassArgs {
+    const T *dudx, *dvdx, *dwdx, *dpdx, *den, *u, *v, *w, *p, *ar;
+    T *denB, *massFlux, *su, *rface;
+    T small;
+    int solveTurbEnergy, solveTurbDiss, solveVisc, solveEnthalpy;
+    const T *teCell, *edCell, *viseffCell, *tCell;
+    T *teB, *edB, *viseffB, *tB;
+};
+
+template <class T> __device__ void mass_interior(const DevMesh<T> &m, const MassArgs<T> &a, int f) {
+    const int2 cn = m.fconn[f];
+    const int ip = cn.x, in = cn.y;
+    const V4<T> s = m.fgeoS[f];
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    const v3<T> dudxac = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
+    const v3<T> dvdxac = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
+    const v3<T> dwdxac = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
+    const T denf = a.den[in] * facn + a.den[ip] * facp;
+    const V4<T> gin = m.cgeo[in], gip = m.cgeo[ip];
+    const v3<T> cin = xyz(gin), cip = xyz(gip);
+    const v3<T> xac = lerp3(cin, facn, cip, facp);
+    const V4<T> xf = m.fgeoX[f];
+    const v3<T> delta = sub3(xyz(xf), xac);
+    const T uFace = a.u[in] * facn + a.u[ip] * facp + dot3(dudxac, delta);
+    const T vFace = a.v[in] * facn + a.v[ip] * facp + dot3(dvdxac, delta);
+    const T wFace = a.w[in] * facn + a.w[ip] * facp + dot3(dwdxac, delta);
+    const T mf = denf * (uFace * s.x + vFace * s.y + wFace * s.z);
+    const v3<T> xnorm = normalise3(v3<T>{s.x, s.y, s.z});
+    const V4<T> pa = m.fgeoP[f];
+    const v3<T> xpac = xyz(pa);
+    const v3<T> xnac = xyz(m.fgeoN[f]);
+    const v3<T> gp_ip = ld3(a.dpdx, ip);
+    const T pip = a.p[ip] + dot3(gp_ip, sub3(xpac, cip));
+    const T pin = a.p[in] + dot3(gp_ip, sub3(xpac, cin));  // dpdx[ip] (MassKernels.ipp:169)
+    const v3<T> xpn = sub3(xnac, xpac);
+    const v3<T> xpn2 = sub3(cin, cip);
+    const T apv1 = a.den[ip] * a.ar[ip];
+    const T apv2 = a.den[in] * a.ar[in];
+    T apv = apv2 * facn + apv1 * facp;
+    const T factv = gin.w * facn + gip.w * facp;
+    apv *= pa.w * factv / dot3(xpn2, xnorm);
+    const v3<T> gp_in = ld3(a.dpdx, in);
+    const T dpx = (gp_in.x * facn + gp_ip.x * facp) * xpn.x;
+    const T dpy = (gp_in.y * facn + gp_ip.y * facp) * xpn.y;
+    const T dpz = (gp_in.z * facn + gp_ip.z * facp) * xpn.z;
+    const T fact = apv;
+    a.rface[2 * (size_t)f] = -fact;
+    a.rface[2 * (size_t)f + 1] = -fact;
+    a.massFlux[f] = mf - fact * ((pin - pip) - dpx - dpy - dpz);
+}
+
+// boundary faces of one owning cell, in face order (MassKernels.ipp:197-305)
+template <class T> __device__ void mass_boundary_cell(const DevMesh<T> &m, const MassArgs<T> &a, int c) {
+    const int e1 = m.row[c + 1];
+    bool touched = false;
+    T su = 0;
+    for (int e = m.row[c]; e < e1; e++) {
+        const int f = m.ent[e] >> 1;
+        const int2 cn = m.fconn[f];
+        if (cn.y >= 0) continue;
+        const int ib = ~cn.y;
+        const int ir = m.bregion[ib];
+        const int it = m.rtype[ir];
+        const int ip = cn.x;
+        const V4<T> s = m.fgeoS[f];
+        const v3<T> S{s.x, s.y, s.z};
+        if (it == CUPCFD_RTYPE_INLET) {
+            const v3<T> uIn = ld3(m.ruvw, ir);
+            const T dens = m.rden[ir];
+            const T mf = dens * dot3(uIn, S);
+            a.massFlux[f] = mf;
+            if (!touched) {
+                su = a.su[ip];
+                touched = true;
+            }
+            su = su - mf;
+        } else if (it == CUPCFD_RTYPE_OUTLET) {
+            const T denf = a.den[ip];
+            a.denB[ib] = denf;
+            T mf = denf * (a.u[ip] * s.x + a.v[ip] * s.y + a.w[ip] * s.z);
+            if (mf < 0.0) {
+                mf = a.small;
+                if (a.solveTurbEnergy) a.teB[ib] = a.teCell[ip];
+                if (a.solveTurbDiss) a.edB[ib] = a.edCell[ip];
+                if (a.solveVisc) a.viseffB[ib] = a.viseffCell[ip];
+                if (a.solveEnthalpy) a.tB[ib] = a.tCell[ip];
+            }
+            a.massFlux[f] = mf;
+        } else if (it == CUPCFD_RTYPE_SYMP || it == CUPCFD_RTYPE_WALL) {
+            a.massFlux[f] = 0.0;
+        }
+    }
+    if (touched) a.su[c] = su;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, int nblk_interior) {
+    if ((int)blockIdx.x < nblk_interior) {
+        const int f = blockIdx.x * kBlock + threadIdx.x;
+        if (f < m.Fi) mass_interior(m, a, f);
+    } else {
+        const int k = (blockIdx.x - nblk_interior) * kBlock + threadIdx.x;
+        if (k < m.nbc) mass_boundary_cell(m, a, m.bcells[k]);
+    }
+}
+
+// a3 FluxMassDolfynBoundaryLoop1 (MassKernels.ipp:311-336)
+template <class T> __global__ void k_mass_bl1(DevMesh<T> m, const T *__restrict__ mf, T *flowin) {
+    const int *lst = m.blist[CUPCFD_RTYPE_INLET];
+    T s = wave_seq_sum<T>(T(0.0), m.nblist[CUPCFD_RTYPE_INLET],
+                          [&](int i) { return mf[m.bface[lst[i]]]; });
+    if (threadIdx.x == 0) *flowin = s;
+}
+
+// a4 FluxMassDolfynBoundaryLoop2 (MassKernels.ipp:338-368)
+template <class T> __global__ void k_mass_bl2(DevMesh<T> m, const T *__restrict__ mf, T *flowRegion, T *flowout) {
+    const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
+    const int n = m.nblist[CUPCFD_RTYPE_OUTLET];
+    const int lane = threadIdx.x;
+    T s = wave_seq_sum<T>(T(0.0), n, [&](int i) { return mf[m.bface[lst[i]]]; });
+    if (lane == 0) *flowout = s;
+    // flowRegion[ir] += mf in boundary order; lane r owns region rbase + r
+    for (int rbase = 0; rbase < m.R; rbase += 64) {
+        const int myr = rbase + lane;
+        T acc = myr < m.R ? flowRegion[myr] : T(0);
+        for (int base = 0; base < n; base += 64) {
+            T v = 0;
+            int r = -1;
+            if (base + lane < n) {
+                const int b = lst[base + lane];
+                v = mf[m.bface[b]];
+                r = m.bregion[b];
+            }
+            const int cnt = min(64, n - base);
+            for (int j = 0; j < cnt; j++) {
+                const T vj = __shfl(v, j, 64);
+                const int rj = __shfl(r, j, 64);
+                if (rj == myr) acc = acc + vj;
+            }
+        }
+        if (myr < m.R) flowRegion[myr] = acc;
+    }
+}
+
+// a5 FluxMassDolfynBoundaryLoop3 (MassKernels.ipp:371-393)
+template <class T> __global__ void k_mass_bl3(DevMesh<T> m, T flowin, T *ratearea) {
+    const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
+    T areaout = wave_seq_sum<T>(T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET],
+                                [&](int i) { return m.fgeoP[m.bface[lst[i]]].w; });
+    if (threadIdx.x == 0) *ratearea = -flowin / areaout;
+}
+
+// a6 FluxMassDolfynBoundaryLoop4 (MassKernels.ipp:395-454): per-boundary part
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mass_bl4_faces(DevMesh<T> m, T *massFlux, T *uB, T *vB, T *wB,
+                                                           const T *denB, T ratearea) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.nblist[CUPCFD_RTYPE_OUTLET]) return;
+    const int ib = m.blist[CUPCFD_RTYPE_OUTLET][k];
+    const int ir = m.bregion[ib];
+    const int i = m.bface[ib];
+    const T split = m.rsplvl[ir];
+    const T area = m.fgeoP[i].w;
+    const T mf = ratearea * area * split;
+    massFlux[i] = mf;
+    const T faceFlux = mf / denB[ib] / area;
+    const V4<T> s = m.fgeoS[i];
+    const v3<T> xnorm = normalise3(v3<T>{s.x, s.y, s.z});
+    uB[ib] = faceFlux * xnorm.x;
+    vB[ib] = faceFlux * xnorm.y;
+    wB[ib] = faceFlux * xnorm.z;
+}
+
+// sum of massFlux over OUTLET boundaries in boundary order, starting from
+// *acc_in (0 for loop 4, the caller's flowout2 for loop 5)
+template <class T> __global__ void k_outlet_sum(DevMesh<T> m, const T *__restrict__ mf, T *acc, int from_zero) {
+    const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
+    T a0 = from_zero ? T(0.0) : *acc;
+    T s = wave_seq_sum<T>(a0, m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i) { return mf[m.bface[lst[i]]]; });
+    if (threadIdx.x == 0) *acc = s;
+}
+
+// a7 FluxMassDolfynBoundaryLoop5 (MassKernels.ipp:457-521): per OUTLET-owning
+// cell, its OUTLET boundaries in ascending id
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mass_bl5_cells(DevMesh<T> m, T *massFlux, T *su, T *uB, T *vB, T *wB,
+                                                           T fact, int solveU, int solveV, int solveW,
+                                                           const T *flowFact) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.noc) return;
+    const int c = m.ocells[k];
+    T s = su[c];
+    for (int e = m.orow[k]; e < m.orow[k + 1]; e++) {
+        const int ib = m.oent[e];
+        const int ir = m.bregion[ib];
+        const int i = m.bface[ib];
+        const T mf = massFlux[i] * flowFact[ir];
+        massFlux[i] = mf;
+        if (solveU) uB[ib] = uB[ib] * fact;
+        if (solveV) vB[ib] = vB[ib] * fact;
+        if (solveW) wB[ib] = wB[ib] * fact;
+        s = s - mf;
+    }
+    su[c] = s;
+}
+
+// a8 FluxMassDolfynRegionLoop (MassKernels.ipp:523-554)
+template <class T> __global__ void k_mass_region(DevMesh<T> m, T *flowFact, const T *flowRegion, T flowIn) {
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir >= m.R) return;
+    if (m.rtype[ir] == CUPCFD_RTYPE_OUTLET) {
+        const T split = m.rsplvl[ir];
+        flowFact[ir] = -(split * flowIn) / flowRegion[ir];
+    } else {
+        flowFact[ir] = 0.0;
+    }
+}
+
+// ===========================================================================
+// a9  FluxScalarDolfynFaceLoop — ScalarKernels.ipp:36-463
+// ===========================================================================
+template <class T> struct ScalarArgs {
+    const T *phi, *visEff, *den, *massFlux, *te, *cpB, *visEffB, *dPhidx;
+    T *phiB, *au, *su, *rface;
+    const int *src;  // PhiFace source per face (typed meshes), else null
+    int solveTurb, solveEnthalpy, ivar, varT, varTE, varED, inletAssigns, wallActive;
+    T visLam, sigmaT, prandtl, sigmaK, sigmaE, sigmaS, schmidt, gammaBlend, small;
+    T slres, cmu25;  // wall-function constants, evaluated on the host
+};
+
+template <class T> __device__ __forceinline__ T scalar_visac(const ScalarArgs<T> &a, T Visac) {
+    if (a.solveTurb) {
+        Visac = Visac - a.visLam;
+        if (a.ivar == a.varT) Visac = (a.visLam + Visac / a.sigmaT) / a.prandtl;
+        else if (a.ivar == a.varTE) Visac = a.visLam + Visac / a.sigmaK;
+        else if (a.ivar == a.varED) Visac = a.visLam + Visac / a.sigmaE;
+        else Visac = (a.visLam + Visac / a.sigmaS) / a.schmidt;
+    } else {
+        if (a.ivar == a.varT) Visac = Visac / a.prandtl;
+        else Visac = Visac / a.schmidt;
+    }
+    return Visac;
+}
+
+// Enthalpy wall branch on a non-adiabatic region (ScalarKernels.ipp:370-445):
+// returns PhiFace and the heat-transfer coefficient Hcoef and VisFace.
+template <class T>
+__device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, int ib, int ir, int ip,
+                                 T &Hcoef, T &VisFace) {
+    T PhiFlux = 0, PhiFace = 0;
+    if (m.rflux[ir]) PhiFlux = m.rt[ir];
+    else PhiFace = m.rt[ir];
+    const T dn = m.bdist[ib];
+    const T Resist = m.rr[ir];
+    const T area = m.fgeoP[f].w;
+    const T cp = a.cpB[ib];
+    if (!a.solveTurb) {
+        VisFace = a.visLam / a.prandtl / dn;
+        Hcoef = 1.0 / (1.0 / VisFace + Resist * cp) * area;
+    } else {
+        const T Tplus = a.sigmaT * (m.buplus[ib] + a.slres);
+        const T utau = a.cmu25 * sqrt((double)a.te[ip]);
+        if (m.byplus[ib] < m.rylog[ir]) {
+            VisFace = a.visLam / a.prandtl / dn;
+            Hcoef = 1.0 / (1.0 / VisFace + Resist * cp) * area;
+        } else {
+            VisFace = a.den[ip] * utau / (Tplus + a.small);
+            Hcoef = 1.0 / (1.0 / VisFace + Resist * cp) * area;
+        }
+    }
+    if (m.rflux[ir]) PhiFace = a.phi[ip] + PhiFlux / (Hcoef * cp / area);
+    return PhiFace;
+}
+
+// PhiFace as carried into face f: the value the last assigning face left
+template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarArgs<T> &a, int f) {
+    if (!a.src) return m.carry_init;
+    const int g = a.src[f];
+    if (g < 0) return m.carry_init;
+    const int ib = ~m.fconn[g].y;
+    const int ip = m.fconn[g].x;
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    if (it == CUPCFD_RTYPE_INLET) return (T)(0);
+    if (it == CUPCFD_RTYPE_OUTLET) {
+        const v3<T> Xpn = sub3(xyz(m.fgeoX[g]), xyz(m.cgeo[ip]));
+        return a.phi[ip] + dot3(ld3(a.dPhidx, ip), Xpn);
+    }
+    T H, V;
+    return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scalar(DevMesh<T> m, ScalarArgs<T> a) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T su = a.su[c];
+    T au = 0;
+    bool au_loaded = false;
+    const int e1 = m.row[c + 1];
+    for (int e = m.row[c]; e < e1; e++) {
+        const int ent = m.ent[e];
+        const int f = ent >> 1;
+        const int2 cn = m.fconn[f];
+        const V4<T> s = m.fgeoS[f];
+        const v3<T> S{s.x, s.y, s.z};
+        const T mf = a.massFlux[f];
+        if (cn.y >= 0) {
+            const int ip = cn.x, in = cn.y;
+            const T facn = s.w;
+            const T facp = 1.0 - facn;
+            T Visac = a.visEff[in] * facn + a.visEff[ip] * facp;
+            Visac = scalar_visac(a, Visac);
+            const v3<T> g = lerp3(ld3(a.dPhidx, in), facn, ld3(a.dPhidx, ip), facp);
+            const v3<T> Xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
+            const T VisFace = Visac * m.fgeoX[f].w;
+            const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
+            const T fce = mf * PhiFace;
+            const T fde1 = Visac * dot3(g, S);
+            const T fci = fmin((double)mf, 0.0) * (double)a.phi[in] + fmax((double)mf, 0.0) * (double)a.phi[ip];
+            const T fdi = VisFace * dot3(g, Xpn);
+            const T blend = a.gammaBlend * (fce - fci);
+            if ((ent & 1) == 0) {
+                su = su - blend + fde1 - fdi;
+                a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
+                a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
+            } else {
+                su = su + blend - fde1 + fdi;
+            }
+        } else {
+            const int ip = cn.x;
+            const int ib = ~cn.y;
+            const int ir = m.bregion[ib];
+            const int it = m.rtype[ir];
+            if (it == CUPCFD_RTYPE_INLET) {
+                const v3<T> g = ld3(a.dPhidx, ip);
+                const T PhiFace = a.inletAssigns ? (T)(0) : scalar_carry(m, a, f);
+                T Visac = scalar_visac(a, a.visEffB[ib]);
+                const V4<T> xf = m.fgeoX[f];
+                const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
+                const T VisFace = Visac * xf.w;
+                const T fde = Visac * dot3(g, S);
+                const T fdi = VisFace * dot3(g, Xpn);
+                const T fF = -VisFace + fmin((double)mf, 0.0);
+                if (!au_loaded) {
+                    au = a.au[c];
+                    au_loaded = true;
+                }
+                au = au - fF;
+                su = su + fde - fdi - fF * PhiFace;
+                a.phiB[ib] = PhiFace;
+            } else if (it == CUPCFD_RTYPE_OUTLET) {
+                const v3<T> g = ld3(a.dPhidx, ip);
+                T Visac = scalar_visac(a, a.visEff[ip]);
+                const V4<T> xf = m.fgeoX[f];
+                const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
+                const T PhiFace = a.phi[ip] + dot3(g, Xpn);
+                const T VisFace = Visac * xf.w;
+                const T fde = Visac * dot3(g, S);
+                const T fdi = VisFace * dot3(g, Xpn);
+                su = su + fde - fdi;
+                a.phiB[ib] = PhiFace;
+            } else if (it == CUPCFD_RTYPE_SYMP) {
+                const v3<T> ds = sub3(xyz(m.fgeoX[f]), xyz(m.cgeo[ip]));
+                a.phiB[ib] = a.phi[ip] + dot3(ld3(a.dPhidx, ip), ds);
+            } else if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
+                if (m.radiab[ir]) {
+                    a.phiB[ib] = a.phi[ip];
+                    m.bq[ib] = 0.0;
+                } else {
+                    T Hcoef, VisFace;
+                    const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
+                    const T area = m.fgeoP[f].w;
+                    const T cp = a.cpB[ib];
+                    a.phiB[ib] = PhiFace;
+                    if (!au_loaded) {
+                        au = a.au[c];
+                        au_loaded = true;
+                    }
+                    au = au + Hcoef;
+                    su = su + Hcoef * PhiFace;
+                    const T Tdif = (PhiFace - a.phi[ip]);
+                    const T h1 = Hcoef * cp / area;
+                    m.bq[ib] = h1 * Tdif;
+                    m.bh[ib] = VisFace * cp;
+                    m.bt[ib] = PhiFace;
+                }
+            }
+        }
+    }
+    a.su[c] = su;
+    if (au_loaded) a.au[c] = au;
+}
+
+// ===========================================================================
+// a10 FluxUVWDolfynFaceLoop1 — UVWKernels.ipp:38-515
+// ===========================================================================
+template <class T> struct UvwArgs {
+    const T *u, *v, *w, *visEff, *visEffB, *dudx, *dvdx, *dwdx;
+    T *uB, *vB, *wB, *massFlux, *rface, *su, *sv, *sw, *au, *av, *aw;
+    T gammaBlend, small;
+};
+
+// uFace/vFace/wFace carried into interior face f (set by INLET and OUTLET faces)
+template <class T> __device__ v3<T> uvw_carry(const DevMesh<T> &m, const UvwArgs<T> &a, int f) {
+    const T ci = m.carry_init;
+    if (!m.typed) return v3<T>{ci, ci, ci};
+    const int g = m.carry_src[f];
+    if (g < 0) return v3<T>{ci, ci, ci};
+    const int2 cg = m.fconn[g];
+    const int ir = m.bregion[~cg.y];
+    if (m.rtype[ir] == CUPCFD_RTYPE_INLET) return ld3(m.ruvw, ir);
+    return v3<T>{a.u[cg.x], a.v[cg.x], a.w[cg.x]};
+}
+
+// visFace carried into SYMP face f (set by interior, INLET and OUTLET faces)
+template <class T> __device__ T uvw_visface_carry(const DevMesh<T> &m, const UvwArgs<T> &a, int f) {
+    const int g = m.carry_src[f];
+    if (g < 0) return m.carry_init;
+    const int2 cg = m.fconn[g];
+    const T rl = m.fgeoX[g].w;
+    if (cg.y >= 0) {
+        const T facn = m.fgeoS[g].w;
+        const T facp = 1.0 - facn;
+        const T visac = a.visEff[cg.y] * facn + a.visEff[cg.x] * facp;
+        return visac * rl;
+    }
+    const int ib = ~cg.y;
+    const int ir = m.bregion[ib];
+    const T visac = m.rtype[ir] == CUPCFD_RTYPE_INLET ? a.visEffB[ib] : a.visEff[cg.x];
+    return visac * rl;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_uvw(DevMesh<T> m, UvwArgs<T> a) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T su = a.su[c], sv = a.sv[c], sw = a.sw[c];
+    T au = 0, av = 0, aw = 0;
+    bool a_loaded = false;
+    const int e1 = m.row[c + 1];
+    for (int e = m.row[c]; e < e1; e++) {
+        const int ent = m.ent[e];
+        const int f = ent >> 1;
+        const int2 cn = m.fconn[f];
+        const V4<T> s = m.fgeoS[f];
+        const T sx = s.x, sy = s.y, sz = s.z;
+        if (cn.y >= 0) {
+            const int ip = cn.x, in = cn.y;
+            const T facn = s.w;
+            const T facp = 1.0 - facn;
+            const v3<T> du = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
+            const v3<T> dv = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
+            const v3<T> dw = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
+            const T visac = a.visEff[in] * facn + a.visEff[ip] * facp;
+            const v3<T> xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
+            const T visFace = visac * m.fgeoX[f].w;
+            const T mf = a.massFlux[f];
+            const v3<T> uf = uvw_carry(m, a, f);
+            const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
+            const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
+            const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
+            const T fwde1 = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
+            const T fude = visac * fude1, fvde = visac * fvde1, fwde = visac * fwde1;
+            const T fmn = smin(mf, (T)(0.0)), fmx = smax(mf, (T)(0.0));
+            const T fuci = fmn * a.u[in] + fmx * a.u[ip];
+            const T fvci = fmn * a.v[in] + fmx * a.v[ip];
+            const T fwci = fmn * a.w[in] + fmx * a.w[ip];
+            const T fudi = visFace * dot3(du, xpn);
+            const T fvdi = visFace * dot3(dv, xpn);
+            const T fwdi = visFace * dot3(dw, xpn);
+            const T blendU = a.gammaBlend * (fuce - fuci);
+            const T blendV = a.gammaBlend * (fvce - fvci);
+            const T blendW = a.gammaBlend * (fwce - fwci);
+            if ((ent & 1) == 0) {
+                a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
+                a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
+                su = su - blendU + fude - fudi;
+                sv = sv - blendV + fvde - fvdi;
+                sw = sw - blendW + fwde - fwdi;
+            } else {
+                su = su + blendU - fude + fudi;
+                sv = sv + blendV - fvde + fvdi;
+                sw = sw + blendW - fwde + fwdi;
+            }
+            continue;
+        }
+        const int ip = cn.x;
+        const int ib = ~cn.y;
+        const int ir = m.bregion[ib];
+        const int it = m.rtype[ir];
+        if (it == CUPCFD_RTYPE_DEFAULT) continue;
+        if (!a_loaded) {
+            au = a.au[c];
+            av = a.av[c];
+            aw = a.aw[c];
+            a_loaded = true;
+        }
+        const V4<T> xf = m.fgeoX[f];
+        const v3<T> cp = xyz(m.cgeo[ip]);
+        if (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) {
+            const v3<T> du = ld3(a.dudx, ip), dv = ld3(a.dvdx, ip), dw = ld3(a.dwdx, ip);
+            v3<T> uf;
+            T visac;
+            if (it == CUPCFD_RTYPE_INLET) {
+                uf = ld3(m.ruvw, ir);
+                visac = a.visEffB[ib];
+            } else {
+                uf = v3<T>{a.u[ip], a.v[ip], a.w[ip]};
+                visac = a.visEff[ip];
+            }
+            const v3<T> xpn = sub3(xyz(xf), cp);
+            const T visFace = visac * xf.w;
+            T fude = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
+            T fvde = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
+            T fwde = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
+            fude = visac * fude;
+            fvde = visac * fvde;
+            fwde = visac * fwde;
+            const T fudi = visFace * dot3(du, xpn);
+            const T fvdi = visFace * dot3(dv, xpn);
+            const T fwdi = visFace * dot3(dw, xpn);
+            T mf = a.massFlux[f];
+            if (it == CUPCFD_RTYPE_OUTLET && mf < 0.0) {
+                mf = a.small;
+                a.massFlux[f] = mf;
+            }
+            const T fF = -visFace + smin(mf, (T)(0.0));
+            au = au - fF;
+            su = su - fF * uf.x + fude - fudi;
+            av = av - fF;
+            sv = sv - fF * uf.y + fvde - fvdi;
+            aw = aw - fF;
+            sw = sw - fF * uf.z + fwde - fwdi;
+            a.uB[ib] = uf.x;
+            a.vB[ib] = uf.y;
+            a.wB[ib] = uf.z;
+        } else if (it == CUPCFD_RTYPE_SYMP) {
+            const v3<T> xpn = sub3(xyz(xf), cp);
+            const T visac = a.visEff[ip];
+            const T du = dot3(ld3(a.dudx, ip), xpn);
+            const T dv = dot3(ld3(a.dvdx, ip), xpn);
+            const T dw = dot3(ld3(a.dwdx, ip), xpn);
+            const v3<T> us{a.u[ip] + du, a.v[ip] + dv, a.w[ip] + dw};
+            const v3<T> xn = normalise3(v3<T>{(T)(0.0 - sx), (T)(0.0 - sy), (T)(0.0 - sz)});
+            const T dp = dot3(us, xn);
+            const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
+            const T dn = m.bdist[ib];
+            const T twoVis = (T)(2.0 * visac);
+            const T area = m.fgeoP[f].w;
+            const v3<T> force{((twoVis * un.x) / dn) * area, ((twoVis * un.y) / dn) * area,
+                              ((twoVis * un.z) / dn) * area};
+            const T visFace = uvw_visface_carry(m, a, f);
+            au = au + visFace;
+            av = av + visFace;
+            aw = aw + visFace;
+            su = su + visFace * a.u[ip] - force.x;
+            sv = sv + visFace * a.v[ip] - force.y;
+            sw = sw + visFace * a.w[ip] - force.z;
+            a.uB[ib] = us.x;
+            a.vB[ib] = us.y;
+            a.wB[ib] = us.z;
+        } else {  // WALL
+            const v3<T> uw = ld3(m.ruvw, ir);
+            const T visac = a.visEffB[ib];
+            const v3<T> xpn = sub3(xyz(xf), cp);
+            const T xpn_length = (T)length3(xpn);
+            const T area = m.fgeoP[f].w;
+            const T coef = visac * area / xpn_length;
+            const v3<T> xn = normalise3(v3<T>{sx, sy, sz});
+            v3<T> up{a.u[ip], a.v[ip], a.w[ip]};
+            up = sub3(up, uw);
+            const T dp = dot3(up, xn);
+            const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
+            const v3<T> ut = sub3(up, un);
+            const T uvel = fabs((double)ut.x) + fabs((double)ut.y) + fabs((double)ut.z);
+            v3<T> force{0.0, 0.0, 0.0};
+            if (uvel > a.small) {
+                const T dn = m.bdist[ib];
+                force = v3<T>{((visac * ut.x) / dn) * area, ((visac * ut.y) / dn) * area,
+                              ((visac * ut.z) / dn) * area};
+            }
+            st3(m.bshear, ib, force);
+            au = au + coef;
+            av = av + coef;
+            aw = aw + coef;
+            su = su + coef * a.u[ip] - force.x;
+            sv = sv + coef * a.v[ip] - force.y;
+            sw = sw + coef * a.w[ip] - force.z;
+            a.uB[ib] = uw.x;
+            a.vB[ib] = uw.y;
+            a.wB[ib] = uw.z;
+        }
+    }
+    a.su[c] = su;
+    a.sv[c] = sv;
+    a.sw[c] = sw;
+    if (a_loaded) {
+        a.au[c] = au;
+        a.av[c] = av;
+        a.aw[c] = aw;
+    }
+}
+
+// a11 FluxUVWDolfynRegionLoop1 (UVWKernels.ipp:517-524)
+template <class T> __global__ void k_uvw_region1(DevMesh<T> m) {
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir < m.R) st3(m.rforcet, ir, v3<T>{(T)0, (T)0, (T)0});
+}
+
+// a12 FluxUVWDolfynBndsLoop1 (UVWKernels.ipp:527-544): WALL shear summed per
+// region in boundary order; lane r owns region rbase + r
+template <class T> __global__ void k_uvw_bnds1(DevMesh<T> m) {
+    const int *lst = m.blist[CUPCFD_RTYPE_WALL];
+    const int n = m.nblist[CUPCFD_RTYPE_WALL];
+    const int lane = threadIdx.x;
+    for (int rbase = 0; rbase < m.R; rbase += 64) {
+        const int myr = rbase + lane;
+        v3<T> acc = myr < m.R ? ld3((const T *)m.rforcet, myr) : v3<T>{0, 0, 0};
+        for (int base = 0; base < n; base += 64) {
+            v3<T> v{0, 0, 0};
+            int r = -1;
+            if (base + lane < n) {
+                const int b = lst[base + lane];
+                v = ld3((const T *)m.bshear, b);
+                r = m.bregion[b];
+            }
+            const int cnt = min(64, n - base);
+            for (int j = 0; j < cnt; j++) {
+                const T x = __shfl(v.x, j, 64), y = __shfl(v.y, j, 64), z = __shfl(v.z, j, 64);
+                const int rj = __shfl(r, j, 64);
+                if (rj == myr) acc = v3<T>{acc.x + x, acc.y + y, acc.z + z};
+            }
+        }
+        if (myr < m.R) st3(m.rforcet, myr, acc);
+    }
+}
+
+// ===========================================================================
+// a13-a16 viscosity loops — ViscosityKernels.ipp
+// ===========================================================================
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_visc_cell1(int C, T small, T cmu, T visURF, T visLam,
+                                                       const T *__restrict__ TE, const T *__restrict__ ED,
+                                                       const T *__restrict__ den, T *__restrict__ visEff) {
+    const int ip = blockIdx.x * kBlock + threadIdx.x;
+    if (ip >= C) return;
+    const T visOld = visEff[ip];
+    T visNew;
+    if (ED[ip] > small) visNew = visLam + cmu * den[ip] * (TE[ip] * TE[ip]) / ED[ip];
+    else visNew = visLam;
+    visEff[ip] = visOld + visURF * (visNew - visOld);
+}
+
+// yplustmp is carried from region to region, as in the reference
+template <class T> __global__ void k_visc_region(DevMesh<T> m, T kappa) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    T yplus, yplustmp = m.carry_init, elog;
+    for (int ir = 0; ir < m.R; ir++) {
+        if (m.rtype[ir] == CUPCFD_RTYPE_WALL && m.rstd[ir]) {
+            yplus = 11.0;
+            elog = m.relog[ir];
+            int i = 0;
+            while ((fabs((double)(yplus - yplustmp)) > 0.001) && (i < 50)) {
+                yplustmp = yplus;
+                yplus = log((double)(elog * yplustmp)) / kappa;
+                i = i + 1;
+            }
+            m.rylog[ir] = yplus;
+        } else if (m.rtype[ir] == CUPCFD_RTYPE_WALL && !(m.rstd[ir])) {
+            m.rylog[ir] = 0.0;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_visc_boundary(DevMesh<T> m, T cmu, T cmu25, T small, T kappa,
+                                                          T visLam, const T *__restrict__ TE,
+                                                          const T *__restrict__ den,
+                                                          const T *__restrict__ visEffCell, T *visEffB) {
+    const int ib = blockIdx.x * kBlock + threadIdx.x;
+    if (ib >= m.B) return;
+    const int i = m.bface[ib];
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    const int ip = m.fconn[i].x;
+    if (it == CUPCFD_RTYPE_INLET) {
+        visEffB[ib] =
+            visLam + m.rdensity[ir] * cmu * (m.rturbke[ir] * m.rturbke[ir]) / (m.rturbdiss[ir] + small);
+    } else if (it == CUPCFD_RTYPE_OUTLET || it == CUPCFD_RTYPE_SYMP) {
+        visEffB[ib] = visEffCell[ip];
+    } else if (it == CUPCFD_RTYPE_WALL) {
+        const T dist = m.bdist[ib];
+        T turb = TE[ip];
+        const T zero = 0.0;
+        if (turb < zero) turb = zero;
+        const T utau = cmu25 * sqrt((double)turb);
+        T yplus = dist * utau * den[ip] / (visLam + small);
+        m.byplus[ib] = yplus;
+        T uplus;
+        if (m.rstd[ir]) {
+            if (yplus < m.rylog[ir]) {
+                uplus = yplus;
+                if (yplus < small) {
+                    yplus = 0.0;
+                    uplus = small;
+                }
+            } else {
+                T tmp = m.relog[ir] * m.byplus[ib];
+                if (tmp < 1.1) tmp = 1.1;
+                uplus = log((double)tmp) / kappa;
+            }
+            m.buplus[ib] = uplus;
+            visEffB[ib] = smax((T)(1.0), yplus / uplus) * visLam;
+        } else {
+            const T d0 = 0.0, z0 = 0.1;
+            uplus = log((double)((dist - d0) / z0)) / kappa;
+            m.buplus[ib] = uplus;
+            visEffB[ib] = smax((T)(1.0), yplus / uplus) * visLam;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_visc_cell2(int C, int B, T visLam, T *visEffCell, T *visEffB) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < C) visEffCell[k] = smin(visEffCell[k], (T)(1000000.0) * visLam);
+    else if (k < C + B) visEffB[k - C] = smin(visEffB[k - C], (T)(1000000.0) * visLam);
+}
+
+// ===========================================================================
+// layout helpers: mesh order <-> internal order
+// ===========================================================================
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_permute(int n, int ncomp, const int *__restrict__ order,
+                                                    const E *__restrict__ src, E *__restrict__ dst, int to_internal) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const size_t a = (size_t)k * ncomp, b = (size_t)order[k] * ncomp;
+    for (int j = 0; j < ncomp; j++) {
+        if (to_internal) dst[a + j] = src[b + j];
+        else dst[b + j] = src[a + j];
+    }
+}
+
+}  // namespace cupcfd_hip
+
+// ===========================================================================
+// Resident-mode C ABI
+// ===========================================================================
+using namespace cupcfd_hip;
+
+namespace {
+
+inline int blocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+inline int launch_ok() { return hipGetLastError() == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR; }
+
+template <class T> Mesh<T> *get(cupcfd_hip_mesh *h);
+template <> Mesh<double> *get<double>(cupcfd_hip_mesh *h) { return h && h->precision == 64 ? h->m64 : nullptr; }
+template <> Mesh<float> *get<float>(cupcfd_hip_mesh *h) { return h && h->precision == 32 ? h->m32 : nullptr; }
+
+template <class T>
+int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB, T *g, T *go, void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    hipStream_t s = (hipStream_t)stream;
+    if (d.C == 0) return CUPCFD_E_SUCCESS;
+    if (nGradient <= 0) {
+        // the reference zeroes dPhidxo and leaves dPhidx untouched
+        return hipMemsetAsync(go, 0, 3 * (size_t)d.C * sizeof(T), s) == hipSuccess ? CUPCFD_E_SUCCESS
+                                                                                   : CUPCFD_E_ERROR;
+    }
+    // Sweeps before the last write alternately into two scratch fields (a
+    // sweep reads every cell's previous gradient, so it cannot write in
+    // place); the last sweep writes dPhidx and dPhidxo together.
+    T *t[2] = {nullptr, nullptr};
+    if (nGradient > 1) {
+        t[0] = (T *)M->ws_get(42, 3 * (size_t)d.C * sizeof(T));
+        t[1] = (T *)M->ws_get(43, 3 * (size_t)d.C * sizeof(T));
+        if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
+    }
+    const T *prev = nullptr;
+    for (int it = 0; it < nGradient; it++) {
+        const bool last = (it == nGradient - 1);
+        T *dst = last ? g : t[it & 1];
+        T *dst2 = last ? go : nullptr;
+        if (it == 0)
+            hipLaunchKernelGGL((k_gradient<T, true>), dim3(blocks(d.C)), dim3(kBlock), 0, s, d, phi, phiB, prev,
+                               dst, dst2);
+        else
+            hipLaunchKernelGGL((k_gradient<T, false>), dim3(blocks(d.C)), dim3(kBlock), 0, s, d, phi, phiB, prev,
+                               dst, dst2);
+        if (hipGetLastError() != hipSuccess) return CUPCFD_E_ERROR;
+        prev = dst;
+    }
+    return CUPCFD_E_SUCCESS;
+}
+
+template <class T>
+int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, const T *dpdx, const T *den, T *denB,
+             const T *u, const T *v, const T *w, T *massFlux, const T *p, const T *ar, T *su, T *rface, T small,
+             int32_t *icinl, int32_t *icout, int32_t *icsym, int32_t *icwal, int sTE, int sTD, int sV, int sE,
+             const T *teC, T *teB, const T *edC, T *edB, const T *vC, T *vB, const T *tC, T *tB, void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    MassArgs<T> a{dudx, dvdx, dwdx, dpdx, den, u, v, w, p, ar, denB, massFlux, su, rface, small, sTE, sTD, sV, sE,
+                  teC, edC, vC, tC, teB, edB, vB, tB};
+    const int nbi = blocks(d.Fi), nbb = blocks(d.nbc);
+    if (nbi + nbb > 0)
+        hipLaunchKernelGGL(k_mass<T>, dim3(nbi + nbb), dim3(kBlock), 0, (hipStream_t)stream, d, a, nbi);
+    // the face loop's counters count boundary faces by region type
+    if (icinl) *icinl += M->ntype[CUPCFD_RTYPE_INLET];
+    if (icout) *icout += M->ntype[CUPCFD_RTYPE_OUTLET];
+    if (icsym) *icsym += M->ntype[CUPCFD_RTYPE_SYMP];
+    if (icwal) *icwal += M->ntype[CUPCFD_RTYPE_WALL];
+    return launch_ok();
+}
+
+template <class T>
+int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den, const T *MF,
+               const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx, int SolveTurb,
+               int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k, int VarED,
+               T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large, T TMCmu, void *stream) {
+    (void)Large;
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    ScalarArgs<T> a{};
+    a.phi = Phi;
+    a.visEff = VisEff;
+    a.den = Den;
+    a.massFlux = MF;
+    a.te = TE;
+    a.cpB = CpB;
+    a.visEffB = VisEffB;
+    a.dPhidx = dPhidx;
+    a.phiB = PhiB;
+    a.au = Au;
+    a.su = Su;
+    a.rface = RFace;
+    a.solveTurb = SolveTurb;
+    a.solveEnthalpy = SolveEnthalpy;
+    a.ivar = ivar;
+    a.varT = VarT;
+    a.varTE = VarTE;
+    a.varED = VarED;
+    a.inletAssigns = !(ivar == VarT || ivar == VarTE || ivar == VarED);
+    a.wallActive = (SolveEnthalpy && ivar == VarT) ? 1 : 0;
+    a.visLam = vislam;
+    a.sigmaT = Sigma_T;
+    a.prandtl = Prandtl;
+    a.sigmaK = Sigma_k;
+    a.sigmaE = Sigma_e;
+    a.sigmaS = Sigma_s;
+    a.schmidt = Schmidt;
+    a.gammaBlend = GammaBlend;
+    a.small = Small;
+    // SLres and Cmu25 (ScalarKernels.ipp:394-395) are per-call constants
+    a.slres = 9.24 * (pow((Prandtl / Sigma_T), 0.75) - 1.0) * (1.0 + 0.28 * exp(-0.007 * Prandtl / Sigma_T));
+    a.cmu25 = pow(TMCmu, 0.25);
+    a.src = nullptr;
+    if (d.typed) {
+        a.src = scalar_carry_src<T>(*M, a.inletAssigns != 0, a.wallActive != 0);
+        if (!a.src) return CUPCFD_E_ERROR;
+    }
+    if (d.C)
+        hipLaunchKernelGGL(k_scalar<T>, dim3(blocks(d.C)), dim3(kBlock), 0, (hipStream_t)stream, d, a);
+    return launch_ok();
+}
+
+template <class T>
+int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, const T *w, T *uB, T *vB, T *wB,
+            const T *visEff, const T *visEffB, T *massFlux, const T *dudx, const T *dvdx, const T *dwdx, T *rFace,
+            T *su, T *sv, T *sw, T *au, T *av, T *aw, void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    UvwArgs<T> a{u, v, w, visEff, visEffB, dudx, dvdx, dwdx, uB, vB, wB, massFlux, rFace, su, sv, sw, au, av, aw,
+                 gammaBlend, small};
+    if (d.C) hipLaunchKernelGGL(k_uvw<T>, dim3(blocks(d.C)), dim3(kBlock), 0, (hipStream_t)stream, d, a);
+    return launch_ok();
+}
+
+int permute(cupcfd_hip_mesh *h, bool cells, const void *src, void *dst, int ncomp, int eb, void *stream, int dir) {
+    if (!h || !src || !dst) return CUPCFD_E_NULL_PTR;
+    if (ncomp < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    int n;
+    const std::vector<int32_t> *ord;
+    if (h->precision == 64) {
+        n = cells ? h->m64->d.C : h->m64->d.F;
+        ord = cells ? &h->m64->cell_order : &h->m64->face_order;
+    } else {
+        n = cells ? h->m32->d.C : h->m32->d.F;
+        ord = cells ? &h->m32->cell_order : &h->m32->face_order;
+    }
+    if (n == 0) return CUPCFD_E_SUCCESS;
+    // device copy of the order, cached in a workspace slot
+    void *dord;
+    int slot = cells ? 40 : 41;
+    if (h->precision == 64) dord = h->m64->ws_get(slot, n * sizeof(int));
+    else dord = h->m32->ws_get(slot, n * sizeof(int));
+    if (!dord) return CUPCFD_E_ERROR;
+    if (hipMemcpyAsync(dord, ord->data(), n * sizeof(int), hipMemcpyHostToDevice, (hipStream_t)stream) !=
+        hipSuccess)
+        return CUPCFD_E_ERROR;
+    hipStream_t s = (hipStream_t)stream;
+    if (eb == 8)
+        hipLaunchKernelGGL(k_permute<uint64_t>, dim3(blocks(n)), dim3(kBlock), 0, s, n, ncomp, (const int *)dord,
+                           (const uint64_t *)src, (uint64_t *)dst, dir);
+    else if (eb == 4)
+        hipLaunchKernelGGL(k_permute<uint32_t>, dim3(blocks(n)), dim3(kBlock), 0, s, n, ncomp, (const int *)dord,
+                           (const uint32_t *)src, (uint32_t *)dst, dir);
+    else
+        return CUPCFD_E_CONFIG_INVALID_VALUE;
+    return launch_ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cupcfd_dev_cells_to_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                 int32_t elem_bytes, void *stream) {
+    return permute(mesh, true, src, dst, ncomp, elem_bytes, stream, 1);
+}
+int cupcfd_dev_cells_from_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                   int32_t elem_bytes, void *stream) {
+    return permute(mesh, true, src, dst, ncomp, elem_bytes, stream, 0);
+}
+int cupcfd_dev_faces_to_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                 int32_t elem_bytes, void *stream) {
+    return permute(mesh, false, src, dst, ncomp, elem_bytes, stream, 1);
+}
+int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                   int32_t elem_bytes, void *stream) {
+    return permute(mesh, false, src, dst, ncomp, elem_bytes, stream, 0);
+}
+
+#define DEV_API(T, S)                                                                                              \
+    int cupcfd_dev_gradient_phi_gauss_dolfyn_##S(cupcfd_hip_mesh *mesh, int32_t nGradient, const T *phiCell,       \
+                                                 const T *phiBoundary, T *dPhidxCell, T *dPhidxoCell,               \
+                                                 void *stream) {                                                   \
+        return dev_gradient<T>(mesh, nGradient, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream);       \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_face_loop_##S(                                                                        \
+        cupcfd_hip_mesh *mesh, const T *dudx, const T *dvdx, const T *dwdx, const T *dpdx, const T *denCell,       \
+        T *denBoundary, const T *uCell, const T *vCell, const T *wCell, T *massFlux, const T *p, const T *ar,      \
+        T *su, T *rface, T small, int32_t *icinl, int32_t *icout, int32_t *icsym, int32_t *icwal,                  \
+        int32_t solveTurbEnergy, int32_t solveTurbDiss, int32_t solveVisc, int32_t solveEnthalpy,                  \
+        const T *teCell, T *teBoundary, const T *edCell, T *edBoundary, const T *viseffCell, T *viseffBoundary,    \
+        const T *tCell, T *tBoundary, void *stream) {                                                              \
+        return dev_mass<T>(mesh, dudx, dvdx, dwdx, dpdx, denCell, denBoundary, uCell, vCell, wCell, massFlux, p,   \
+                           ar, su, rface, small, icinl, icout, icsym, icwal, solveTurbEnergy, solveTurbDiss,       \
+                           solveVisc, solveEnthalpy, teCell, teBoundary, edCell, edBoundary, viseffCell,           \
+                           viseffBoundary, tCell, tBoundary, stream);                                              \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_boundary_loop1_##S(cupcfd_hip_mesh *mesh, const T *massFlux, T *flowin,               \
+                                                void *stream) {                                                    \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        hipLaunchKernelGGL(k_mass_bl1<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, massFlux, flowin);      \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_boundary_loop2_##S(cupcfd_hip_mesh *mesh, const T *massFlux, T *flowRegion,           \
+                                                T *flowout, void *stream) {                                        \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        hipLaunchKernelGGL(k_mass_bl2<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, massFlux, flowRegion,   \
+                           flowout);                                                                               \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_boundary_loop3_##S(cupcfd_hip_mesh *mesh, T flowin, T *ratearea, void *stream) {      \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        hipLaunchKernelGGL(k_mass_bl3<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, flowin, ratearea);      \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_boundary_loop4_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *uBoundary, T *vBoundary,    \
+                                                T *wBoundary, const T *denBoundary, T ratearea, T *flowout,        \
+                                                void *stream) {                                                    \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        const int n = M->d.nblist[CUPCFD_RTYPE_OUTLET];                                                            \
+        if (n)                                                                                                     \
+            hipLaunchKernelGGL(k_mass_bl4_faces<T>, dim3(blocks(n)), dim3(kBlock), 0, (hipStream_t)stream, M->d,   \
+                               massFlux, uBoundary, vBoundary, wBoundary, denBoundary, ratearea);                  \
+        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, (const T *)massFlux,  \
+                           flowout, 1);                                                                            \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_boundary_loop5_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *su, T *uBoundary,           \
+                                                T *vBoundary, T *wBoundary, T fact, int32_t solveU,                \
+                                                int32_t solveV, int32_t solveW, const T *flowFact, T *flowout2,    \
+                                                void *stream) {                                                    \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        if (M->d.noc)                                                                                              \
+            hipLaunchKernelGGL(k_mass_bl5_cells<T>, dim3(blocks(M->d.noc)), dim3(kBlock), 0, (hipStream_t)stream,  \
+                               M->d, massFlux, su, uBoundary, vBoundary, wBoundary, fact, solveU, solveV, solveW,  \
+                               flowFact);                                                                          \
+        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, (const T *)massFlux,  \
+                           flowout2, 0);                                                                           \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_mass_region_loop_##S(cupcfd_hip_mesh *mesh, T *flowFact, const T *flowRegion, T flowIn,    \
+                                             void *stream) {                                                       \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        if (M->d.R)                                                                                                \
+            hipLaunchKernelGGL(k_mass_region<T>, dim3(blocks(M->d.R)), dim3(kBlock), 0, (hipStream_t)stream,       \
+                               M->d, flowFact, flowRegion, flowIn);                                                \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_scalar_face_loop_##S(                                                                      \
+        cupcfd_hip_mesh *mesh, const T *PhiCell, T *PhiBoundary, const T *VisEff, T *Au, T *Su, const T *Den,      \
+        const T *MassFlux, const T *TE, const T *CpBoundary, const T *visEffBoundary, T *RFace, const T *dPhidx,   \
+        int32_t SolveTurb, int32_t SolveEnthalpy, T vislam, int32_t ivar, int32_t VarT, T Sigma_T, T Prandtl,      \
+        int32_t VarTE, T Sigma_k, int32_t VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large,  \
+        T TMCmu, void *stream) {                                                                                   \
+        return dev_scalar<T>(mesh, PhiCell, PhiBoundary, VisEff, Au, Su, Den, MassFlux, TE, CpBoundary,            \
+                             visEffBoundary, RFace, dPhidx, SolveTurb, SolveEnthalpy, vislam, ivar, VarT, Sigma_T, \
+                             Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, Small, Large,  \
+                             TMCmu, stream);                                                                       \
+    }                                                                                                              \
+    int cupcfd_dev_flux_uvw_face_loop1_##S(cupcfd_hip_mesh *mesh, T gammaBlend, T small, const T *uCell,           \
+                                           const T *vCell, const T *wCell, T *uBoundary, T *vBoundary,             \
+                                           T *wBoundary, const T *visEffCell, const T *visEffBoundary,             \
+                                           T *massFlux, const T *dudx, const T *dvdx, const T *dwdx, T *rFace,     \
+                                           T *su, T *sv, T *sw, T *au, T *av, T *aw, void *stream) {               \
+        return dev_uvw<T>(mesh, gammaBlend, small, uCell, vCell, wCell, uBoundary, vBoundary, wBoundary,           \
+                          visEffCell, visEffBoundary, massFlux, dudx, dvdx, dwdx, rFace, su, sv, sw, au, av, aw,   \
+                          stream);                                                                                 \
+    }                                                                                                              \
+    int cupcfd_dev_flux_uvw_region_loop1_##S(cupcfd_hip_mesh *mesh, void *stream) {                                \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        if (M->d.R)                                                                                                \
+            hipLaunchKernelGGL(k_uvw_region1<T>, dim3(blocks(M->d.R)), dim3(kBlock), 0, (hipStream_t)stream,       \
+                               M->d);                                                                              \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_flux_uvw_bnds_loop1_##S(cupcfd_hip_mesh *mesh, void *stream) {                                  \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        hipLaunchKernelGGL(k_uvw_bnds1<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d);                       \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_viscosity_cell_loop1_##S(cupcfd_hip_mesh *mesh, T small, T cmu, T visURF, T visLam,             \
+                                            const T *TE, const T *ED, const T *den, T *visEff, void *stream) {     \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        if (M->d.C)                                                                                                \
+            hipLaunchKernelGGL(k_visc_cell1<T>, dim3(blocks(M->d.C)), dim3(kBlock), 0, (hipStream_t)stream,        \
+                               M->d.C, small, cmu, visURF, visLam, TE, ED, den, visEff);                           \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_viscosity_region_loop_##S(T kappa, cupcfd_hip_mesh *mesh, void *stream) {                       \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        hipLaunchKernelGGL(k_visc_region<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, kappa);              \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_viscosity_boundary_loop_##S(cupcfd_hip_mesh *mesh, T tmcmu, T large, T small, T kappa,          \
+                                               T visLam, const T *TE, const T *den, const T *visEffCell,           \
+                                               T *visEffBoundary, void *stream) {                                  \
+        (void)large;                                                                                               \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        const T cmu = tmcmu;                                                                                       \
+        const T cmu25 = pow(cmu, 0.25);                                                                            \
+        if (M->d.B)                                                                                                \
+            hipLaunchKernelGGL(k_visc_boundary<T>, dim3(blocks(M->d.B)), dim3(kBlock), 0, (hipStream_t)stream,     \
+                               M->d, cmu, cmu25, small, kappa, visLam, TE, den, visEffCell, visEffBoundary);       \
+        return launch_ok();                                                                                        \
+    }                                                                                                              \
+    int cupcfd_dev_viscosity_cell_loop2_##S(cupcfd_hip_mesh *mesh, T visLam, T *visEffCell, T *visEffBoundary,     \
+                                            void *stream) {                                                        \
+        Mesh<T> *M = get<T>(mesh);                                                                                 \
+        if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
+        const int n = M->d.C + M->d.B;                                                                             \
+        if (n)                                                                                                     \
+            hipLaunchKernelGGL(k_visc_cell2<T>, dim3(blocks(n)), dim3(kBlock), 0, (hipStream_t)stream, M->d.C,     \
+                               M->d.B, visLam, visEffCell, visEffBoundary);                                        \
+        return launch_ok();                                                                                        \
+    }
+
+DEV_API(double, f64)
+DEV_API(float, f32)
+
+#undef DEV_API
+
+const char *cupcfd_hip_version(void) { return "cupcfd_hip 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,354 @@
+"""ctypes binding of libcupcfd_hip.so (the C ABI declared in include/cupcfd_hip.h).
+
+The library is the product; this module only marshals numpy arrays and
+torch device pointers into it. Loading fails loudly when the built library is
+missing: there is no fallback implementation.
+
+Host meshes come from ``HostMesh`` (StructGen numbering, see
+``cupcfd_structgen_create``); ``DeviceMesh`` repacks one into HBM and exposes
+the 16 reference kernels in drop-in form (numpy arrays in the mesh's
+numbering, synchronous) and resident form (device pointers in internal order).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcupcfd_hip.so")
+
+E_SUCCESS = 95
+E_INVALID_INDEX = 68
+E_ARRAY_SIZE_MISMATCH = 25
+E_ERROR = 98
+E_NULL_PTR = 99
+E_NOT_IMPLEMENTED = 109
+
+RTYPE_DEFAULT, RTYPE_INLET, RTYPE_OUTLET, RTYPE_SYMP, RTYPE_WALL = range(5)
+MESH_REORDER = 1
+
+
+class CupCfdError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} returned eCodes {code}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        _lib = ct.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+# ---------------------------------------------------------------------------
+# C structures
+# ---------------------------------------------------------------------------
+class V3F64(ct.Structure):
+    _fields_ = [("base", ct.c_void_p), ("stride", ct.c_int64)]
+
+
+class V3F32(ct.Structure):
+    _fields_ = [("base", ct.c_void_p), ("stride", ct.c_int64)]
+
+
+def _view_fields(v3):
+    P = ct.c_void_p
+    return [
+        ("nCells", ct.c_int32), ("nFaces", ct.c_int32), ("nBoundaries", ct.c_int32), ("nRegions", ct.c_int32),
+        ("faceCell1ID", P), ("faceCell2ID", P), ("faceBoundaryID", P),
+        ("faceLambda", P), ("faceRLencos", P), ("faceArea", P),
+        ("faceNorm", v3), ("faceCenter", v3), ("faceXpac", v3), ("faceXnac", v3),
+        ("cellCenter", v3), ("cellVolume", P),
+        ("boundaryFaceID", P), ("boundaryRegionID", P), ("boundaryDistance", P),
+        ("boundaryYPlus", P), ("boundaryUPlus", P), ("boundaryQ", P), ("boundaryH", P), ("boundaryT", P),
+        ("boundaryShear", v3),
+        ("regionType", P), ("regionStd", P), ("regionFlux", P), ("regionAdiab", P),
+        ("regionYLog", P), ("regionELog", P), ("regionDensity", P), ("regionTurbKE", P), ("regionTurbDiss", P),
+        ("regionSplvl", P), ("regionDen", P), ("regionR", P), ("regionT", P),
+        ("regionForceTangent", v3), ("regionUVW", v3),
+    ]
+
+
+class MeshViewF64(ct.Structure):
+    _fields_ = _view_fields(V3F64)
+
+
+class MeshViewF32(ct.Structure):
+    _fields_ = _view_fields(V3F32)
+
+
+class StructGenParams(ct.Structure):
+    _fields_ = [
+        ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
+        ("xmin", ct.c_double), ("xmax", ct.c_double), ("ymin", ct.c_double), ("ymax", ct.c_double),
+        ("zmin", ct.c_double), ("zmax", ct.c_double),
+        ("seed", ct.c_uint64),
+        ("permute_cells", ct.c_int32), ("permute_faces", ct.c_int32),
+        ("regions", ct.c_int32), ("face_extras", ct.c_int32), ("tets", ct.c_int32),
+        ("jitter", ct.c_double),
+    ]
+
+
+KERNELS = [
+    "gradient_phi_gauss_dolfyn", "flux_mass_face_loop", "flux_mass_boundary_loop1", "flux_mass_boundary_loop2",
+    "flux_mass_boundary_loop3", "flux_mass_boundary_loop4", "flux_mass_boundary_loop5", "flux_mass_region_loop",
+    "flux_scalar_face_loop", "flux_uvw_face_loop1", "flux_uvw_region_loop1", "flux_uvw_bnds_loop1",
+    "viscosity_cell_loop1", "viscosity_region_loop", "viscosity_boundary_loop", "viscosity_cell_loop2",
+]
+
+HOST_SYMBOLS = [
+    "cupcfd_structgen_create", "cupcfd_host_mesh_view_f64", "cupcfd_host_mesh_view_f32",
+    "cupcfd_host_mesh_destroy", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
+    "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
+    "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_push_state",
+    "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
+    "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
+]
+
+
+def all_symbols():
+    """Every symbol include/cupcfd_hip.h declares."""
+    out = list(HOST_SYMBOLS)
+    for s in ("f64", "f32"):
+        for k in KERNELS:
+            out.append(f"cupcfd_fvm_{k}_{s}")
+            out.append(f"cupcfd_dev_{k}_{s}")
+    return out
+
+
+def _declare(L):
+    L.cupcfd_structgen_create.argtypes = [ct.POINTER(StructGenParams), ct.POINTER(ct.c_void_p)]
+    L.cupcfd_host_mesh_view_f64.argtypes = [ct.c_void_p, ct.POINTER(MeshViewF64)]
+    L.cupcfd_host_mesh_view_f32.argtypes = [ct.c_void_p, ct.POINTER(MeshViewF32)]
+    L.cupcfd_host_mesh_destroy.argtypes = [ct.c_void_p]
+    L.cupcfd_host_mesh_destroy.restype = None
+    L.cupcfd_hip_mesh_create_f64.argtypes = [ct.POINTER(MeshViewF64), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_create_f32.argtypes = [ct.POINTER(MeshViewF32), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_destroy.argtypes = [ct.c_void_p]
+    L.cupcfd_hip_mesh_counts.argtypes = [ct.c_void_p] + [ct.POINTER(ct.c_int32)] * 4
+    L.cupcfd_hip_mesh_cell_order.argtypes = [ct.c_void_p, ct.c_void_p]
+    L.cupcfd_hip_mesh_face_order.argtypes = [ct.c_void_p, ct.c_void_p]
+    L.cupcfd_hip_mesh_set_carry_init.argtypes = [ct.c_void_p, ct.c_double]
+    L.cupcfd_hip_mesh_push_state.argtypes = [ct.c_void_p]
+    L.cupcfd_hip_mesh_pull_state.argtypes = [ct.c_void_p]
+    for n in ("cells_to_internal", "cells_from_internal", "faces_to_internal", "faces_from_internal"):
+        getattr(L, f"cupcfd_dev_{n}").argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32,
+                                                   ct.c_void_p]
+    L.cupcfd_hip_version.restype = ct.c_char_p
+    # kernel entry points take pointers, int32 and T scalars; arguments are
+    # converted explicitly by the wrappers below
+    for s in ("f64", "f32"):
+        for k in KERNELS:
+            for mode in ("fvm", "dev"):
+                getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
+
+
+def check(fn, code):
+    if code != E_SUCCESS:
+        raise CupCfdError(fn, code)
+
+
+# ---------------------------------------------------------------------------
+# Host mesh
+# ---------------------------------------------------------------------------
+def _np(ptr, n, dtype, shape=None):
+    if n == 0:
+        return np.zeros(shape if shape else (0,), dtype=dtype)
+    ctype = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
+    a = np.ctypeslib.as_array(ct.cast(ptr, ct.POINTER(ctype)), shape=(n,))
+    return a.reshape(shape) if shape else a
+
+
+class HostMesh:
+    """A StructGen-numbered mesh held by the library (cupcfd_structgen_create)."""
+
+    def __init__(self, nx, ny=None, nz=None, bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), seed=0,
+                 permute_cells=False, permute_faces=False, typed=False, face_extras=False, tets=False,
+                 jitter=0.0):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        p = StructGenParams(nx, ny, nz, *[float(b) for b in bounds], seed, int(permute_cells), int(permute_faces),
+                            int(typed), int(face_extras), int(tets), float(jitter))
+        self._h = ct.c_void_p()
+        L = lib()
+        check("cupcfd_structgen_create", L.cupcfd_structgen_create(ct.byref(p), ct.byref(self._h)))
+        self.params = dict(nx=nx, ny=ny, nz=nz, bounds=bounds, seed=seed, permute_cells=permute_cells,
+                           permute_faces=permute_faces, typed=typed, face_extras=face_extras, tets=tets,
+                           jitter=jitter)
+        self.view64 = MeshViewF64()
+        check("cupcfd_host_mesh_view_f64", L.cupcfd_host_mesh_view_f64(self._h, ct.byref(self.view64)))
+        self._view32 = None
+        v = self.view64
+        self.nCells, self.nFaces, self.nBoundaries, self.nRegions = v.nCells, v.nFaces, v.nBoundaries, v.nRegions
+        C, F, B, R = self.nCells, self.nFaces, self.nBoundaries, self.nRegions
+        i32, f64, u8 = np.int32, np.float64, np.uint8
+        self.a = {
+            "faceCell1": _np(v.faceCell1ID, F, i32), "faceCell2": _np(v.faceCell2ID, F, i32),
+            "faceBnd": _np(v.faceBoundaryID, F, i32), "faceLambda": _np(v.faceLambda, F, f64),
+            "faceRLencos": _np(v.faceRLencos, F, f64), "faceArea": _np(v.faceArea, F, f64),
+            "faceNorm": _np(v.faceNorm.base, 3 * F, f64, (F, 3)),
+            "faceCenter": _np(v.faceCenter.base, 3 * F, f64, (F, 3)),
+            "faceXpac": _np(v.faceXpac.base, 3 * F, f64, (F, 3)),
+            "faceXnac": _np(v.faceXnac.base, 3 * F, f64, (F, 3)),
+            "cellCenter": _np(v.cellCenter.base, 3 * C, f64, (C, 3)), "cellVolume": _np(v.cellVolume, C, f64),
+            "bndFace": _np(v.boundaryFaceID, B, i32), "bndRegion": _np(v.boundaryRegionID, B, i32),
+            "bndDistance": _np(v.boundaryDistance, B, f64), "bndYPlus": _np(v.boundaryYPlus, B, f64),
+            "bndUPlus": _np(v.boundaryUPlus, B, f64), "bndQ": _np(v.boundaryQ, B, f64),
+            "bndH": _np(v.boundaryH, B, f64), "bndT": _np(v.boundaryT, B, f64),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f64, (B, 3)),
+            "regType": _np(v.regionType, R, i32), "regStd": _np(v.regionStd, R, u8),
+            "regFlux": _np(v.regionFlux, R, u8), "regAdiab": _np(v.regionAdiab, R, u8),
+            "regYLog": _np(v.regionYLog, R, f64), "regELog": _np(v.regionELog, R, f64),
+            "regDensity": _np(v.regionDensity, R, f64), "regTurbKE": _np(v.regionTurbKE, R, f64),
+            "regTurbDiss": _np(v.regionTurbDiss, R, f64), "regSplvl": _np(v.regionSplvl, R, f64),
+            "regDen": _np(v.regionDen, R, f64), "regR": _np(v.regionR, R, f64), "regT": _np(v.regionT, R, f64),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f64, (R, 3)),
+            "regUVW": _np(v.regionUVW.base, 3 * R, f64, (R, 3)),
+        }
+
+    STATE = ("bndYPlus", "bndUPlus", "bndQ", "bndH", "bndT", "bndShear", "regYLog", "regForceTangent")
+
+    def view32(self):
+        if self._view32 is None:
+            self._view32 = MeshViewF32()
+            check("cupcfd_host_mesh_view_f32", lib().cupcfd_host_mesh_view_f32(self._h, ct.byref(self._view32)))
+        return self._view32
+
+    def state(self, precision=64):
+        """Copies of the mutable mesh state arrays (as the view of that precision holds them)."""
+        if precision == 64:
+            return {k: self.a[k].copy() for k in self.STATE}
+        v = self.view32()
+        B, R = self.nBoundaries, self.nRegions
+        f32 = np.float32
+        return {
+            "bndYPlus": _np(v.boundaryYPlus, B, f32).copy(), "bndUPlus": _np(v.boundaryUPlus, B, f32).copy(),
+            "bndQ": _np(v.boundaryQ, B, f32).copy(), "bndH": _np(v.boundaryH, B, f32).copy(),
+            "bndT": _np(v.boundaryT, B, f32).copy(),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3)).copy(),
+            "regYLog": _np(v.regionYLog, R, f32).copy(),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3)).copy(),
+        }
+
+    def set_state(self, st, precision=64):
+        if precision == 64:
+            for k, v in st.items():
+                self.a[k][...] = v
+            return
+        v = self.view32()
+        B, R = self.nBoundaries, self.nRegions
+        f32 = np.float32
+        tgt = {
+            "bndYPlus": _np(v.boundaryYPlus, B, f32), "bndUPlus": _np(v.boundaryUPlus, B, f32),
+            "bndQ": _np(v.boundaryQ, B, f32), "bndH": _np(v.boundaryH, B, f32), "bndT": _np(v.boundaryT, B, f32),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3)), "regYLog": _np(v.regionYLog, R, f32),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3)),
+        }
+        for k, val in st.items():
+            tgt[k][...] = val
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.cupcfd_host_mesh_destroy(h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+# Device mesh and kernels
+# ---------------------------------------------------------------------------
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+        return a.ctypes.data_as(ct.c_void_p)
+    if hasattr(a, "data_ptr"):  # torch tensor on the device
+        return ct.c_void_p(a.data_ptr())
+    if isinstance(a, int):
+        return ct.c_void_p(a)
+    raise TypeError(type(a))
+
+
+def _n(a, per=1):
+    return ct.c_int32(0 if a is None else (a.size if isinstance(a, np.ndarray) else a.numel()) // per)
+
+
+class DeviceMesh:
+    """A mesh repacked into HBM (cupcfd_hip_mesh_create_f64 / _f32)."""
+
+    def __init__(self, host: HostMesh, precision=64, reorder=True):
+        self.host = host
+        self.precision = precision
+        self.T = ct.c_double if precision == 64 else ct.c_float
+        self.np_t = np.float64 if precision == 64 else np.float32
+        self.sfx = "f64" if precision == 64 else "f32"
+        self.V3 = V3F64 if precision == 64 else V3F32
+        self._h = ct.c_void_p()
+        L = lib()
+        flags = MESH_REORDER if reorder else 0
+        if precision == 64:
+            check("cupcfd_hip_mesh_create_f64",
+                  L.cupcfd_hip_mesh_create_f64(ct.byref(host.view64), flags, ct.byref(self._h)))
+        else:
+            check("cupcfd_hip_mesh_create_f32",
+                  L.cupcfd_hip_mesh_create_f32(ct.byref(host.view32()), flags, ct.byref(self._h)))
+        c = [ct.c_int32() for _ in range(4)]
+        check("cupcfd_hip_mesh_counts", L.cupcfd_hip_mesh_counts(self._h, *[ct.byref(x) for x in c]))
+        self.nCells, self.nFaces, self.nBoundaries, self.nRegions = [x.value for x in c]
+        self.cell_order = np.zeros(self.nCells, np.int32)
+        self.face_order = np.zeros(self.nFaces, np.int32)
+        L.cupcfd_hip_mesh_cell_order(self._h, _ptr(self.cell_order))
+        L.cupcfd_hip_mesh_face_order(self._h, _ptr(self.face_order))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_carry_init(self, v):
+        check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
+
+    def push_state(self):
+        check("push_state", lib().cupcfd_hip_mesh_push_state(self._h))
+
+    def pull_state(self):
+        check("pull_state", lib().cupcfd_hip_mesh_pull_state(self._h))
+
+    def v3(self, a):
+        """cupcfd_v3 descriptor of a packed (n,3) numpy array."""
+        assert a.dtype == self.np_t and a.flags["C_CONTIGUOUS"]
+        return self.V3(a.ctypes.data, 0)
+
+    def _s(self, x):
+        return self.T(x)
+
+    def call(self, mode, name, *args):
+        """Call cupcfd_<mode>_<name>_<sfx> with already-converted ctypes args."""
+        fn = f"cupcfd_{mode}_{name}_{self.sfx}"
+        code = getattr(lib(), fn)(*args)
+        check(fn, code)
+        return code
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.cupcfd_hip_mesh_destroy(h)
+            self._h = None
+
+
+def version():
+    return lib().cupcfd_hip_version().decode()
+
+
+from .fvm import *  # noqa: E402,F401,F403  (kernel wrappers)

@@ -1,0 +1,233 @@
+"""Python mirror of namespace cupcfd::fvm (include/fvm/implementation/component/*Kernels.h).
+
+Each function keeps the reference's name and argument order; the
+(pointer, length) pairs collapse to one numpy array whose length is passed
+through. Arrays are host numpy arrays in the mesh's numbering (drop-in mode,
+synchronous); 3-vectors are (n, 3) arrays. Every function returns the
+eCodes value of the C ABI (E_SUCCESS = 95).
+
+``resident`` holds the same kernels over device tensors in the device mesh's
+internal order (cupcfd_dev_*), launched on a HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+__all__ = [
+    "GradientPhiGaussDolfyn", "FluxMassDolfynFaceLoop", "FluxMassDolfynBoundaryLoop1",
+    "FluxMassDolfynBoundaryLoop2", "FluxMassDolfynBoundaryLoop3", "FluxMassDolfynBoundaryLoop4",
+    "FluxMassDolfynBoundaryLoop5", "FluxMassDolfynRegionLoop", "FluxScalarDolfynFaceLoop",
+    "FluxUVWDolfynFaceLoop1", "FluxUVWDolfynRegionLoop1", "FluxUVWDolfynBndsLoop1",
+    "calculateViscosityDolfynCellLoop1", "calculateViscosityDolfynRegionLoop",
+    "calculateViscosityDolfynBoundaryLoop", "calculateViscosityDolfynCellLoop2", "resident",
+]
+
+
+def _P(a):
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"], "C-contiguous numpy arrays expected"
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _N(a, per=1):
+    return ct.c_int32(0 if a is None else a.size // per)
+
+
+def _I(x):
+    return ct.c_int32(int(x))
+
+
+def _A(m, a):  # array + length
+    return [_P(a), _N(a)]
+
+
+def _V(m, a):  # 3-vector array + element count
+    if a is None:
+        return [m.V3(0, 0), ct.c_int32(0)]
+    return [m.v3(a), ct.c_int32(a.shape[0])]
+
+
+def _raw(m, name, *args):
+    from . import lib
+    return getattr(lib(), f"cupcfd_fvm_{name}_{m.sfx}")(*args)
+
+
+def GradientPhiGaussDolfyn(mesh, nGradient, phiCell, phiBoundary, dPhidxCell, dPhidxoCell):
+    """GradientKernels.h:48-54"""
+    return _raw(mesh, "gradient_phi_gauss_dolfyn", mesh.handle, _I(nGradient), *_A(mesh, phiCell),
+                *_A(mesh, phiBoundary), *_V(mesh, dPhidxCell), *_V(mesh, dPhidxoCell))
+
+
+def FluxMassDolfynFaceLoop(mesh, dudx, dvdx, dwdx, dpdx, denCell, denBoundary, uCell, vCell, wCell, massFlux, p, ar,
+                           su, rface, small, icinl, icout, icsym, icwal, solveTurbEnergy, solveTurbDiss, solveVisc,
+                           solveEnthalpy, teCell, teBoundary, edCell, edBoundary, viseffCell, viseffBoundary, tCell,
+                           tBoundary):
+    """MassKernels.h:48-81. icinl..icwal are 1-element int32 arrays (incremented in place)."""
+    T = mesh.T
+    return _raw(mesh, "flux_mass_face_loop", mesh.handle, *_V(mesh, dudx), *_V(mesh, dvdx), *_V(mesh, dwdx),
+                *_V(mesh, dpdx), *_A(mesh, denCell), *_A(mesh, denBoundary), *_A(mesh, uCell), *_A(mesh, vCell),
+                *_A(mesh, wCell), *_A(mesh, massFlux), *_A(mesh, p), *_A(mesh, ar), *_A(mesh, su),
+                *_A(mesh, rface), T(small), _P(icinl), _P(icout), _P(icsym), _P(icwal), _I(solveTurbEnergy),
+                _I(solveTurbDiss), _I(solveVisc), _I(solveEnthalpy), *_A(mesh, teCell), *_A(mesh, teBoundary),
+                *_A(mesh, edCell), *_A(mesh, edBoundary), *_A(mesh, viseffCell), *_A(mesh, viseffBoundary),
+                *_A(mesh, tCell), *_A(mesh, tBoundary))
+
+
+def FluxMassDolfynBoundaryLoop1(mesh, massFlux, flowin):
+    """MassKernels.h:90-94. flowin: 1-element array (output)."""
+    return _raw(mesh, "flux_mass_boundary_loop1", mesh.handle, *_A(mesh, massFlux), _P(flowin))
+
+
+def FluxMassDolfynBoundaryLoop2(mesh, massFlux, flowRegion, flowout):
+    """MassKernels.h:103-108"""
+    return _raw(mesh, "flux_mass_boundary_loop2", mesh.handle, *_A(mesh, massFlux), *_A(mesh, flowRegion),
+                _P(flowout))
+
+
+def FluxMassDolfynBoundaryLoop3(mesh, flowin, ratearea):
+    """MassKernels.h:117-120"""
+    return _raw(mesh, "flux_mass_boundary_loop3", mesh.handle, mesh.T(flowin), _P(ratearea))
+
+
+def FluxMassDolfynBoundaryLoop4(mesh, massFlux, uBoundary, vBoundary, wBoundary, denBoundary, ratearea, flowout):
+    """MassKernels.h:129-138"""
+    return _raw(mesh, "flux_mass_boundary_loop4", mesh.handle, *_A(mesh, massFlux), *_A(mesh, uBoundary),
+                *_A(mesh, vBoundary), *_A(mesh, wBoundary), *_A(mesh, denBoundary), mesh.T(ratearea), _P(flowout))
+
+
+def FluxMassDolfynBoundaryLoop5(mesh, massFlux, su, uBoundary, vBoundary, wBoundary, fact, solveU, solveV, solveW,
+                                flowFact, flowout2):
+    """MassKernels.h:147-157"""
+    return _raw(mesh, "flux_mass_boundary_loop5", mesh.handle, *_A(mesh, massFlux), *_A(mesh, su),
+                *_A(mesh, uBoundary), *_A(mesh, vBoundary), *_A(mesh, wBoundary), mesh.T(fact), _I(solveU),
+                _I(solveV), _I(solveW), *_A(mesh, flowFact), _P(flowout2))
+
+
+def FluxMassDolfynRegionLoop(mesh, flowFact, flowRegion, flowIn):
+    """MassKernels.h:166-171"""
+    return _raw(mesh, "flux_mass_region_loop", mesh.handle, *_A(mesh, flowFact), *_A(mesh, flowRegion),
+                mesh.T(flowIn))
+
+
+def FluxScalarDolfynFaceLoop(mesh, PhiCell, PhiBoundary, VisEff, Au, Su, Den, MassFlux, TE, CpBoundary,
+                             visEffBoundary, RFace, dPhidx, SolveTurb, SolveEnthalpy, vislam, ivar, VarT, Sigma_T,
+                             Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, Small, Large,
+                             TMCmu):
+    """ScalarKernels.h:47-67"""
+    T = mesh.T
+    return _raw(mesh, "flux_scalar_face_loop", mesh.handle, *_A(mesh, PhiCell), *_A(mesh, PhiBoundary),
+                *_A(mesh, VisEff), *_A(mesh, Au), *_A(mesh, Su), *_A(mesh, Den), *_A(mesh, MassFlux),
+                *_A(mesh, TE), *_A(mesh, CpBoundary), *_A(mesh, visEffBoundary), *_A(mesh, RFace),
+                *_V(mesh, dPhidx), _I(SolveTurb), _I(SolveEnthalpy), T(vislam), _I(ivar), _I(VarT), T(Sigma_T),
+                T(Prandtl), _I(VarTE), T(Sigma_k), _I(VarED), T(Sigma_e), T(Sigma_s), T(Schmidt), T(GammaBlend),
+                T(Small), T(Large), T(TMCmu))
+
+
+def FluxUVWDolfynFaceLoop1(mesh, gammaBlend, small, uCell, vCell, wCell, uBoundary, vBoundary, wBoundary, visEffCell,
+                           visEffBoundary, massFlux, dudx, dvdx, dwdx, rFace, su, sv, sw, au, av, aw):
+    """UVWKernels.h:43-67"""
+    T = mesh.T
+    return _raw(mesh, "flux_uvw_face_loop1", mesh.handle, T(gammaBlend), T(small), *_A(mesh, uCell),
+                *_A(mesh, vCell), *_A(mesh, wCell), *_A(mesh, uBoundary), *_A(mesh, vBoundary), *_A(mesh, wBoundary),
+                *_A(mesh, visEffCell), *_A(mesh, visEffBoundary), *_A(mesh, massFlux), *_V(mesh, dudx),
+                *_V(mesh, dvdx), *_V(mesh, dwdx), *_A(mesh, rFace), *_A(mesh, su), *_A(mesh, sv), *_A(mesh, sw),
+                *_A(mesh, au), *_A(mesh, av), *_A(mesh, aw))
+
+
+def FluxUVWDolfynRegionLoop1(mesh):
+    """UVWKernels.h:72-73"""
+    return _raw(mesh, "flux_uvw_region_loop1", mesh.handle)
+
+
+def FluxUVWDolfynBndsLoop1(mesh):
+    """UVWKernels.h:78-79"""
+    return _raw(mesh, "flux_uvw_bnds_loop1", mesh.handle)
+
+
+def calculateViscosityDolfynCellLoop1(mesh, small, cmu, visURF, visLam, TE, ED, den, visEff):
+    """ViscosityKernels.h:38-48"""
+    T = mesh.T
+    return _raw(mesh, "viscosity_cell_loop1", mesh.handle, T(small), T(cmu), T(visURF), T(visLam), *_A(mesh, TE),
+                *_A(mesh, ED), *_A(mesh, den), *_A(mesh, visEff))
+
+
+def calculateViscosityDolfynRegionLoop(kappa, mesh):
+    """ViscosityKernels.h:50-51"""
+    return _raw(mesh, "viscosity_region_loop", mesh.T(kappa), mesh.handle)
+
+
+def calculateViscosityDolfynBoundaryLoop(mesh, tmcmu, large, small, kappa, visLam, TE, den, visEffCell,
+                                         visEffBoundary):
+    """ViscosityKernels.h:53-60"""
+    T = mesh.T
+    return _raw(mesh, "viscosity_boundary_loop", mesh.handle, T(tmcmu), T(large), T(small), T(kappa), T(visLam),
+                *_A(mesh, TE), *_A(mesh, den), *_A(mesh, visEffCell), *_A(mesh, visEffBoundary))
+
+
+def calculateViscosityDolfynCellLoop2(mesh, visLam, visEffCell, visEffBoundary):
+    """ViscosityKernels.h:62-67"""
+    return _raw(mesh, "viscosity_cell_loop2", mesh.handle, mesh.T(visLam), *_A(mesh, visEffCell),
+                *_A(mesh, visEffBoundary))
+
+
+class resident:
+    """cupcfd_dev_* over torch device tensors (internal order, packed xyz vectors).
+
+    ``stream`` is a torch.cuda.Stream or None (the current stream)."""
+
+    @staticmethod
+    def _s(stream):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return ct.c_void_p(s.cuda_stream)
+
+    @staticmethod
+    def _t(t):
+        return None if t is None else ct.c_void_p(t.data_ptr())
+
+    @staticmethod
+    def call(mesh, name, *args):
+        from . import lib, check
+        fn = f"cupcfd_dev_{name}_{mesh.sfx}"
+        check(fn, getattr(lib(), fn)(*args))
+
+    @classmethod
+    def gradient(cls, mesh, nGradient, phi, phiB, g, go, stream=None):
+        cls.call(mesh, "gradient_phi_gauss_dolfyn", mesh.handle, _I(nGradient), cls._t(phi), cls._t(phiB),
+                 cls._t(g), cls._t(go), cls._s(stream))
+
+    @classmethod
+    def mass(cls, mesh, dudx, dvdx, dwdx, dpdx, den, denB, u, v, w, mf, p, ar, su, rface, small=1e-18,
+             counters=None, flags=(0, 0, 1, 0), teC=None, teB=None, edC=None, edB=None, visC=None, visB=None,
+             tC=None, tB=None, stream=None):
+        cnt = counters if counters is not None else np.zeros(4, np.int32)
+        t = cls._t
+        base = cnt.ctypes.data
+        cls.call(mesh, "flux_mass_face_loop", mesh.handle, t(dudx), t(dvdx), t(dwdx), t(dpdx), t(den), t(denB),
+                 t(u), t(v), t(w), t(mf), t(p), t(ar), t(su), t(rface), mesh.T(small), ct.c_void_p(base),
+                 ct.c_void_p(base + 4), ct.c_void_p(base + 8), ct.c_void_p(base + 12), *[_I(x) for x in flags],
+                 t(teC), t(teB), t(edC), t(edB), t(visC), t(visB), t(tC), t(tB), cls._s(stream))
+        return cnt
+
+    @classmethod
+    def scalar(cls, mesh, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx, params,
+               stream=None):
+        t = cls._t
+        T = mesh.T
+        p = params
+        cls.call(mesh, "flux_scalar_face_loop", mesh.handle, t(phi), t(phiB), t(visEff), t(au), t(su), t(den),
+                 t(mf), t(te), t(cpB), t(visEffB), t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]),
+                 T(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]), T(p["Sigma_T"]), T(p["Prandtl"]), _I(p["VarTE"]),
+                 T(p["Sigma_k"]), _I(p["VarED"]), T(p["Sigma_e"]), T(p["Sigma_s"]), T(p["Schmidt"]),
+                 T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"]), cls._s(stream))
+
+    @classmethod
+    def uvw(cls, mesh, gammaBlend, small, u, v, w, uB, vB, wB, visEff, visEffB, mf, dudx, dvdx, dwdx, rface, su, sv,
+            sw, au, av, aw, stream=None):
+        t = cls._t
+        cls.call(mesh, "flux_uvw_face_loop1", mesh.handle, mesh.T(gammaBlend), mesh.T(small), t(u), t(v), t(w),
+                 t(uB), t(vB), t(wB), t(visEff), t(visEffB), t(mf), t(dudx), t(dvdx), t(dwdx), t(rface), t(su),
+                 t(sv), t(sw), t(au), t(av), t(aw), cls._s(stream))

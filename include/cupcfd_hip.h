@@ -1,0 +1,309 @@
+/*
+ * cupcfd_hip.h — C ABI of the MI355X (gfx950) finite-volume kernel library.
+ *
+ * It replaces the 16 function templates of namespace cupcfd::fvm
+ * (include/fvm/implementation/component/{Gradient,Mass,Scalar,UVW,Viscosity}Kernels.h
+ * in CUP-CFD) for a MiniSoA mesh (CupCfdSoAMesh<int,T,int>, T = double or float).
+ *
+ * Conventions
+ *   - Every entry point returns an int equal to a cupcfd::error::eCodes value
+ *     (include/error/Error.h): CUPCFD_E_SUCCESS (95) on success. HIP runtime
+ *     failures map to CUPCFD_E_ERROR. Nothing here throws.
+ *   - 3-vectors are described by cupcfd_v3_*: component k of element i lives
+ *     at ((char*)base + i*stride) + k*sizeof(T). stride == 0 means packed xyz.
+ *     The reference's EuclideanVector<double,3> (vptr + cmp[3]) is
+ *     {&v[0].cmp[0], 32}; EuclideanVector<float,3> is {&v[0].cmp[0], 24}.
+ *   - Two calling modes per kernel:
+ *       cupcfd_fvm_*  "drop-in": host arrays in the mesh's own numbering, with
+ *                     the reference's (pointer, length) pairs; synchronous:
+ *                     results (and mesh state the kernel mutates) are visible
+ *                     on return, as in the reference.
+ *       cupcfd_dev_*  "resident": device arrays in the device mesh's internal
+ *                     numbering (cell and face orders from
+ *                     cupcfd_hip_mesh_cell_order / _face_order; boundary and
+ *                     region arrays keep the mesh's numbering), 3-vectors
+ *                     packed xyz, launched asynchronously on `stream`
+ *                     (a hipStream_t, NULL = default stream). Mesh state stays
+ *                     on the device until cupcfd_hip_mesh_pull_state.
+ *   - Carried locals. The reference reads PhiFace (scalar loop),
+ *     uFace/vFace/wFace/visFace (UVW loop) and yplustmp (viscosity region
+ *     loop) before any assignment on meshes whose boundary regions do not
+ *     assign them. This library starts them at the mesh's carry-init value
+ *     (default 0, cupcfd_hip_mesh_set_carry_init) and otherwise carries them
+ *     from face to face in ascending face order exactly as the reference does.
+ */
+#ifndef CUPCFD_HIP_H
+#define CUPCFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cupcfd::error::eCodes values used at this boundary (include/error/Error.h) */
+#define CUPCFD_E_ARRAY_SIZE_MISMATCH 25
+#define CUPCFD_E_INVALID_INDEX 68
+#define CUPCFD_E_CONFIG_INVALID_VALUE 90
+#define CUPCFD_E_SUCCESS 95
+#define CUPCFD_E_ERROR 98
+#define CUPCFD_E_NULL_PTR 99
+#define CUPCFD_E_NOT_IMPLEMENTED 109
+
+/* cupcfd::geometry::mesh::RType (UnstructuredMeshInterface.h:58-65) */
+#define CUPCFD_RTYPE_DEFAULT 0
+#define CUPCFD_RTYPE_INLET 1
+#define CUPCFD_RTYPE_OUTLET 2
+#define CUPCFD_RTYPE_SYMP 3
+#define CUPCFD_RTYPE_WALL 4
+
+typedef struct cupcfd_v3_f64 { double *base; int64_t stride; } cupcfd_v3_f64;
+typedef struct cupcfd_v3_f32 { float *base; int64_t stride; } cupcfd_v3_f32;
+
+/*
+ * A borrowed view of a MiniSoA mesh: one pointer per std::vector member of
+ * CupCfdSoAMesh (CupCfdSoAMesh.h:71-154) that the fvm kernels read or write,
+ * plus UnstructuredMeshProperties' local counts (lTCells, lFaces,
+ * lBoundaries, lRegions). std::vector<bool> members (regionStd/Flux/Adiab)
+ * are passed as uint8_t arrays. Pointers marked "mutable" are mesh state
+ * that kernels set (setBoundaryShear/Q/H/T/YPlus/UPlus,
+ * setRegionYLog/ForceTangent); drop-in calls write them back here.
+ */
+#define CUPCFD_MESH_VIEW_FIELDS(T, V3)                                                    \
+    int32_t nCells, nFaces, nBoundaries, nRegions;                                        \
+    const int32_t *faceCell1ID, *faceCell2ID, *faceBoundaryID;                            \
+    const T *faceLambda, *faceRLencos, *faceArea;                                         \
+    V3 faceNorm, faceCenter, faceXpac, faceXnac;                                          \
+    V3 cellCenter;                                                                        \
+    const T *cellVolume;                                                                  \
+    const int32_t *boundaryFaceID, *boundaryRegionID;                                     \
+    const T *boundaryDistance;                                                            \
+    T *boundaryYPlus, *boundaryUPlus, *boundaryQ, *boundaryH, *boundaryT; /* mutable */   \
+    V3 boundaryShear;                                                     /* mutable */   \
+    const int32_t *regionType;                                                            \
+    const uint8_t *regionStd, *regionFlux, *regionAdiab;                                  \
+    T *regionYLog;                                                        /* mutable */   \
+    const T *regionELog, *regionDensity, *regionTurbKE, *regionTurbDiss;                  \
+    const T *regionSplvl, *regionDen, *regionR, *regionT;                                 \
+    V3 regionForceTangent;                                                /* mutable */   \
+    V3 regionUVW;
+
+typedef struct cupcfd_mesh_view_f64 { CUPCFD_MESH_VIEW_FIELDS(double, cupcfd_v3_f64) } cupcfd_mesh_view_f64;
+typedef struct cupcfd_mesh_view_f32 { CUPCFD_MESH_VIEW_FIELDS(float, cupcfd_v3_f32) } cupcfd_mesh_view_f32;
+
+/* ---------------------------------------------------------------------------
+ * Host mesh sources (run before the kernels; host C++).
+ * ------------------------------------------------------------------------- */
+typedef struct cupcfd_host_mesh cupcfd_host_mesh;
+
+/*
+ * Structured hex grid with MeshStructGenSource's numbering
+ * (src/geometry/mesh/implementation/source/MeshStructGenSource.cpp:33-1259:
+ * cell label x + y*nx + z*nx*ny, faces YZ then XZ then XY, boundary labels,
+ * cell1 = low-side cell, cell2 = high-side cell or -1, lambda 0.5 / -1) as
+ * assembled by UnstructuredMeshInterface::addData on one rank. Geometry is
+ * evaluated in closed form for the axis-aligned grid.
+ *   regions = 0: one RTYPE_DEFAULT region, boundary distance 0 (what the
+ *                reference builds); regions = 1: six typed regions, x-low
+ *                INLET, x-high OUTLET, y-low/y-high WALL, z-low/z-high SYMP,
+ *                boundary distance = half the cell width normal to the face.
+ *   face_extras = 0: xpac = xnac = rlencos = 0 (what the reference builds);
+ *                1: seeded non-zero xpac/xnac/rlencos so those terms are live.
+ *   permute_cells / permute_faces: seeded Fisher-Yates relabelling of cell /
+ *                face ids (boundary ids keep StructGen's labels).
+ *   tets = 1:    split every hex into 6 tetrahedra around its main diagonal
+ *                (12 faces per hex interior) with vertices jittered by
+ *                jitter * spacing (boundary vertices stay on their planes).
+ */
+typedef struct cupcfd_structgen_params {
+    int32_t nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    uint64_t seed;
+    int32_t permute_cells, permute_faces;
+    int32_t regions;
+    int32_t face_extras;
+    int32_t tets;
+    double jitter;
+} cupcfd_structgen_params;
+
+int cupcfd_structgen_create(const cupcfd_structgen_params *params, cupcfd_host_mesh **out);
+int cupcfd_host_mesh_view_f64(cupcfd_host_mesh *mesh, cupcfd_mesh_view_f64 *out);
+/* float copies of every array, owned by the host mesh */
+int cupcfd_host_mesh_view_f32(cupcfd_host_mesh *mesh, cupcfd_mesh_view_f32 *out);
+void cupcfd_host_mesh_destroy(cupcfd_host_mesh *mesh);
+
+/* ---------------------------------------------------------------------------
+ * Device mesh: the view repacked once into HBM (cell→face CSR whose rows are
+ * sorted by the mesh's face id, so per-cell accumulation follows the
+ * reference's sequential face order bit for bit, with no atomics).
+ * ------------------------------------------------------------------------- */
+typedef struct cupcfd_hip_mesh cupcfd_hip_mesh;
+
+/* Renumber cells along a Morton curve of their centres (and faces by owner)
+ * for locality. Results are returned in the mesh's numbering either way. */
+#define CUPCFD_MESH_REORDER 1
+
+/* Repacks `view` (CupCfdSoAMesh::finalize must have run: CupCfdSoAMesh.cpp:842-888).
+ * The view's pointers are kept: drop-in calls read and write the mutable
+ * mesh state through them, so they must outlive the device mesh. */
+int cupcfd_hip_mesh_create_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out);
+int cupcfd_hip_mesh_create_f32(const cupcfd_mesh_view_f32 *view, int32_t flags, cupcfd_hip_mesh **out);
+int cupcfd_hip_mesh_destroy(cupcfd_hip_mesh *mesh);
+int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t *nFaces,
+                           int32_t *nBoundaries, int32_t *nRegions);
+/* internal → mesh numbering: order[k] is the mesh id at internal position k */
+int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order);
+int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order);
+int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
+/* copy mutable mesh state view → device (push) or device → view (pull) */
+int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh);
+int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh);
+/* resident-mode layout helpers: mesh-numbered device arrays ⇄ internal order,
+ * ncomp interleaved components per element (1 for scalars, 3 for vectors,
+ * 2 for rface) */
+int cupcfd_dev_cells_to_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                 int32_t elem_bytes, void *stream);
+int cupcfd_dev_cells_from_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                   int32_t elem_bytes, void *stream);
+int cupcfd_dev_faces_to_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                 int32_t elem_bytes, void *stream);
+int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
+                                   int32_t elem_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Kernels. Declared once per precision by CUPCFD_FVM_API(T, V3, S).
+ * Reference interfaces replaced (include/fvm/implementation/component/):
+ *   gradient_phi_gauss_dolfyn   GradientPhiGaussDolfyn           GradientKernels.h:48-54
+ *   flux_mass_face_loop         FluxMassDolfynFaceLoop           MassKernels.h:48-81
+ *   flux_mass_boundary_loop1    FluxMassDolfynBoundaryLoop1      MassKernels.h:90-94
+ *   flux_mass_boundary_loop2    FluxMassDolfynBoundaryLoop2      MassKernels.h:103-108
+ *   flux_mass_boundary_loop3    FluxMassDolfynBoundaryLoop3      MassKernels.h:117-120
+ *   flux_mass_boundary_loop4    FluxMassDolfynBoundaryLoop4      MassKernels.h:129-138
+ *   flux_mass_boundary_loop5    FluxMassDolfynBoundaryLoop5      MassKernels.h:147-157
+ *   flux_mass_region_loop       FluxMassDolfynRegionLoop         MassKernels.h:166-171
+ *   flux_scalar_face_loop       FluxScalarDolfynFaceLoop         ScalarKernels.h:47-67
+ *   flux_uvw_face_loop1         FluxUVWDolfynFaceLoop1           UVWKernels.h:43-67
+ *   flux_uvw_region_loop1       FluxUVWDolfynRegionLoop1         UVWKernels.h:72-73
+ *   flux_uvw_bnds_loop1         FluxUVWDolfynBndsLoop1           UVWKernels.h:78-79
+ *   viscosity_cell_loop1        calculateViscosityDolfynCellLoop1    ViscosityKernels.h:38-48
+ *   viscosity_region_loop       calculateViscosityDolfynRegionLoop   ViscosityKernels.h:50-51
+ *   viscosity_boundary_loop     calculateViscosityDolfynBoundaryLoop ViscosityKernels.h:53-60
+ *   viscosity_cell_loop2        calculateViscosityDolfynCellLoop2    ViscosityKernels.h:62-67
+ * Argument order, meaning and lengths follow the reference declarations;
+ * bool parameters are int32_t (0 / non-zero). The mass face loop's counters
+ * icinl..icwal are host pointers in both modes.
+ * ------------------------------------------------------------------------- */
+#define CUPCFD_FVM_API(T, V3, S)                                                                   \
+/* ---- drop-in: host arrays ---- */                                                              \
+int cupcfd_fvm_gradient_phi_gauss_dolfyn_##S(cupcfd_hip_mesh *mesh, int32_t nGradient,             \
+    const T *phiCell, int32_t nPhiCell, const T *phiBoundary, int32_t nPhiBoundary,                \
+    V3 dPhidxCell, int32_t nDPhidxCell, V3 dPhidxoCell, int32_t nDPhidxoCell);                     \
+int cupcfd_fvm_flux_mass_face_loop_##S(cupcfd_hip_mesh *mesh,                                      \
+    V3 dudx, int32_t nDudx, V3 dvdx, int32_t nDvdx, V3 dwdx, int32_t nDwdx, V3 dpdx, int32_t nDpdx,\
+    const T *denCell, int32_t nDenCell, T *denBoundary, int32_t nDenBoundary,                      \
+    const T *uCell, int32_t nUCell, const T *vCell, int32_t nVCell, const T *wCell, int32_t nWCell,\
+    T *massFlux, int32_t nMassFlux, const T *p, int32_t nP, const T *ar, int32_t nAr,              \
+    T *su, int32_t nSu, T *rface, int32_t nRFace, T small,                                         \
+    int32_t *icinl, int32_t *icout, int32_t *icsym, int32_t *icwal,                                \
+    int32_t solveTurbEnergy, int32_t solveTurbDiss, int32_t solveVisc, int32_t solveEnthalpy,      \
+    const T *teCell, int32_t nTeCell, T *teBoundary, int32_t nTeBoundary,                          \
+    const T *edCell, int32_t nEdCell, T *edBoundary, int32_t nEdBoundary,                          \
+    const T *viseffCell, int32_t nViseffCell, T *viseffBoundary, int32_t nViseffBoundary,          \
+    const T *tCell, int32_t nTCell, T *tBoundary, int32_t nTBoundary);                             \
+int cupcfd_fvm_flux_mass_boundary_loop1_##S(cupcfd_hip_mesh *mesh, const T *massFlux,              \
+    int32_t nMassFlux, T *flowin);                                                                 \
+int cupcfd_fvm_flux_mass_boundary_loop2_##S(cupcfd_hip_mesh *mesh, const T *massFlux,              \
+    int32_t nMassFlux, T *flowRegion, int32_t nFlowRegion, T *flowout);                            \
+int cupcfd_fvm_flux_mass_boundary_loop3_##S(cupcfd_hip_mesh *mesh, T flowin, T *ratearea);         \
+int cupcfd_fvm_flux_mass_boundary_loop4_##S(cupcfd_hip_mesh *mesh, T *massFlux, int32_t nMassFlux, \
+    T *uBoundary, int32_t nUBoundary, T *vBoundary, int32_t nVBoundary, T *wBoundary,              \
+    int32_t nWBoundary, const T *denBoundary, int32_t nDenBoundary, T ratearea, T *flowout);       \
+int cupcfd_fvm_flux_mass_boundary_loop5_##S(cupcfd_hip_mesh *mesh, T *massFlux, int32_t nMassFlux, \
+    T *su, int32_t nSu, T *uBoundary, int32_t nUBoundary, T *vBoundary, int32_t nVBoundary,        \
+    T *wBoundary, int32_t nWBoundary, T fact, int32_t solveU, int32_t solveV, int32_t solveW,      \
+    const T *flowFact, int32_t nFlowFact, T *flowout2);                                            \
+int cupcfd_fvm_flux_mass_region_loop_##S(cupcfd_hip_mesh *mesh, T *flowFact, int32_t nFlowFact,    \
+    const T *flowRegion, int32_t nFlowRegion, T flowIn);                                           \
+int cupcfd_fvm_flux_scalar_face_loop_##S(cupcfd_hip_mesh *mesh,                                    \
+    const T *PhiCell, int32_t nPhiCell, T *PhiBoundary, int32_t nPhiBoundary,                      \
+    const T *VisEff, int32_t nVisEff, T *Au, int32_t nAu, T *Su, int32_t nSu,                      \
+    const T *Den, int32_t nDen, const T *MassFlux, int32_t nMassFlux, const T *TE, int32_t nTE,    \
+    const T *CpBoundary, int32_t nCpBoundary, const T *visEffBoundary, int32_t nVisEffBoundary,    \
+    T *RFace, int32_t nRFace, V3 dPhidx, int32_t ndPhidx, int32_t SolveTurb,                       \
+    int32_t SolveEnthalpy, T vislam, int32_t ivar, int32_t VarT, T Sigma_T, T Prandtl,             \
+    int32_t VarTE, T Sigma_k, int32_t VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend,        \
+    T Small, T Large, T TMCmu);                                                                    \
+int cupcfd_fvm_flux_uvw_face_loop1_##S(cupcfd_hip_mesh *mesh, T gammaBlend, T small,               \
+    const T *uCell, int32_t nUCell, const T *vCell, int32_t nVCell, const T *wCell, int32_t nWCell,\
+    T *uBoundary, int32_t nUBoundary, T *vBoundary, int32_t nVBoundary, T *wBoundary,              \
+    int32_t nWBoundary, const T *visEffCell, int32_t nVisEffCell, const T *visEffBoundary,         \
+    int32_t nVisEffBoundary, T *massFlux, int32_t nMassFlux, V3 dudx, int32_t nDudx,               \
+    V3 dvdx, int32_t nDvdx, V3 dwdx, int32_t nDwdx, T *rFace, int32_t nRFace,                      \
+    T *su, int32_t nSu, T *sv, int32_t nSv, T *sw, int32_t nSw,                                    \
+    T *au, int32_t nAu, T *av, int32_t nAv, T *aw, int32_t nAw);                                   \
+int cupcfd_fvm_flux_uvw_region_loop1_##S(cupcfd_hip_mesh *mesh);                                   \
+int cupcfd_fvm_flux_uvw_bnds_loop1_##S(cupcfd_hip_mesh *mesh);                                     \
+int cupcfd_fvm_viscosity_cell_loop1_##S(cupcfd_hip_mesh *mesh, T small, T cmu, T visURF,           \
+    T visLam, const T *TE, int32_t nTE, const T *ED, int32_t nED, const T *den, int32_t nDen,      \
+    T *visEff, int32_t nVisEff);                                                                   \
+int cupcfd_fvm_viscosity_region_loop_##S(T kappa, cupcfd_hip_mesh *mesh);                          \
+int cupcfd_fvm_viscosity_boundary_loop_##S(cupcfd_hip_mesh *mesh, T tmcmu, T large, T small,       \
+    T kappa, T visLam, const T *TE, int32_t nTE, const T *den, int32_t nDen,                       \
+    const T *visEffCell, int32_t nVisEffCell, T *visEffBoundary, int32_t nVisEffBoundary);         \
+int cupcfd_fvm_viscosity_cell_loop2_##S(cupcfd_hip_mesh *mesh, T visLam, T *visEffCell,            \
+    int32_t nVisEffCell, T *visEffBoundary, int32_t nVisEffBoundary);                              \
+/* ---- resident: device arrays, internal numbering, async on stream ---- */                      \
+int cupcfd_dev_gradient_phi_gauss_dolfyn_##S(cupcfd_hip_mesh *mesh, int32_t nGradient,             \
+    const T *phiCell, const T *phiBoundary, T *dPhidxCell, T *dPhidxoCell, void *stream);          \
+int cupcfd_dev_flux_mass_face_loop_##S(cupcfd_hip_mesh *mesh,                                      \
+    const T *dudx, const T *dvdx, const T *dwdx, const T *dpdx, const T *denCell, T *denBoundary,  \
+    const T *uCell, const T *vCell, const T *wCell, T *massFlux, const T *p, const T *ar,          \
+    T *su, T *rface, T small, int32_t *icinl, int32_t *icout, int32_t *icsym, int32_t *icwal,      \
+    int32_t solveTurbEnergy, int32_t solveTurbDiss, int32_t solveVisc, int32_t solveEnthalpy,      \
+    const T *teCell, T *teBoundary, const T *edCell, T *edBoundary, const T *viseffCell,           \
+    T *viseffBoundary, const T *tCell, T *tBoundary, void *stream);                                \
+int cupcfd_dev_flux_mass_boundary_loop1_##S(cupcfd_hip_mesh *mesh, const T *massFlux, T *flowin,   \
+    void *stream);                                                                                 \
+int cupcfd_dev_flux_mass_boundary_loop2_##S(cupcfd_hip_mesh *mesh, const T *massFlux,              \
+    T *flowRegion, T *flowout, void *stream);                                                      \
+int cupcfd_dev_flux_mass_boundary_loop3_##S(cupcfd_hip_mesh *mesh, T flowin, T *ratearea,          \
+    void *stream);                                                                                 \
+int cupcfd_dev_flux_mass_boundary_loop4_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *uBoundary,      \
+    T *vBoundary, T *wBoundary, const T *denBoundary, T ratearea, T *flowout, void *stream);       \
+int cupcfd_dev_flux_mass_boundary_loop5_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *su,             \
+    T *uBoundary, T *vBoundary, T *wBoundary, T fact, int32_t solveU, int32_t solveV,              \
+    int32_t solveW, const T *flowFact, T *flowout2, void *stream);                                 \
+int cupcfd_dev_flux_mass_region_loop_##S(cupcfd_hip_mesh *mesh, T *flowFact, const T *flowRegion,  \
+    T flowIn, void *stream);                                                                       \
+int cupcfd_dev_flux_scalar_face_loop_##S(cupcfd_hip_mesh *mesh,                                    \
+    const T *PhiCell, T *PhiBoundary, const T *VisEff, T *Au, T *Su, const T *Den,                 \
+    const T *MassFlux, const T *TE, const T *CpBoundary, const T *visEffBoundary, T *RFace,        \
+    const T *dPhidx, int32_t SolveTurb, int32_t SolveEnthalpy, T vislam, int32_t ivar,             \
+    int32_t VarT, T Sigma_T, T Prandtl, int32_t VarTE, T Sigma_k, int32_t VarED, T Sigma_e,        \
+    T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large, T TMCmu, void *stream);                  \
+int cupcfd_dev_flux_uvw_face_loop1_##S(cupcfd_hip_mesh *mesh, T gammaBlend, T small,               \
+    const T *uCell, const T *vCell, const T *wCell, T *uBoundary, T *vBoundary, T *wBoundary,      \
+    const T *visEffCell, const T *visEffBoundary, T *massFlux, const T *dudx, const T *dvdx,       \
+    const T *dwdx, T *rFace, T *su, T *sv, T *sw, T *au, T *av, T *aw, void *stream);              \
+int cupcfd_dev_flux_uvw_region_loop1_##S(cupcfd_hip_mesh *mesh, void *stream);                     \
+int cupcfd_dev_flux_uvw_bnds_loop1_##S(cupcfd_hip_mesh *mesh, void *stream);                       \
+int cupcfd_dev_viscosity_cell_loop1_##S(cupcfd_hip_mesh *mesh, T small, T cmu, T visURF,           \
+    T visLam, const T *TE, const T *ED, const T *den, T *visEff, void *stream);                    \
+int cupcfd_dev_viscosity_region_loop_##S(T kappa, cupcfd_hip_mesh *mesh, void *stream);            \
+int cupcfd_dev_viscosity_boundary_loop_##S(cupcfd_hip_mesh *mesh, T tmcmu, T large, T small,       \
+    T kappa, T visLam, const T *TE, const T *den, const T *visEffCell, T *visEffBoundary,          \
+    void *stream);                                                                                 \
+int cupcfd_dev_viscosity_cell_loop2_##S(cupcfd_hip_mesh *mesh, T visLam, T *visEffCell,            \
+    T *visEffBoundary, void *stream);
+
+CUPCFD_FVM_API(double, cupcfd_v3_f64, f64)
+CUPCFD_FVM_API(float, cupcfd_v3_f32, f32)
+
+/* Library version string and the gfx target it was built for. */
+const char *cupcfd_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUPCFD_HIP_H */
