@@ -59,9 +59,10 @@ class Fields:
         self.flowFact = U(0.5, 1.5, R)
 
     def cast(self, dtype):
+        """An independent copy in ``dtype`` (never aliases this object's arrays)."""
         out = Fields.__new__(Fields)
         for k, v in self.__dict__.items():
-            out.__dict__[k] = np.ascontiguousarray(v, dtype=dtype)
+            out.__dict__[k] = np.array(v, dtype=dtype, order="C", copy=True)
         return out
 
     def copy(self):

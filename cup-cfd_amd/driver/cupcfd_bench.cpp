@@ -1,0 +1,470 @@
+// cupcfd_bench — host C++ driver with the reference's BenchmarkKernels entry.
+//
+// Reads a cupcfd.json in CUP-CFD's schema (README.md:100-160;
+// examples/benchmarks.json): "DataTypes".FloatDataType selects double or
+// float, "Mesh"."MeshSource"."MeshSourceStructGen" gives the grid, and every
+// "Benchmarks"[i]."BenchmarkKernels" {"BenchmarkName", "Repetitions"} entry
+// runs the 16 fvm kernels per repetition in BenchmarkKernels::runBenchmark's
+// order (src/benchmarks/implementation/component/BenchmarkKernels.cpp:55-115)
+// with its parameter values, through the cupcfd::fvm-shaped shim
+// (cupcfd_hip_fvm.hpp) on a MiniSoA-layout mesh whose 3-vectors carry a
+// vptr as EuclideanVector does. Other benchmark types are reported and
+// skipped. Optional keys under "BenchmarkKernels" that the reference ignores:
+// "Seed" (input stream), "Permute" (relabel cell/face ids), "Typed" (six
+// typed boundary regions). Each kernel call is timed on the host around the
+// (synchronous) call, as TreeTimer times the reference's; the report is JSON
+// on stdout.
+//
+// Usage: cupcfd_bench [path/to/cupcfd.json]   (default ./cupcfd.json)
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "cupcfd_hip.h"
+#include "cupcfd_hip_fvm.hpp"
+
+// ----------------------------------------------------------------------------
+// Minimal JSON reader (objects, arrays, strings, numbers, booleans, null)
+// ----------------------------------------------------------------------------
+struct Json {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    double num = 0;
+    bool b = false;
+    std::string str;
+    std::vector<Json> arr;
+    std::vector<std::pair<std::string, Json>> obj;
+    const Json *get(const std::string &k) const {
+        for (auto &kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct Parser {
+    const std::string &s;
+    size_t i = 0;
+    bool ok = true;
+    explicit Parser(const std::string &t) : s(t) {}
+    void ws() {
+        while (i < s.size() && std::isspace((unsigned char)s[i])) i++;
+    }
+    Json value() {
+        ws();
+        Json j;
+        if (i >= s.size()) {
+            ok = false;
+            return j;
+        }
+        char c = s[i];
+        if (c == '{') {
+            j.kind = Json::OBJ;
+            i++;
+            ws();
+            if (s[i] == '}') {
+                i++;
+                return j;
+            }
+            while (ok) {
+                ws();
+                Json k = value();
+                ws();
+                if (k.kind != Json::STR || s[i] != ':') {
+                    ok = false;
+                    break;
+                }
+                i++;
+                j.obj.emplace_back(k.str, value());
+                ws();
+                if (s[i] == ',') {
+                    i++;
+                    continue;
+                }
+                if (s[i] == '}') {
+                    i++;
+                    break;
+                }
+                ok = false;
+            }
+        } else if (c == '[') {
+            j.kind = Json::ARR;
+            i++;
+            ws();
+            if (s[i] == ']') {
+                i++;
+                return j;
+            }
+            while (ok) {
+                j.arr.push_back(value());
+                ws();
+                if (s[i] == ',') {
+                    i++;
+                    continue;
+                }
+                if (s[i] == ']') {
+                    i++;
+                    break;
+                }
+                ok = false;
+            }
+        } else if (c == '"') {
+            j.kind = Json::STR;
+            i++;
+            while (i < s.size() && s[i] != '"') {
+                if (s[i] == '\\' && i + 1 < s.size()) i++;
+                j.str += s[i++];
+            }
+            i++;
+        } else if (s.compare(i, 4, "true") == 0) {
+            j.kind = Json::BOOL;
+            j.b = true;
+            i += 4;
+        } else if (s.compare(i, 5, "false") == 0) {
+            j.kind = Json::BOOL;
+            i += 5;
+        } else if (s.compare(i, 4, "null") == 0) {
+            i += 4;
+        } else {
+            j.kind = Json::NUM;
+            char *end = nullptr;
+            j.num = std::strtod(s.c_str() + i, &end);
+            if (end == s.c_str() + i) ok = false;
+            i = end - s.c_str();
+        }
+        return j;
+    }
+};
+
+// ----------------------------------------------------------------------------
+// MiniSoA-layout mesh: the members CupCfdSoAMesh exposes, with vptr-bearing
+// 3-vectors (CupCfdSoAMesh.h:71-154, EuclideanVector.h:40,57)
+// ----------------------------------------------------------------------------
+template <class T> struct Vec3 {
+    virtual ~Vec3() {}
+    T cmp[3] = {0, 0, 0};
+};
+enum RType { RTYPE_DEFAULT, RTYPE_INLET, RTYPE_OUTLET, RTYPE_SYMP, RTYPE_WALL };
+
+template <class T> struct SoAMesh {
+    struct {
+        int lTCells = 0, lFaces = 0, lBoundaries = 0, lRegions = 0;
+    } properties;
+    std::vector<int> faceCell1ID, faceCell2ID, faceBoundaryID, boundaryFaceID, boundaryRegionID;
+    std::vector<T> faceLambda, faceRLencos, faceArea, cellVolume, boundaryDistance, boundaryYPlus, boundaryUPlus,
+        boundaryQ, boundaryH, boundaryT, regionYLog, regionELog, regionDensity, regionTurbKE, regionTurbDiss,
+        regionSplvl, regionDen, regionR, regionT;
+    std::vector<Vec3<T>> faceNorm, faceCenter, faceXpac, faceXnac, cellCenter, boundaryShear, regionForceTangent,
+        regionUVW;
+    std::vector<RType> regionType;
+    std::vector<bool> regionStd, regionFlux, regionAdiab;
+};
+
+template <class T> void copy3(std::vector<Vec3<T>> &dst, const cupcfd_v3_f64 &v, int n) {
+    dst.resize(n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) dst[i].cmp[k] = (T)v.base[3 * (size_t)i + k];
+}
+template <class T, class S> void copy1(std::vector<T> &dst, const S *src, int n) {
+    dst.resize(n);
+    for (int i = 0; i < n; i++) dst[i] = (T)src[i];
+}
+
+template <class T> int load_mesh(SoAMesh<T> &m, const cupcfd_structgen_params &p) {
+    cupcfd_host_mesh *h = nullptr;
+    int rc = cupcfd_structgen_create(&p, &h);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    cupcfd_mesh_view_f64 v;
+    cupcfd_host_mesh_view_f64(h, &v);
+    const int C = v.nCells, F = v.nFaces, B = v.nBoundaries, R = v.nRegions;
+    m.properties = {C, F, B, R};
+    copy1(m.faceCell1ID, v.faceCell1ID, F);
+    copy1(m.faceCell2ID, v.faceCell2ID, F);
+    copy1(m.faceBoundaryID, v.faceBoundaryID, F);
+    copy1(m.faceLambda, v.faceLambda, F);
+    copy1(m.faceRLencos, v.faceRLencos, F);
+    copy1(m.faceArea, v.faceArea, F);
+    copy3(m.faceNorm, v.faceNorm, F);
+    copy3(m.faceCenter, v.faceCenter, F);
+    copy3(m.faceXpac, v.faceXpac, F);
+    copy3(m.faceXnac, v.faceXnac, F);
+    copy3(m.cellCenter, v.cellCenter, C);
+    copy1(m.cellVolume, v.cellVolume, C);
+    copy1(m.boundaryFaceID, v.boundaryFaceID, B);
+    copy1(m.boundaryRegionID, v.boundaryRegionID, B);
+    copy1(m.boundaryDistance, v.boundaryDistance, B);
+    copy1(m.boundaryYPlus, v.boundaryYPlus, B);
+    copy1(m.boundaryUPlus, v.boundaryUPlus, B);
+    copy1(m.boundaryQ, v.boundaryQ, B);
+    copy1(m.boundaryH, v.boundaryH, B);
+    copy1(m.boundaryT, v.boundaryT, B);
+    copy3(m.boundaryShear, v.boundaryShear, B);
+    m.regionType.resize(R);
+    m.regionStd.resize(R);
+    m.regionFlux.resize(R);
+    m.regionAdiab.resize(R);
+    for (int r = 0; r < R; r++) {
+        m.regionType[r] = (RType)v.regionType[r];
+        m.regionStd[r] = v.regionStd[r] != 0;
+        m.regionFlux[r] = v.regionFlux[r] != 0;
+        m.regionAdiab[r] = v.regionAdiab[r] != 0;
+    }
+    copy1(m.regionYLog, v.regionYLog, R);
+    copy1(m.regionELog, v.regionELog, R);
+    copy1(m.regionDensity, v.regionDensity, R);
+    copy1(m.regionTurbKE, v.regionTurbKE, R);
+    copy1(m.regionTurbDiss, v.regionTurbDiss, R);
+    copy1(m.regionSplvl, v.regionSplvl, R);
+    copy1(m.regionDen, v.regionDen, R);
+    copy1(m.regionR, v.regionR, R);
+    copy1(m.regionT, v.regionT, R);
+    copy3(m.regionForceTangent, v.regionForceTangent, R);
+    copy3(m.regionUVW, v.regionUVW, R);
+    cupcfd_host_mesh_destroy(h);
+    return CUPCFD_E_SUCCESS;
+}
+
+// ----------------------------------------------------------------------------
+// BenchmarkKernels: 16 wrappers, each timing one call
+// ----------------------------------------------------------------------------
+template <class T> struct Bench {
+    SoAMesh<T> &mesh;
+    std::mt19937_64 rng;
+    std::map<std::string, std::vector<double>> times;
+    int failures = 0;
+    Bench(SoAMesh<T> &m, uint64_t seed) : mesh(m), rng(seed) {}
+
+    // randomUniform(1e-6, 1e-2) as the reference's wrappers fill their inputs
+    // (include/utility/implementation/component/ArrayKernels.ipp:307-315), seeded
+    std::vector<T> U(int n, double lo = 1e-6, double hi = 1e-2) {
+        std::uniform_real_distribution<double> d(lo, hi);
+        std::vector<T> v(n);
+        for (auto &x : v) x = (T)d(rng);
+        return v;
+    }
+    std::vector<Vec3<T>> U3(int n) {
+        std::uniform_real_distribution<double> d(1e-6, 1e-2);
+        std::vector<Vec3<T>> v(n);
+        for (auto &x : v)
+            for (int k = 0; k < 3; k++) x.cmp[k] = (T)d(rng);
+        return v;
+    }
+    template <class F> void timed(const char *name, F &&f) {
+        auto t0 = std::chrono::steady_clock::now();
+        int rc = f();
+        auto t1 = std::chrono::steady_clock::now();
+        times[name].push_back(std::chrono::duration<double>(t1 - t0).count());
+        if (rc != CUPCFD_E_SUCCESS) {
+            failures++;
+            std::fprintf(stderr, "%s returned %d\n", name, rc);
+        }
+    }
+
+    void repetition() {
+        namespace fvm = cupcfd_hip::fvm;
+        const int C = mesh.properties.lTCells, F = mesh.properties.lFaces, B = mesh.properties.lBoundaries,
+                  R = mesh.properties.lRegions;
+        {  // BenchmarkKernels.cpp:117-194
+            auto phi = U(C), phiB = U(B);
+            auto g = U3(C), go = U3(C);
+            timed("GradientPhiGaussDolfynBenchmark", [&] {
+                return fvm::GradientPhiGaussDolfyn(mesh, 1, phi.data(), C, phiB.data(), B, g.data(), C, go.data(), C);
+            });
+        }
+        std::vector<T> massFlux = U(F);
+        {  // :196-326
+            auto dudx = U3(C), dvdx = U3(C), dwdx = U3(C), dpdx = U3(C);
+            auto den = U(C), denB = U(B), u = U(C), v = U(C), w = U(C), p = U(C), ar = U(C), su = U(C),
+                 rface = U(2 * F), te = U(C), teB = U(B), ed = U(C), edB = U(B), vis = U(C), visB = U(B), t = U(C),
+                 tB = U(B);
+            int icinl = 0, icout = 0, icsym = 0, icwal = 0;
+            timed("FluxMassDolfynFaceLoopBenchmark", [&] {
+                return fvm::FluxMassDolfynFaceLoop(
+                    mesh, dudx.data(), C, dvdx.data(), C, dwdx.data(), C, dpdx.data(), C, den.data(), C, denB.data(),
+                    B, u.data(), C, v.data(), C, w.data(), C, massFlux.data(), F, p.data(), C, ar.data(), C,
+                    su.data(), C, rface.data(), 2 * F, (T)1e-18, &icinl, &icout, &icsym, &icwal, false, false, true,
+                    false, te.data(), C, teB.data(), B, ed.data(), C, edB.data(), B, vis.data(), C, visB.data(), B,
+                    t.data(), C, tB.data(), B);
+            });
+        }
+        std::vector<T> flowRegion = U(R), flowFact = U(R);
+        {
+            T flowin = 0, flowout = 0, ratearea = 0, flowout2 = 0;
+            timed("FluxMassDolfynBoundaryLoop1Benchmark",
+                  [&] { return fvm::FluxMassDolfynBoundaryLoop1(mesh, massFlux.data(), F, &flowin); });
+            timed("FluxMassDolfynBoundaryLoop2Benchmark", [&] {
+                return fvm::FluxMassDolfynBoundaryLoop2(mesh, massFlux.data(), F, flowRegion.data(), R, &flowout);
+            });
+            timed("FluxMassDolfynBoundaryLoop3Benchmark",
+                  [&] { return fvm::FluxMassDolfynBoundaryLoop3(mesh, (T)1.1, &ratearea); });
+            auto uB = U(B), vB = U(B), wB = U(B), denB = U(B);
+            timed("FluxMassDolfynBoundaryLoop4Benchmark", [&] {
+                return fvm::FluxMassDolfynBoundaryLoop4(mesh, massFlux.data(), F, uB.data(), B, vB.data(), B,
+                                                        wB.data(), B, denB.data(), B, (T)1.0, &flowout);
+            });
+            auto su = U(C);
+            timed("FluxMassDolfynBoundaryLoop5Benchmark", [&] {
+                return fvm::FluxMassDolfynBoundaryLoop5(mesh, massFlux.data(), F, su.data(), C, uB.data(), B,
+                                                        vB.data(), B, wB.data(), B, (T)1.1, true, true, true,
+                                                        flowFact.data(), R, &flowout2);
+            });
+            timed("FluxMassDolfynRegionLoopBenchmark", [&] {
+                return fvm::FluxMassDolfynRegionLoop(mesh, flowFact.data(), R, flowRegion.data(), R, (T)0.0);
+            });
+        }
+        {  // :634-738, parameter values :677-695
+            auto phi = U(C), phiB = U(B), vis = U(C), au = U(C), su = U(C), den = U(C), te = U(C), cpB = U(B),
+                 visB = U(B), rf = U(2 * F);
+            auto g = U3(C);
+            timed("FluxScalarDolfynFaceLoopBenchmark", [&] {
+                return fvm::FluxScalarDolfynFaceLoop(
+                    mesh, phi.data(), C, phiB.data(), B, vis.data(), C, au.data(), C, su.data(), C, den.data(), C,
+                    massFlux.data(), F, te.data(), C, cpB.data(), B, visB.data(), B, rf.data(), 2 * F, g.data(), C,
+                    true, false, (T)0.001, 1, 2, (T)1.4, (T)1.2e-3, 3, (T)1.5, 4, (T)1.6, (T)1.7, (T)1.4e-3, (T)1.1,
+                    (T)1e-18, (T)1e18, (T)1.8);
+            });
+        }
+        {  // :740-880, :761-762
+            auto u = U(C), v = U(C), w = U(C), uB = U(B), vB = U(B), wB = U(B), vis = U(C), visB = U(B),
+                 rf = U(2 * F), su = U(C), sv = U(C), sw = U(C), au = U(C), av = U(C), aw = U(C);
+            auto du = U3(C), dv = U3(C), dw = U3(C);
+            timed("FluxUVWDolfynFaceLoop1Benchmark", [&] {
+                return fvm::FluxUVWDolfynFaceLoop1(mesh, (T)0.9, (T)1e-18, u.data(), C, v.data(), C, w.data(), C,
+                                                   uB.data(), B, vB.data(), B, wB.data(), B, vis.data(), C,
+                                                   visB.data(), B, massFlux.data(), F, du.data(), C, dv.data(), C,
+                                                   dw.data(), C, rf.data(), 2 * F, su.data(), C, sv.data(), C,
+                                                   sw.data(), C, au.data(), C, av.data(), C, aw.data(), C);
+            });
+        }
+        timed("FluxUVWDolfynRegionLoop1Benchmark", [&] { return fvm::FluxUVWDolfynRegionLoop1<T>(mesh); });
+        timed("FluxUVWDolfynBndsLoop1Benchmark", [&] { return fvm::FluxUVWDolfynBndsLoop1<T>(mesh); });
+        {  // :938-1133
+            auto te = U(C), ed = U(C), den = U(C), vis = U(C), visB = U(B);
+            timed("calculateViscosityDolfynCellLoop1Benchmark", [&] {
+                return fvm::calculateViscosityDolfynCellLoop1(mesh, (T)1e-18, (T)1e-4, (T)0.232, (T)0.00547,
+                                                              te.data(), C, ed.data(), C, den.data(), C, vis.data(),
+                                                              C);
+            });
+            timed("calculateViscosityDolfynRegionLoopBenchmark",
+                  [&] { return fvm::calculateViscosityDolfynRegionLoop((T)1.1, mesh); });
+            timed("calculateViscosityDolfynBoundaryLoopBenchmark", [&] {
+                return fvm::calculateViscosityDolfynBoundaryLoop(mesh, (T)1.0, (T)1e18, (T)1e-18, (T)1.1, (T)1.2,
+                                                                 te.data(), C, den.data(), C, vis.data(), C,
+                                                                 visB.data(), B);
+            });
+            timed("calculateViscosityDolfynCellLoop2Benchmark", [&] {
+                return fvm::calculateViscosityDolfynCellLoop2(mesh, (T)1.1, vis.data(), C, visB.data(), B);
+            });
+        }
+    }
+};
+
+static double num(const Json *o, const char *k, double dflt) {
+    const Json *v = o ? o->get(k) : nullptr;
+    return v && v->kind == Json::NUM ? v->num : (v && v->kind == Json::BOOL ? (v->b ? 1.0 : 0.0) : dflt);
+}
+
+template <class T> int run_all(const Json &cfg, const cupcfd_structgen_params &base) {
+    const Json *benches = cfg.get("Benchmarks");
+    if (!benches || benches->kind != Json::ARR) {
+        std::fprintf(stderr, "cupcfd.json: no Benchmarks array\n");
+        return CUPCFD_E_CONFIG_INVALID_VALUE;  // E_CONFIG_OPT_NOT_FOUND in the reference
+    }
+    std::printf("{\"benchmarks\": [");
+    bool first = true;
+    int status = CUPCFD_E_SUCCESS;
+    for (const Json &b : benches->arr) {
+        const Json *bk = b.get("BenchmarkKernels");
+        if (!bk) {
+            std::fprintf(stderr, "skipping %s (not part of this build)\n",
+                         b.obj.empty() ? "?" : b.obj[0].first.c_str());
+            continue;
+        }
+        const Json *nm = bk->get("BenchmarkName");
+        const Json *reps = bk->get("Repetitions");
+        if (!nm || nm->kind != Json::STR || !reps || reps->kind != Json::NUM) {
+            std::fprintf(stderr, "BenchmarkKernels needs BenchmarkName and Repetitions\n");
+            return CUPCFD_E_CONFIG_INVALID_VALUE;
+        }
+        cupcfd_structgen_params p = base;
+        p.seed = (uint64_t)num(bk, "Seed", (double)p.seed);
+        p.permute_cells = p.permute_faces = (int)num(bk, "Permute", 0);
+        p.regions = (int)num(bk, "Typed", 0);
+        SoAMesh<T> mesh;
+        int rc = load_mesh(mesh, p);
+        if (rc != CUPCFD_E_SUCCESS) return rc;
+        Bench<T> bench(mesh, p.seed ^ 0x5eedull);
+        const int R = (int)reps->num;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < R; r++) bench.repetition();
+        double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("%s{\"BenchmarkName\": \"%s\", \"Repetitions\": %d, \"LocalCells\": %d, \"LocalFaces\": %d, "
+                    "\"LocalBounds\": %d, \"total_s\": %.6f, \"failures\": %d, \"kernels\": {",
+                    first ? "" : ", ", nm->str.c_str(), R, mesh.properties.lTCells, mesh.properties.lFaces,
+                    mesh.properties.lBoundaries, total, bench.failures);
+        bool fk = true;
+        for (auto &kv : bench.times) {
+            double s = 0;
+            for (double t : kv.second) s += t;
+            std::printf("%s\"%s\": {\"calls\": %zu, \"mean_s\": %.9f}", fk ? "" : ", ", kv.first.c_str(),
+                        kv.second.size(), s / kv.second.size());
+            fk = false;
+        }
+        std::printf("}}");
+        first = false;
+        cupcfd_hip::shim::forget(&mesh);
+        if (bench.failures) status = CUPCFD_E_ERROR;
+    }
+    std::printf("], \"library\": \"%s\"}\n", cupcfd_hip_version());
+    return status;
+}
+
+int main(int argc, char **argv) {
+    const char *path = argc > 1 ? argv[1] : "./cupcfd.json";
+    std::ifstream in(path);
+    if (!in) {
+        std::fprintf(stderr, "cannot open %s\n", path);
+        return 1;
+    }
+    std::stringstream ss;
+    ss << in.rdbuf();
+    std::string text = ss.str();
+    Parser ps(text);
+    Json cfg = ps.value();
+    if (!ps.ok || cfg.kind != Json::OBJ) {
+        std::fprintf(stderr, "%s: malformed JSON\n", path);
+        return 1;
+    }
+    const Json *dt = cfg.get("DataTypes");
+    const Json *fdt = dt ? dt->get("FloatDataType") : nullptr;
+    const Json *mdt = dt ? dt->get("MeshDataType") : nullptr;
+    if (mdt && mdt->str == "MiniAoS")
+        std::fprintf(stderr, "MeshDataType MiniAoS: running the MiniSoA layout this build provides\n");
+    const Json *mesh = cfg.get("Mesh");
+    const Json *src = mesh ? mesh->get("MeshSource") : nullptr;
+    const Json *sg = src ? src->get("MeshSourceStructGen") : nullptr;
+    if (!sg) {
+        std::fprintf(stderr, "only Mesh.MeshSource.MeshSourceStructGen is supported\n");
+        return 1;
+    }
+    cupcfd_structgen_params p{};
+    p.nx = (int)num(sg, "CellX", 0);
+    p.ny = (int)num(sg, "CellY", 0);
+    p.nz = (int)num(sg, "CellZ", 0);
+    p.xmin = num(sg, "SpatialXMin", 0);
+    p.xmax = num(sg, "SpatialXMax", 1);
+    p.ymin = num(sg, "SpatialYMin", 0);
+    p.ymax = num(sg, "SpatialYMax", 1);
+    p.zmin = num(sg, "SpatialZMin", 0);
+    p.zmax = num(sg, "SpatialZMax", 1);
+    p.seed = 1;
+    int rc = (fdt && fdt->str == "float") ? run_all<float>(cfg, p) : run_all<double>(cfg, p);
+    return rc == CUPCFD_E_SUCCESS ? 0 : 2;
+}

@@ -1,0 +1,41 @@
+"""The BenchmarkKernels-style C++ driver (cup-cfd_amd/cupcfd_bench) runs the
+16 kernels through the cupcfd::fvm-shaped shim on a reference-schema JSON."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "cup-cfd_amd", "cupcfd_bench")
+CFG = os.path.join(ROOT, "cup-cfd_amd", "driver", "cupcfd.json")
+
+KERNELS = ["GradientPhiGaussDolfynBenchmark", "FluxMassDolfynFaceLoopBenchmark",
+           "FluxMassDolfynBoundaryLoop1Benchmark", "FluxMassDolfynBoundaryLoop2Benchmark",
+           "FluxMassDolfynBoundaryLoop3Benchmark", "FluxMassDolfynBoundaryLoop4Benchmark",
+           "FluxMassDolfynBoundaryLoop5Benchmark", "FluxMassDolfynRegionLoopBenchmark",
+           "FluxScalarDolfynFaceLoopBenchmark", "FluxUVWDolfynFaceLoop1Benchmark", "FluxUVWDolfynRegionLoop1Benchmark",
+           "FluxUVWDolfynBndsLoop1Benchmark", "calculateViscosityDolfynCellLoop1Benchmark",
+           "calculateViscosityDolfynRegionLoopBenchmark", "calculateViscosityDolfynBoundaryLoopBenchmark",
+           "calculateViscosityDolfynCellLoop2Benchmark"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_driver_runs_reference_schema(tmp_path, precision):
+    cfg = json.load(open(CFG))
+    cfg["DataTypes"]["FloatDataType"] = precision
+    p = tmp_path / "cupcfd.json"
+    p.write_text(json.dumps(cfg))
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout)
+    names = [b["BenchmarkName"] for b in rep["benchmarks"]]
+    assert names == ["KernelTest", "KernelTestTypedPermuted"]  # BenchmarkExchange is skipped
+    assert "BenchmarkExchange" in r.stderr
+    for b in rep["benchmarks"]:
+        assert b["failures"] == 0
+        assert b["LocalCells"] == 11 * 12 * 14
+        assert sorted(b["kernels"]) == sorted(KERNELS)
+        for k in KERNELS:
+            assert b["kernels"][k]["calls"] == b["Repetitions"]

@@ -103,3 +103,45 @@ def test_deterministic_repeat():
     r2 = run("uvw", fvm, dev, f.copy(), lambda: host.state(64))[1]
     for k in r1:
         assert np.array_equal(r1[k], r2[k]), k
+
+
+@pytest.mark.gpu
+def test_strided_vectors_match_packed():
+    """3-vectors laid out as the reference's EuclideanVector<double,3> (an 8-byte
+    vptr slot, then cmp[3]: 32-byte stride) give the packed-layout results."""
+    host = ch.HostMesh(**MESHES["typed_perm"])
+    dev = ch.DeviceMesh(host, 64)
+    f = W.Fields(host, seed=31)
+    C = host.nCells
+
+    def strided(a):
+        s = np.full((a.shape[0], 4), np.nan)
+        s[:, 1:] = a
+        return s
+
+    g_p, go_p = np.zeros((C, 3)), np.zeros((C, 3))
+    assert fvm.GradientPhiGaussDolfyn(dev, 1, f.phi, f.phiB, g_p, go_p) == ch.E_SUCCESS
+    g_s, go_s = strided(np.zeros((C, 3))), strided(np.zeros((C, 3)))
+    L = ch.lib()
+    V = ch.V3F64
+    rc = L.cupcfd_fvm_gradient_phi_gauss_dolfyn_f64(
+        dev.handle, 1, f.phi.ctypes.data_as(ch.ct.c_void_p), C, f.phiB.ctypes.data_as(ch.ct.c_void_p),
+        host.nBoundaries, V(g_s.ctypes.data + 8, 32), C, V(go_s.ctypes.data + 8, 32), C)
+    assert rc == ch.E_SUCCESS
+    assert np.array_equal(g_s[:, 1:], g_p) and np.array_equal(go_s[:, 1:], go_p)
+    assert np.isnan(g_s[:, 0]).all()  # the vptr slot is never written
+
+
+@pytest.mark.gpu
+def test_size_errors_are_reported():
+    """Arrays shorter than the mesh needs: E_INVALID_INDEX, as the reference's
+    bounds checks return (ScalarKernels.ipp:102-124)."""
+    host, dev = _setup("ref5", 64)
+    f = W.Fields(host, seed=1)
+    g, go = np.zeros((host.nCells, 3)), np.zeros((host.nCells, 3))
+    assert fvm.GradientPhiGaussDolfyn(dev, 1, f.phi[:-1].copy(), f.phiB, g, go) == ch.E_INVALID_INDEX
+    assert fvm.GradientPhiGaussDolfyn(dev, 1, f.phi, f.phiB, g[:-1].copy(), go) == ch.E_INVALID_INDEX
+    # longer arrays are fine; entries past the mesh's cells are zeroed like the reference's reset loop
+    g2, go2 = np.full((host.nCells + 4, 3), 5.0), np.full((host.nCells + 4, 3), 5.0)
+    assert fvm.GradientPhiGaussDolfyn(dev, 1, f.phi, f.phiB, g2, go2) == ch.E_SUCCESS
+    assert (g2[host.nCells:] == 0).all() and (go2[host.nCells:] == 0).all()
