@@ -265,6 +265,8 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     d.B = B;
     d.R = R;
     d.nnz = nnz;
+    d.maxrow = 0;
+    for (int c = 0; c < C; c++) d.maxrow = std::max(d.maxrow, row[c + 1] - row[c]);
     d.fconn = up.put(fconn.data(), F);
     d.fgeoS = up.put(fS.data(), F);
     d.fgeoX = up.put(fX.data(), F);
@@ -515,4 +517,13 @@ extern "C" int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh) {
 extern "C" int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
     return sync_state_any(mesh, false);
+}
+
+extern "C" int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 8)
+        return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if (mesh->precision == 64) mesh->m64->schedule_lpc = lanes_per_cell;
+    else mesh->m32->schedule_lpc = lanes_per_cell;
+    return CUPCFD_E_SUCCESS;
 }

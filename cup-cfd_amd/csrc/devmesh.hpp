@@ -32,6 +32,7 @@ template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
 
 template <class T> struct DevMesh {
     int C, F, Fi, B, R, nnz;
+    int maxrow;  // longest cell→face row (faces per cell)
     const int2 *fconn;
     const V4<T> *fgeoS, *fgeoX, *fgeoP, *fgeoN;
     const V4<T> *cgeo;
@@ -77,6 +78,8 @@ template <class T> struct Mesh {
     std::vector<int32_t> fconn_h;                     // 2F, host copy for carry precompute
     std::vector<int32_t> btype_h;                     // type per boundary
     int32_t ntype[5] = {0, 0, 0, 0, 0};               // boundary faces per region type
+    // gather schedule: lanes per cell (0 = automatic, 1 = one thread per cell, 4, 8)
+    int schedule_lpc = 0;
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
     // workspace pool for drop-in staging and multi-sweep scratch

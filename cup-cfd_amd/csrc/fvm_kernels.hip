@@ -97,46 +97,116 @@ template <class T, class Get> __device__ T wave_seq_sum(T acc, int n, Get get) {
 }
 
 // ===========================================================================
-// a1  GradientPhiGaussDolfyn — GradientKernels.ipp:38-136
-// One thread per cell. SWEEP0: dPhidxo is still the zero field of the first
-// sweep, so the face-centre correction dot(0, Xf - xac) adds +0 and is
-// skipped together with its centre and gradient loads.
+// Per-cell gathers: the face loops that scatter into both cells of a face
+// (gradient, scalar, UVW) are evaluated per cell over its CSR row.
+//
+// Each face contributes a fixed sequence of additions to each accumulated
+// cell output: the reference's `x = ((x - a) + b) - c` is `((x + (-a)) + b)
+// + (-c)`, bit for bit, since IEEE negation is exact. A face therefore yields
+// a few "terms"; a step with no third operation uses -0.0, for which x + -0.0
+// == x holds exactly for every x. Folding the terms of a cell's faces in row
+// order (the reference's face order) reproduces the sequential loop.
+//
+// Two launch shapes share the per-face term functions:
+//   serial: one thread per cell, folding as it walks the row (any row length);
+//   vector: LPC lanes per cell, lane j evaluating the j-th face of the row so
+//           all of a cell's face loads are in flight at once; the terms go
+//           through LDS and one lane per output component folds them in order.
+// Both use XCD-aware block numbering: the GPU hands workgroups to its 8 XCDs
+// round-robin, so logical block (b % 8) * per + b / 8 gives each XCD (and its
+// private L2) one contiguous range of the locality-ordered cells.
 // ===========================================================================
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int per) { return (b % kXcd) * per + b / kXcd; }
+
+template <class T> __device__ __forceinline__ T nzero() { return (T)(-0.0); }
+
+// ===========================================================================
+// a1  GradientPhiGaussDolfyn — GradientKernels.ipp:38-136
+// SWEEP0: dPhidxo is still the zero field of the first sweep, so the
+// face-centre correction dot(0, Xf - xac) adds +0 and is skipped together
+// with its centre and gradient loads.
+// ===========================================================================
+template <class T> struct GradArgs {
+    const T *phi, *phiB, *grado;
+    T *out1, *out2;
+};
+
 template <class T, bool SWEEP0>
-__global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, const T *__restrict__ phi,
-                                                     const T *__restrict__ phiB, const T *__restrict__ grado,
-                                                     T *__restrict__ out1, T *__restrict__ out2) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= m.C) return;
-    T gx = 0, gy = 0, gz = 0;
-    const int e1 = m.row[c + 1];
-    for (int e = m.row[c]; e < e1; e++) {
-        const int f = m.ent[e] >> 1;
-        const int2 cn = m.fconn[f];
-        const V4<T> s = m.fgeoS[f];
-        T phiF;
-        if (cn.y >= 0) {
-            const int ip = cn.x, in = cn.y;
-            const T facn = s.w;
-            const T facp = 1.0 - facn;
-            phiF = (phi[in] * facn) + (phi[ip] * facp);
-            if (!SWEEP0) {
-                const v3<T> xac = lerp3(xyz(m.cgeo[in]), facn, xyz(m.cgeo[ip]), facp);
-                const v3<T> g = lerp3(ld3(grado, in), facn, ld3(grado, ip), facp);
-                phiF += dot3(g, sub3(xyz(m.fgeoX[f]), xac));
-            }
-        } else {
-            phiF = phiB[~cn.y];
+__device__ __forceinline__ void gradient_terms(const DevMesh<T> &m, const GradArgs<T> &a, int ent, T t[3]) {
+    const int f = ent >> 1;
+    const int2 cn = m.fconn[f];
+    const V4<T> s = m.fgeoS[f];
+    T phiF;
+    if (cn.y >= 0) {
+        const int ip = cn.x, in = cn.y;
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        phiF = (a.phi[in] * facn) + (a.phi[ip] * facp);
+        if (!SWEEP0) {
+            const v3<T> xac = lerp3(xyz(m.cgeo[in]), facn, xyz(m.cgeo[ip]), facp);
+            const v3<T> g = lerp3(ld3(a.grado, in), facn, ld3(a.grado, ip), facp);
+            phiF += dot3(g, sub3(xyz(m.fgeoX[f]), xac));
         }
-        gx += phiF * s.x;
-        gy += phiF * s.y;
-        gz += phiF * s.z;
+    } else {
+        phiF = a.phiB[~cn.y];
     }
+    t[0] = phiF * s.x;
+    t[1] = phiF * s.y;
+    t[2] = phiF * s.z;
+}
+
+template <class T> __device__ __forceinline__ void gradient_store(const DevMesh<T> &m, const GradArgs<T> &a, int c,
+                                                                  int k, T acc) {
     const T vol = m.cvol[c];
     const T fact = 1.0 / vol;
-    const v3<T> r{gx * fact, gy * fact, gz * fact};
-    st3(out1, c, r);
-    if (out2) st3(out2, c, r);
+    const T r = acc * fact;
+    a.out1[3 * (size_t)c + k] = r;
+    if (a.out2) a.out2[3 * (size_t)c + k] = r;
+}
+
+template <class T, bool SWEEP0>
+__global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int c = lb * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T acc[3] = {0, 0, 0};
+    const int e1 = m.row[c + 1];
+    for (int e = m.row[c]; e < e1; e++) {
+        T t[3];
+        gradient_terms<T, SWEEP0>(m, a, m.ent[e], t);
+        acc[0] = acc[0] + t[0];
+        acc[1] = acc[1] + t[1];
+        acc[2] = acc[2] + t[2];
+    }
+    for (int k = 0; k < 3; k++) gradient_store(m, a, c, k, acc[k]);
+}
+
+template <class T, bool SWEEP0, int LPC>
+__global__ __launch_bounds__(kBlock) void k_gradient_vec(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kBlock / LPC, NT = 3, P = LPC + 1;
+    __shared__ T lds[NT * CPB * P];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
+    const int c = lb * CPB + cl;
+    int e0 = 0, n = 0;
+    if (c < m.C) {
+        e0 = m.row[c];
+        n = m.row[c + 1] - e0;
+    }
+    if (j < n) {
+        T t[NT];
+        gradient_terms<T, SWEEP0>(m, a, m.ent[e0 + j], t);
+        for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
+    }
+    __syncthreads();
+    if (c < m.C && j < 3) {
+        T acc = 0;
+        for (int i = 0; i < n; i++) acc = acc + lds[(j * CPB + cl) * P + i];
+        gradient_store(m, a, c, j, acc);
+    }
 }
 
 // ===========================================================================
@@ -435,106 +505,174 @@ template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarAr
     return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
 }
 
+// Terms of one face of the scalar loop for the cell the entry belongs to:
+// t[0..2] the three additions to Su, t[3] the addition to Au. Returns
+// whether the face updates Au (INLET, enthalpy WALL). Per-face outputs
+// (RFace, PhiB, mesh Q/H/T) are written here by the face's owner.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_scalar(DevMesh<T> m, ScalarArgs<T> a) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarArgs<T> &a, int ent, T t[4]) {
+    const int f = ent >> 1;
+    const int2 cn = m.fconn[f];
+    const V4<T> s = m.fgeoS[f];
+    const v3<T> S{s.x, s.y, s.z};
+    const T mf = a.massFlux[f];
+    t[0] = t[1] = t[2] = t[3] = nzero<T>();
+    if (cn.y >= 0) {
+        const int ip = cn.x, in = cn.y;
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        T Visac = a.visEff[in] * facn + a.visEff[ip] * facp;
+        Visac = scalar_visac(a, Visac);
+        const v3<T> g = lerp3(ld3(a.dPhidx, in), facn, ld3(a.dPhidx, ip), facp);
+        const v3<T> Xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
+        const T VisFace = Visac * m.fgeoX[f].w;
+        const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
+        const T fce = mf * PhiFace;
+        const T fde1 = Visac * dot3(g, S);
+        const T fci = fmin((double)mf, 0.0) * (double)a.phi[in] + fmax((double)mf, 0.0) * (double)a.phi[ip];
+        const T fdi = VisFace * dot3(g, Xpn);
+        const T blend = a.gammaBlend * (fce - fci);
+        if ((ent & 1) == 0) {
+            // Su[ip] = ((Su[ip] - blend) + fde1) - fdi
+            t[0] = -blend;
+            t[1] = fde1;
+            t[2] = -fdi;
+            a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
+            a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
+        } else {
+            // Su[in] = ((Su[in] + blend) - fde1) + fdi
+            t[0] = blend;
+            t[1] = -fde1;
+            t[2] = fdi;
+        }
+        return false;
+    }
+    const int ip = cn.x;
+    const int ib = ~cn.y;
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    if (it == CUPCFD_RTYPE_INLET) {
+        const v3<T> g = ld3(a.dPhidx, ip);
+        const T PhiFace = a.inletAssigns ? (T)(0) : scalar_carry(m, a, f);
+        const T Visac = scalar_visac(a, a.visEffB[ib]);
+        const V4<T> xf = m.fgeoX[f];
+        const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
+        const T VisFace = Visac * xf.w;
+        const T fde = Visac * dot3(g, S);
+        const T fdi = VisFace * dot3(g, Xpn);
+        const T fF = -VisFace + fmin((double)mf, 0.0);
+        // Au = Au - f;  Su = ((Su + fde) - fdi) - f * PhiFace
+        t[0] = fde;
+        t[1] = -fdi;
+        t[2] = -(fF * PhiFace);
+        t[3] = -fF;
+        a.phiB[ib] = PhiFace;
+        return true;
+    }
+    if (it == CUPCFD_RTYPE_OUTLET) {
+        const v3<T> g = ld3(a.dPhidx, ip);
+        const T Visac = scalar_visac(a, a.visEff[ip]);
+        const V4<T> xf = m.fgeoX[f];
+        const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
+        const T PhiFace = a.phi[ip] + dot3(g, Xpn);
+        const T VisFace = Visac * xf.w;
+        // Su = (Su + fde) - fdi
+        t[0] = Visac * dot3(g, S);
+        t[1] = -(VisFace * dot3(g, Xpn));
+        a.phiB[ib] = PhiFace;
+        return false;
+    }
+    if (it == CUPCFD_RTYPE_SYMP) {
+        const v3<T> ds = sub3(xyz(m.fgeoX[f]), xyz(m.cgeo[ip]));
+        a.phiB[ib] = a.phi[ip] + dot3(ld3(a.dPhidx, ip), ds);
+        return false;
+    }
+    if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
+        if (m.radiab[ir]) {
+            a.phiB[ib] = a.phi[ip];
+            m.bq[ib] = 0.0;
+            return false;
+        }
+        T Hcoef, VisFace;
+        const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
+        const T area = m.fgeoP[f].w;
+        const T cp = a.cpB[ib];
+        a.phiB[ib] = PhiFace;
+        // Au = Au + Hcoef;  Su = Su + Hcoef * PhiFace
+        t[0] = Hcoef * PhiFace;
+        t[3] = Hcoef;
+        const T Tdif = (PhiFace - a.phi[ip]);
+        const T h1 = Hcoef * cp / area;
+        m.bq[ib] = h1 * Tdif;
+        m.bh[ib] = VisFace * cp;
+        m.bt[ib] = PhiFace;
+        return true;
+    }
+    return false;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scalar(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int c = lb * kBlock + threadIdx.x;
     if (c >= m.C) return;
     T su = a.su[c];
     T au = 0;
     bool au_loaded = false;
     const int e1 = m.row[c + 1];
     for (int e = m.row[c]; e < e1; e++) {
-        const int ent = m.ent[e];
-        const int f = ent >> 1;
-        const int2 cn = m.fconn[f];
-        const V4<T> s = m.fgeoS[f];
-        const v3<T> S{s.x, s.y, s.z};
-        const T mf = a.massFlux[f];
-        if (cn.y >= 0) {
-            const int ip = cn.x, in = cn.y;
-            const T facn = s.w;
-            const T facp = 1.0 - facn;
-            T Visac = a.visEff[in] * facn + a.visEff[ip] * facp;
-            Visac = scalar_visac(a, Visac);
-            const v3<T> g = lerp3(ld3(a.dPhidx, in), facn, ld3(a.dPhidx, ip), facp);
-            const v3<T> Xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-            const T VisFace = Visac * m.fgeoX[f].w;
-            const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
-            const T fce = mf * PhiFace;
-            const T fde1 = Visac * dot3(g, S);
-            const T fci = fmin((double)mf, 0.0) * (double)a.phi[in] + fmax((double)mf, 0.0) * (double)a.phi[ip];
-            const T fdi = VisFace * dot3(g, Xpn);
-            const T blend = a.gammaBlend * (fce - fci);
-            if ((ent & 1) == 0) {
-                su = su - blend + fde1 - fdi;
-                a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
-                a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
-            } else {
-                su = su + blend - fde1 + fdi;
+        T t[4];
+        const bool ta = scalar_terms(m, a, m.ent[e], t);
+        su = ((su + t[0]) + t[1]) + t[2];
+        if (ta) {
+            if (!au_loaded) {
+                au = a.au[c];
+                au_loaded = true;
             }
-        } else {
-            const int ip = cn.x;
-            const int ib = ~cn.y;
-            const int ir = m.bregion[ib];
-            const int it = m.rtype[ir];
-            if (it == CUPCFD_RTYPE_INLET) {
-                const v3<T> g = ld3(a.dPhidx, ip);
-                const T PhiFace = a.inletAssigns ? (T)(0) : scalar_carry(m, a, f);
-                T Visac = scalar_visac(a, a.visEffB[ib]);
-                const V4<T> xf = m.fgeoX[f];
-                const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
-                const T VisFace = Visac * xf.w;
-                const T fde = Visac * dot3(g, S);
-                const T fdi = VisFace * dot3(g, Xpn);
-                const T fF = -VisFace + fmin((double)mf, 0.0);
-                if (!au_loaded) {
-                    au = a.au[c];
-                    au_loaded = true;
-                }
-                au = au - fF;
-                su = su + fde - fdi - fF * PhiFace;
-                a.phiB[ib] = PhiFace;
-            } else if (it == CUPCFD_RTYPE_OUTLET) {
-                const v3<T> g = ld3(a.dPhidx, ip);
-                T Visac = scalar_visac(a, a.visEff[ip]);
-                const V4<T> xf = m.fgeoX[f];
-                const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
-                const T PhiFace = a.phi[ip] + dot3(g, Xpn);
-                const T VisFace = Visac * xf.w;
-                const T fde = Visac * dot3(g, S);
-                const T fdi = VisFace * dot3(g, Xpn);
-                su = su + fde - fdi;
-                a.phiB[ib] = PhiFace;
-            } else if (it == CUPCFD_RTYPE_SYMP) {
-                const v3<T> ds = sub3(xyz(m.fgeoX[f]), xyz(m.cgeo[ip]));
-                a.phiB[ib] = a.phi[ip] + dot3(ld3(a.dPhidx, ip), ds);
-            } else if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
-                if (m.radiab[ir]) {
-                    a.phiB[ib] = a.phi[ip];
-                    m.bq[ib] = 0.0;
-                } else {
-                    T Hcoef, VisFace;
-                    const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
-                    const T area = m.fgeoP[f].w;
-                    const T cp = a.cpB[ib];
-                    a.phiB[ib] = PhiFace;
-                    if (!au_loaded) {
-                        au = a.au[c];
-                        au_loaded = true;
-                    }
-                    au = au + Hcoef;
-                    su = su + Hcoef * PhiFace;
-                    const T Tdif = (PhiFace - a.phi[ip]);
-                    const T h1 = Hcoef * cp / area;
-                    m.bq[ib] = h1 * Tdif;
-                    m.bh[ib] = VisFace * cp;
-                    m.bt[ib] = PhiFace;
-                }
-            }
+            au = au + t[3];
         }
     }
     a.su[c] = su;
     if (au_loaded) a.au[c] = au;
+}
+
+template <class T, int LPC>
+__global__ __launch_bounds__(kBlock) void k_scalar_vec(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kBlock / LPC, NT = 4, P = LPC + 1;
+    __shared__ T lds[NT * CPB * P];
+    __shared__ uint8_t flag[CPB * P];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
+    const int c = lb * CPB + cl;
+    int e0 = 0, n = 0;
+    if (c < m.C) {
+        e0 = m.row[c];
+        n = m.row[c + 1] - e0;
+    }
+    if (j < n) {
+        T t[NT];
+        flag[cl * P + j] = scalar_terms(m, a, m.ent[e0 + j], t) ? 1 : 0;
+        for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
+    }
+    __syncthreads();
+    if (c >= m.C) return;
+    if (j == 0) {
+        T su = a.su[c];
+        for (int i = 0; i < n; i++)
+            su = ((su + lds[(0 * CPB + cl) * P + i]) + lds[(1 * CPB + cl) * P + i]) + lds[(2 * CPB + cl) * P + i];
+        a.su[c] = su;
+    } else if (j == 1) {
+        bool any = false;
+        for (int i = 0; i < n; i++) any |= flag[cl * P + i] != 0;
+        if (any) {
+            T au = a.au[c];
+            for (int i = 0; i < n; i++)
+                if (flag[cl * P + i]) au = au + lds[(3 * CPB + cl) * P + i];
+            a.au[c] = au;
+        }
+    }
 }
 
 // ===========================================================================
@@ -576,174 +714,238 @@ template <class T> __device__ T uvw_visface_carry(const DevMesh<T> &m, const Uvw
     return visac * rl;
 }
 
+// Terms of one face of the UVW loop for the cell the entry belongs to:
+// t[3c..3c+2] the three additions to su/sv/sw (component c), t[9] the addition
+// to au, av and aw alike. Returns whether the face updates au/av/aw.
+// Per-face outputs (rFace, uB/vB/wB, the OUTLET massFlux clamp, boundary
+// shear) are written here by the face's owner.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_uvw(DevMesh<T> m, UvwArgs<T> a) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> &a, int ent, T t[10]) {
+    const int f = ent >> 1;
+    const int2 cn = m.fconn[f];
+    const V4<T> s = m.fgeoS[f];
+    const T sx = s.x, sy = s.y, sz = s.z;
+    for (int k = 0; k < 10; k++) t[k] = nzero<T>();
+    if (cn.y >= 0) {
+        const int ip = cn.x, in = cn.y;
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const v3<T> du = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
+        const v3<T> dv = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
+        const v3<T> dw = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
+        const T visac = a.visEff[in] * facn + a.visEff[ip] * facp;
+        const v3<T> xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
+        const T visFace = visac * m.fgeoX[f].w;
+        const T mf = a.massFlux[f];
+        const v3<T> uf = uvw_carry(m, a, f);
+        const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
+        const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
+        const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
+        const T fwde1 = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
+        const T fude = visac * fude1, fvde = visac * fvde1, fwde = visac * fwde1;
+        const T fmn = smin(mf, (T)(0.0)), fmx = smax(mf, (T)(0.0));
+        const T fuci = fmn * a.u[in] + fmx * a.u[ip];
+        const T fvci = fmn * a.v[in] + fmx * a.v[ip];
+        const T fwci = fmn * a.w[in] + fmx * a.w[ip];
+        const T fudi = visFace * dot3(du, xpn);
+        const T fvdi = visFace * dot3(dv, xpn);
+        const T fwdi = visFace * dot3(dw, xpn);
+        const T blendU = a.gammaBlend * (fuce - fuci);
+        const T blendV = a.gammaBlend * (fvce - fvci);
+        const T blendW = a.gammaBlend * (fwce - fwci);
+        if ((ent & 1) == 0) {
+            // su[ip] = ((su[ip] - blendU) + fude) - fudi   (sv, sw alike)
+            t[0] = -blendU, t[1] = fude, t[2] = -fudi;
+            t[3] = -blendV, t[4] = fvde, t[5] = -fvdi;
+            t[6] = -blendW, t[7] = fwde, t[8] = -fwdi;
+            a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
+            a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
+        } else {
+            // su[in] = ((su[in] + blendU) - fude) + fudi
+            t[0] = blendU, t[1] = -fude, t[2] = fudi;
+            t[3] = blendV, t[4] = -fvde, t[5] = fvdi;
+            t[6] = blendW, t[7] = -fwde, t[8] = fwdi;
+        }
+        return false;
+    }
+    const int ip = cn.x;
+    const int ib = ~cn.y;
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    if (it == CUPCFD_RTYPE_DEFAULT) return false;
+    const V4<T> xf = m.fgeoX[f];
+    const v3<T> cp = xyz(m.cgeo[ip]);
+    const T up = a.u[ip], vp = a.v[ip], wp = a.w[ip];
+    if (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) {
+        const v3<T> du = ld3(a.dudx, ip), dv = ld3(a.dvdx, ip), dw = ld3(a.dwdx, ip);
+        v3<T> uf;
+        T visac;
+        if (it == CUPCFD_RTYPE_INLET) {
+            uf = ld3(m.ruvw, ir);
+            visac = a.visEffB[ib];
+        } else {
+            uf = v3<T>{up, vp, wp};
+            visac = a.visEff[ip];
+        }
+        const v3<T> xpn = sub3(xyz(xf), cp);
+        const T visFace = visac * xf.w;
+        T fude = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
+        T fvde = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
+        T fwde = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
+        fude = visac * fude;
+        fvde = visac * fvde;
+        fwde = visac * fwde;
+        const T fudi = visFace * dot3(du, xpn);
+        const T fvdi = visFace * dot3(dv, xpn);
+        const T fwdi = visFace * dot3(dw, xpn);
+        T mf = a.massFlux[f];
+        if (it == CUPCFD_RTYPE_OUTLET && mf < 0.0) {
+            mf = a.small;
+            a.massFlux[f] = mf;
+        }
+        const T fF = -visFace + smin(mf, (T)(0.0));
+        // au = au - f;  su = ((su - f * uFace) + fude) - fudi
+        t[0] = -(fF * uf.x), t[1] = fude, t[2] = -fudi;
+        t[3] = -(fF * uf.y), t[4] = fvde, t[5] = -fvdi;
+        t[6] = -(fF * uf.z), t[7] = fwde, t[8] = -fwdi;
+        t[9] = -fF;
+        a.uB[ib] = uf.x;
+        a.vB[ib] = uf.y;
+        a.wB[ib] = uf.z;
+        return true;
+    }
+    if (it == CUPCFD_RTYPE_SYMP) {
+        const v3<T> xpn = sub3(xyz(xf), cp);
+        const T visac = a.visEff[ip];
+        const T du = dot3(ld3(a.dudx, ip), xpn);
+        const T dv = dot3(ld3(a.dvdx, ip), xpn);
+        const T dw = dot3(ld3(a.dwdx, ip), xpn);
+        const v3<T> us{up + du, vp + dv, wp + dw};
+        const v3<T> xn = normalise3(v3<T>{(T)(0.0 - sx), (T)(0.0 - sy), (T)(0.0 - sz)});
+        const T dp = dot3(us, xn);
+        const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
+        const T dn = m.bdist[ib];
+        const T twoVis = (T)(2.0 * visac);
+        const T area = m.fgeoP[f].w;
+        const v3<T> force{((twoVis * un.x) / dn) * area, ((twoVis * un.y) / dn) * area,
+                          ((twoVis * un.z) / dn) * area};
+        const T visFace = uvw_visface_carry(m, a, f);
+        // au = au + visFace;  su = (su + visFace * u) - force.x
+        t[0] = visFace * up, t[1] = -force.x;
+        t[3] = visFace * vp, t[4] = -force.y;
+        t[6] = visFace * wp, t[7] = -force.z;
+        t[9] = visFace;
+        a.uB[ib] = us.x;
+        a.vB[ib] = us.y;
+        a.wB[ib] = us.z;
+        return true;
+    }
+    // WALL
+    const v3<T> uw = ld3(m.ruvw, ir);
+    const T visac = a.visEffB[ib];
+    const v3<T> xpn = sub3(xyz(xf), cp);
+    const T xpn_length = (T)length3(xpn);
+    const T area = m.fgeoP[f].w;
+    const T coef = visac * area / xpn_length;
+    const v3<T> xn = normalise3(v3<T>{sx, sy, sz});
+    const v3<T> upr = sub3(v3<T>{up, vp, wp}, uw);
+    const T dp = dot3(upr, xn);
+    const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
+    const v3<T> ut = sub3(upr, un);
+    const T uvel = fabs((double)ut.x) + fabs((double)ut.y) + fabs((double)ut.z);
+    v3<T> force{0.0, 0.0, 0.0};
+    if (uvel > a.small) {
+        const T dn = m.bdist[ib];
+        force = v3<T>{((visac * ut.x) / dn) * area, ((visac * ut.y) / dn) * area, ((visac * ut.z) / dn) * area};
+    }
+    st3(m.bshear, ib, force);
+    // au = au + coef;  su = (su + coef * u) - force.x
+    t[0] = coef * up, t[1] = -force.x;
+    t[3] = coef * vp, t[4] = -force.y;
+    t[6] = coef * wp, t[7] = -force.z;
+    t[9] = coef;
+    a.uB[ib] = uw.x;
+    a.vB[ib] = uw.y;
+    a.wB[ib] = uw.z;
+    return true;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_uvw(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int c = lb * kBlock + threadIdx.x;
     if (c >= m.C) return;
-    T su = a.su[c], sv = a.sv[c], sw = a.sw[c];
-    T au = 0, av = 0, aw = 0;
+    T s[3] = {a.su[c], a.sv[c], a.sw[c]};
+    T aa[3] = {0, 0, 0};
     bool a_loaded = false;
     const int e1 = m.row[c + 1];
     for (int e = m.row[c]; e < e1; e++) {
-        const int ent = m.ent[e];
-        const int f = ent >> 1;
-        const int2 cn = m.fconn[f];
-        const V4<T> s = m.fgeoS[f];
-        const T sx = s.x, sy = s.y, sz = s.z;
-        if (cn.y >= 0) {
-            const int ip = cn.x, in = cn.y;
-            const T facn = s.w;
-            const T facp = 1.0 - facn;
-            const v3<T> du = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
-            const v3<T> dv = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
-            const v3<T> dw = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
-            const T visac = a.visEff[in] * facn + a.visEff[ip] * facp;
-            const v3<T> xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-            const T visFace = visac * m.fgeoX[f].w;
-            const T mf = a.massFlux[f];
-            const v3<T> uf = uvw_carry(m, a, f);
-            const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
-            const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
-            const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
-            const T fwde1 = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
-            const T fude = visac * fude1, fvde = visac * fvde1, fwde = visac * fwde1;
-            const T fmn = smin(mf, (T)(0.0)), fmx = smax(mf, (T)(0.0));
-            const T fuci = fmn * a.u[in] + fmx * a.u[ip];
-            const T fvci = fmn * a.v[in] + fmx * a.v[ip];
-            const T fwci = fmn * a.w[in] + fmx * a.w[ip];
-            const T fudi = visFace * dot3(du, xpn);
-            const T fvdi = visFace * dot3(dv, xpn);
-            const T fwdi = visFace * dot3(dw, xpn);
-            const T blendU = a.gammaBlend * (fuce - fuci);
-            const T blendV = a.gammaBlend * (fvce - fvci);
-            const T blendW = a.gammaBlend * (fwce - fwci);
-            if ((ent & 1) == 0) {
-                a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
-                a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
-                su = su - blendU + fude - fudi;
-                sv = sv - blendV + fvde - fvdi;
-                sw = sw - blendW + fwde - fwdi;
-            } else {
-                su = su + blendU - fude + fudi;
-                sv = sv + blendV - fvde + fvdi;
-                sw = sw + blendW - fwde + fwdi;
+        T t[10];
+        const bool ta = uvw_terms(m, a, m.ent[e], t);
+        for (int k = 0; k < 3; k++) s[k] = ((s[k] + t[3 * k]) + t[3 * k + 1]) + t[3 * k + 2];
+        if (ta) {
+            if (!a_loaded) {
+                aa[0] = a.au[c];
+                aa[1] = a.av[c];
+                aa[2] = a.aw[c];
+                a_loaded = true;
             }
-            continue;
-        }
-        const int ip = cn.x;
-        const int ib = ~cn.y;
-        const int ir = m.bregion[ib];
-        const int it = m.rtype[ir];
-        if (it == CUPCFD_RTYPE_DEFAULT) continue;
-        if (!a_loaded) {
-            au = a.au[c];
-            av = a.av[c];
-            aw = a.aw[c];
-            a_loaded = true;
-        }
-        const V4<T> xf = m.fgeoX[f];
-        const v3<T> cp = xyz(m.cgeo[ip]);
-        if (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) {
-            const v3<T> du = ld3(a.dudx, ip), dv = ld3(a.dvdx, ip), dw = ld3(a.dwdx, ip);
-            v3<T> uf;
-            T visac;
-            if (it == CUPCFD_RTYPE_INLET) {
-                uf = ld3(m.ruvw, ir);
-                visac = a.visEffB[ib];
-            } else {
-                uf = v3<T>{a.u[ip], a.v[ip], a.w[ip]};
-                visac = a.visEff[ip];
-            }
-            const v3<T> xpn = sub3(xyz(xf), cp);
-            const T visFace = visac * xf.w;
-            T fude = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
-            T fvde = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
-            T fwde = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
-            fude = visac * fude;
-            fvde = visac * fvde;
-            fwde = visac * fwde;
-            const T fudi = visFace * dot3(du, xpn);
-            const T fvdi = visFace * dot3(dv, xpn);
-            const T fwdi = visFace * dot3(dw, xpn);
-            T mf = a.massFlux[f];
-            if (it == CUPCFD_RTYPE_OUTLET && mf < 0.0) {
-                mf = a.small;
-                a.massFlux[f] = mf;
-            }
-            const T fF = -visFace + smin(mf, (T)(0.0));
-            au = au - fF;
-            su = su - fF * uf.x + fude - fudi;
-            av = av - fF;
-            sv = sv - fF * uf.y + fvde - fvdi;
-            aw = aw - fF;
-            sw = sw - fF * uf.z + fwde - fwdi;
-            a.uB[ib] = uf.x;
-            a.vB[ib] = uf.y;
-            a.wB[ib] = uf.z;
-        } else if (it == CUPCFD_RTYPE_SYMP) {
-            const v3<T> xpn = sub3(xyz(xf), cp);
-            const T visac = a.visEff[ip];
-            const T du = dot3(ld3(a.dudx, ip), xpn);
-            const T dv = dot3(ld3(a.dvdx, ip), xpn);
-            const T dw = dot3(ld3(a.dwdx, ip), xpn);
-            const v3<T> us{a.u[ip] + du, a.v[ip] + dv, a.w[ip] + dw};
-            const v3<T> xn = normalise3(v3<T>{(T)(0.0 - sx), (T)(0.0 - sy), (T)(0.0 - sz)});
-            const T dp = dot3(us, xn);
-            const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
-            const T dn = m.bdist[ib];
-            const T twoVis = (T)(2.0 * visac);
-            const T area = m.fgeoP[f].w;
-            const v3<T> force{((twoVis * un.x) / dn) * area, ((twoVis * un.y) / dn) * area,
-                              ((twoVis * un.z) / dn) * area};
-            const T visFace = uvw_visface_carry(m, a, f);
-            au = au + visFace;
-            av = av + visFace;
-            aw = aw + visFace;
-            su = su + visFace * a.u[ip] - force.x;
-            sv = sv + visFace * a.v[ip] - force.y;
-            sw = sw + visFace * a.w[ip] - force.z;
-            a.uB[ib] = us.x;
-            a.vB[ib] = us.y;
-            a.wB[ib] = us.z;
-        } else {  // WALL
-            const v3<T> uw = ld3(m.ruvw, ir);
-            const T visac = a.visEffB[ib];
-            const v3<T> xpn = sub3(xyz(xf), cp);
-            const T xpn_length = (T)length3(xpn);
-            const T area = m.fgeoP[f].w;
-            const T coef = visac * area / xpn_length;
-            const v3<T> xn = normalise3(v3<T>{sx, sy, sz});
-            v3<T> up{a.u[ip], a.v[ip], a.w[ip]};
-            up = sub3(up, uw);
-            const T dp = dot3(up, xn);
-            const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
-            const v3<T> ut = sub3(up, un);
-            const T uvel = fabs((double)ut.x) + fabs((double)ut.y) + fabs((double)ut.z);
-            v3<T> force{0.0, 0.0, 0.0};
-            if (uvel > a.small) {
-                const T dn = m.bdist[ib];
-                force = v3<T>{((visac * ut.x) / dn) * area, ((visac * ut.y) / dn) * area,
-                              ((visac * ut.z) / dn) * area};
-            }
-            st3(m.bshear, ib, force);
-            au = au + coef;
-            av = av + coef;
-            aw = aw + coef;
-            su = su + coef * a.u[ip] - force.x;
-            sv = sv + coef * a.v[ip] - force.y;
-            sw = sw + coef * a.w[ip] - force.z;
-            a.uB[ib] = uw.x;
-            a.vB[ib] = uw.y;
-            a.wB[ib] = uw.z;
+            for (int k = 0; k < 3; k++) aa[k] = aa[k] + t[9];
         }
     }
-    a.su[c] = su;
-    a.sv[c] = sv;
-    a.sw[c] = sw;
+    a.su[c] = s[0];
+    a.sv[c] = s[1];
+    a.sw[c] = s[2];
     if (a_loaded) {
-        a.au[c] = au;
-        a.av[c] = av;
-        a.aw[c] = aw;
+        a.au[c] = aa[0];
+        a.av[c] = aa[1];
+        a.aw[c] = aa[2];
+    }
+}
+
+template <class T, int LPC>
+__global__ __launch_bounds__(kBlock) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kBlock / LPC, NT = 10, P = LPC + 1;
+    __shared__ T lds[NT * CPB * P];
+    __shared__ uint8_t flag[CPB * P];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
+    const int c = lb * CPB + cl;
+    int e0 = 0, n = 0;
+    if (c < m.C) {
+        e0 = m.row[c];
+        n = m.row[c + 1] - e0;
+    }
+    if (j < n) {
+        T t[NT];
+        flag[cl * P + j] = uvw_terms(m, a, m.ent[e0 + j], t) ? 1 : 0;
+        for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
+    }
+    __syncthreads();
+    if (c >= m.C) return;
+    // lanes 0..2 fold su/sv/sw, lanes 3..5 fold au/av/aw (LPC >= 6), or lanes
+    // 0..3 take two components each (LPC == 4)
+    for (int k = j; k < 6; k += LPC) {
+        if (k < 3) {
+            T *dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
+            T acc = dst[c];
+            for (int i = 0; i < n; i++)
+                acc = ((acc + lds[((3 * k) * CPB + cl) * P + i]) + lds[((3 * k + 1) * CPB + cl) * P + i]) +
+                      lds[((3 * k + 2) * CPB + cl) * P + i];
+            dst[c] = acc;
+        } else {
+            bool any = false;
+            for (int i = 0; i < n; i++) any |= flag[cl * P + i] != 0;
+            if (any) {
+                T *dst = k == 3 ? a.au : (k == 4 ? a.av : a.aw);
+                T acc = dst[c];
+                for (int i = 0; i < n; i++)
+                    if (flag[cl * P + i]) acc = acc + lds[(9 * CPB + cl) * P + i];
+                dst[c] = acc;
+            }
+        }
     }
 }
 
@@ -901,6 +1103,25 @@ inline int blocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 inline int launch_ok() { return hipGetLastError() == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR; }
 
+// Grid of `per` logical blocks per XCD covering `items` at `per_block` each
+struct Grid {
+    int blocks, per, nblk;
+};
+inline Grid xcd_grid(int64_t items, int per_block) {
+    const int nblk = (int)((items + per_block - 1) / per_block);
+    const int per = (nblk + kXcd - 1) / kXcd;
+    return Grid{per * kXcd, per, nblk};
+}
+
+// Lanes per cell for the gathers: the mesh's schedule if set (1 = one thread
+// per cell), else the smallest of 4 / 8 that holds every row, else 1.
+template <class T> int lanes_per_cell(const Mesh<T> &M) {
+    int l = M.schedule_lpc;
+    if (l == 0) l = M.d.maxrow <= 4 ? 4 : (M.d.maxrow <= 8 ? 8 : 1);
+    if (l != 1 && M.d.maxrow > l) l = 1;
+    return l == 4 || l == 8 ? l : 0;
+}
+
 template <class T> Mesh<T> *get(cupcfd_hip_mesh *h);
 template <> Mesh<double> *get<double>(cupcfd_hip_mesh *h) { return h && h->precision == 64 ? h->m64 : nullptr; }
 template <> Mesh<float> *get<float>(cupcfd_hip_mesh *h) { return h && h->precision == 32 ? h->m32 : nullptr; }
@@ -926,17 +1147,24 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         t[1] = (T *)M->ws_get(43, 3 * (size_t)d.C * sizeof(T));
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
+    const int lpc = lanes_per_cell(*M);
+    const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
         const bool last = (it == nGradient - 1);
         T *dst = last ? g : t[it & 1];
-        T *dst2 = last ? go : nullptr;
-        if (it == 0)
-            hipLaunchKernelGGL((k_gradient<T, true>), dim3(blocks(d.C)), dim3(kBlock), 0, s, d, phi, phiB, prev,
-                               dst, dst2);
-        else
-            hipLaunchKernelGGL((k_gradient<T, false>), dim3(blocks(d.C)), dim3(kBlock), 0, s, d, phi, phiB, prev,
-                               dst, dst2);
+        GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
+        const bool s0 = (it == 0);
+        if (lpc == 8) {
+            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 8>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_vec<T, false, 8>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+        } else if (lpc == 4) {
+            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 4>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_vec<T, false, 4>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+        } else {
+            if (s0) hipLaunchKernelGGL((k_gradient<T, true>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient<T, false>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+        }
         if (hipGetLastError() != hipSuccess) return CUPCFD_E_ERROR;
         prev = dst;
     }
@@ -1011,8 +1239,14 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         a.src = scalar_carry_src<T>(*M, a.inletAssigns != 0, a.wallActive != 0);
         if (!a.src) return CUPCFD_E_ERROR;
     }
-    if (d.C)
-        hipLaunchKernelGGL(k_scalar<T>, dim3(blocks(d.C)), dim3(kBlock), 0, (hipStream_t)stream, d, a);
+    if (d.C) {
+        const int lpc = lanes_per_cell(*M);
+        const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
+        hipStream_t st = (hipStream_t)stream;
+        if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        else hipLaunchKernelGGL(k_scalar<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+    }
     return launch_ok();
 }
 
@@ -1025,7 +1259,14 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
     const DevMesh<T> &d = M->d;
     UvwArgs<T> a{u, v, w, visEff, visEffB, dudx, dvdx, dwdx, uB, vB, wB, massFlux, rFace, su, sv, sw, au, av, aw,
                  gammaBlend, small};
-    if (d.C) hipLaunchKernelGGL(k_uvw<T>, dim3(blocks(d.C)), dim3(kBlock), 0, (hipStream_t)stream, d, a);
+    if (d.C) {
+        const int lpc = lanes_per_cell(*M);
+        const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
+        hipStream_t st = (hipStream_t)stream;
+        if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        else hipLaunchKernelGGL(k_uvw<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+    }
     return launch_ok();
 }
 

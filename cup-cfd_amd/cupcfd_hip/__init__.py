@@ -111,7 +111,8 @@ HOST_SYMBOLS = [
     "cupcfd_structgen_create", "cupcfd_host_mesh_view_f64", "cupcfd_host_mesh_view_f32",
     "cupcfd_host_mesh_destroy", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
-    "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_push_state",
+    "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
+    "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
 ]
@@ -140,6 +141,7 @@ def _declare(L):
     L.cupcfd_hip_mesh_cell_order.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_face_order.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_set_carry_init.argtypes = [ct.c_void_p, ct.c_double]
+    L.cupcfd_hip_mesh_set_schedule.argtypes = [ct.c_void_p, ct.c_int32]
     L.cupcfd_hip_mesh_push_state.argtypes = [ct.c_void_p]
     L.cupcfd_hip_mesh_pull_state.argtypes = [ct.c_void_p]
     for n in ("cells_to_internal", "cells_from_internal", "faces_to_internal", "faces_from_internal"):
@@ -318,6 +320,10 @@ class DeviceMesh:
 
     def set_carry_init(self, v):
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
+
+    def set_schedule(self, lanes_per_cell):
+        """0 = automatic, 1 = one thread per cell, 4 or 8 lanes per cell."""
+        check("set_schedule", lib().cupcfd_hip_mesh_set_schedule(self._h, int(lanes_per_cell)))
 
     def push_state(self):
         check("push_state", lib().cupcfd_hip_mesh_push_state(self._h))

@@ -155,6 +155,11 @@ int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t
 int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
+/* Gather schedule of the per-cell face loops (gradient, scalar, UVW): lanes
+ * evaluating one cell's faces in parallel, 4 or 8, or 1 for one thread per
+ * cell; 0 (default) picks the smallest of 4 / 8 that holds the longest row.
+ * Every schedule gives bit-identical results. */
+int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
 /* copy mutable mesh state view → device (push) or device → view (pull) */
 int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh);
 int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh);

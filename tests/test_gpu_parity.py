@@ -145,3 +145,21 @@ def test_size_errors_are_reported():
     g2, go2 = np.full((host.nCells + 4, 3), 5.0), np.full((host.nCells + 4, 3), 5.0)
     assert fvm.GradientPhiGaussDolfyn(dev, 1, f.phi, f.phiB, g2, go2) == ch.E_SUCCESS
     assert (g2[host.nCells:] == 0).all() and (go2[host.nCells:] == 0).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32"])
+def test_schedules_bit_identical(meshname):
+    """One thread per cell and eight lanes per cell give the same bits."""
+    host, _ = _setup(meshname, 64)
+    outs = {}
+    for lpc in (1, 8):
+        dev = ch.DeviceMesh(host, 64)
+        dev.set_schedule(lpc)
+        for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
+            f = W.Fields(host, seed=77)
+            host.set_state(seed_state(host))
+            outs[(lpc, kernel)] = run(kernel, fvm, dev, f, lambda: host.state(64))[1]
+    for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
+        for k in outs[(1, kernel)]:
+            assert compare(outs[(1, kernel)][k], outs[(8, kernel)][k])[0], (kernel, k)
