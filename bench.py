@@ -211,9 +211,12 @@ def main():
     copy_gbs = stream_copy_gbs(torch)
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{n}.json")
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by the
+    # copy-kernel calibration recorded beside them)
+    tfile = os.path.join(ROOT, "profiles", f"fvm_kernels_structgen{n}_f64.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch", {}).get(dom)
 
     out = {
         "metric": "Gcell-sweeps/s & % HBM peak per FVM kernel; batched & fwd+adjoint vs plain",
