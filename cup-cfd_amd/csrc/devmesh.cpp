@@ -143,7 +143,9 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
 
     // ---- face streams -------------------------------------------------------
     std::vector<int2> fconn(F);
-    std::vector<V4<T>> fS(F), fX(F), fP(F), fN(F);
+    std::vector<V4<T>> fS(F), fX(F), fP(F);
+    std::vector<V2<T>> fN(F);
+    std::vector<T> frl(F);
     M.fconn_h.assign(2 * (size_t)F, 0);
     for (int k = 0; k < F; k++) {
         int f = M.face_order[k];
@@ -153,10 +155,11 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
         M.fconn_h[2 * k] = fconn[k].x;
         M.fconn_h[2 * k + 1] = nb;
         fS[k] = {*v3ptr(v.faceNorm, f, 0), *v3ptr(v.faceNorm, f, 1), *v3ptr(v.faceNorm, f, 2), v.faceLambda[f]};
-        fX[k] = {*v3ptr(v.faceCenter, f, 0), *v3ptr(v.faceCenter, f, 1), *v3ptr(v.faceCenter, f, 2),
-                 v.faceRLencos[f]};
-        fP[k] = {*v3ptr(v.faceXpac, f, 0), *v3ptr(v.faceXpac, f, 1), *v3ptr(v.faceXpac, f, 2), v.faceArea[f]};
-        fN[k] = {*v3ptr(v.faceXnac, f, 0), *v3ptr(v.faceXnac, f, 1), *v3ptr(v.faceXnac, f, 2), (T)0};
+        fX[k] = {*v3ptr(v.faceCenter, f, 0), *v3ptr(v.faceCenter, f, 1), *v3ptr(v.faceCenter, f, 2), v.faceArea[f]};
+        fP[k] = {*v3ptr(v.faceXpac, f, 0), *v3ptr(v.faceXpac, f, 1), *v3ptr(v.faceXpac, f, 2),
+                 *v3ptr(v.faceXnac, f, 0)};
+        fN[k] = {*v3ptr(v.faceXnac, f, 1), *v3ptr(v.faceXnac, f, 2)};
+        frl[k] = v.faceRLencos[f];
     }
     std::vector<V4<T>> cg(C);
     std::vector<T> cv(C);
@@ -175,11 +178,19 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     }
     for (int c = 0; c < C; c++) row[c + 1] += row[c];
     const int nnz = row[C];
-    std::vector<int> ent(nnz), fill(row.begin(), row.end() - 1);
+    std::vector<int2> ent(nnz);
+    std::vector<int> fill(row.begin(), row.end() - 1);
     for (int f = 0; f < F; f++) {
-        int k = M.face_rank[f];
-        ent[fill[M.cell_rank[v.faceCell1ID[f]]]++] = (k << 1);
-        if (v.faceBoundaryID[f] <= -1) ent[fill[M.cell_rank[v.faceCell2ID[f]]]++] = (k << 1) | 1;
+        const int k = M.face_rank[f];
+        const int c1 = M.cell_rank[v.faceCell1ID[f]];
+        const int b = v.faceBoundaryID[f];
+        if (b > -1) {
+            ent[fill[c1]++] = make_int2(k << 1, ~b);
+        } else {
+            const int c2 = M.cell_rank[v.faceCell2ID[f]];
+            ent[fill[c1]++] = make_int2(k << 1, c2);
+            ent[fill[c2]++] = make_int2((k << 1) | 1, c1);
+        }
     }
 
     // ---- boundaries / regions ----------------------------------------------
@@ -266,12 +277,17 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     d.R = R;
     d.nnz = nnz;
     d.maxrow = 0;
-    for (int c = 0; c < C; c++) d.maxrow = std::max(d.maxrow, row[c + 1] - row[c]);
+    d.rowlen = C ? row[1] - row[0] : 0;
+    for (int c = 0; c < C; c++) {
+        d.maxrow = std::max(d.maxrow, row[c + 1] - row[c]);
+        if (row[c + 1] - row[c] != d.rowlen) d.rowlen = 0;
+    }
     d.fconn = up.put(fconn.data(), F);
     d.fgeoS = up.put(fS.data(), F);
     d.fgeoX = up.put(fX.data(), F);
     d.fgeoP = up.put(fP.data(), F);
     d.fgeoN = up.put(fN.data(), F);
+    d.frl = up.put(frl.data(), F);
     d.cgeo = up.put(cg.data(), C);
     d.cvol = up.put(cv.data(), C);
     d.row = up.put(row.data(), C + 1);

@@ -6,11 +6,15 @@
 //   faces  : interior faces first (positions [0, Fi)), then boundary faces;
 //            each group sorted by (internal owner cell, mesh face id).
 //   fconn  : int2 (c1, nb) per face; nb >= 0 is cell2, nb < 0 is ~boundary id.
-//   fgeoS  : (Sx, Sy, Sz, lambda)     fgeoX : (Xf, rlencos)
-//   fgeoP  : (xpac, area)             fgeoN : (xnac, 0)
+//   fgeoS  : (Sx, Sy, Sz, lambda)     fgeoX : (Xf, area)
+//   fgeoP  : (xpac, xnac.x)           fgeoN : (xnac.y, xnac.z)
+//   frl    : rlencos
+//            so each kernel streams exactly the face geometry it uses (the
+//            gradient and interior scalar/UVW faces read only fgeoS and frl).
 //   cgeo   : (Cc, volume) per cell, plus a separate volume stream.
-//   row/ent: cell→face CSR; ent = (internal face << 1) | role, role 1 when
-//            the cell is the face's cell2 ("in"). Each row is sorted by the
+//   row/ent: cell→face CSR; ent = ((internal face << 1) | role, other) with
+//            role 1 when the cell is the face's cell2 ("in") and `other` the
+//            face's other cell, or ~boundary id. Each row is sorted by the
 //            mesh's face id, which is the order the reference's sequential
 //            face loop visits them, so a per-cell gather reproduces the
 //            reference's accumulation order exactly without atomics.
@@ -29,15 +33,20 @@ struct cupcfd_hip_mesh;
 namespace cupcfd_hip {
 
 template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
+template <class T> struct alignas(2 * sizeof(T)) V2 { T x, y; };
 
 template <class T> struct DevMesh {
     int C, F, Fi, B, R, nnz;
     int maxrow;  // longest cell→face row (faces per cell)
     const int2 *fconn;
-    const V4<T> *fgeoS, *fgeoX, *fgeoP, *fgeoN;
+    const V4<T> *fgeoS, *fgeoX, *fgeoP;
+    const V2<T> *fgeoN;
+    const T *frl;
     const V4<T> *cgeo;
     const T *cvol;
-    const int *row, *ent;
+    const int *row;
+    const int2 *ent;
+    int rowlen;  // faces per cell when every row has the same length, else 0
     // boundaries (mesh numbering)
     const int *bface, *bregion, *bcell;
     const T *bdist;

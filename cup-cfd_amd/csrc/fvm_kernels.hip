@@ -121,6 +121,25 @@ __device__ __forceinline__ int xcd_block(int b, int per) { return (b % kXcd) * p
 
 template <class T> __device__ __forceinline__ T nzero() { return (T)(-0.0); }
 
+// CSR entries are (face << 1 | role, other): `other` is the face's other cell,
+// or ~boundary id. entry_cells returns the face's (cell1, cell2 or ~boundary).
+__device__ __forceinline__ int2 entry_cells(int c, int2 en) {
+    if (en.y < 0) return make_int2(c, en.y);
+    return (en.x & 1) ? make_int2(en.y, c) : make_int2(c, en.y);
+}
+
+// Row of cell c: computed for meshes whose cells all have the same number of
+// faces (no row-pointer load), read from the CSR otherwise
+template <class T> __device__ __forceinline__ void row_range(const DevMesh<T> &m, int c, int &e0, int &n) {
+    if (m.rowlen) {
+        e0 = c * m.rowlen;
+        n = m.rowlen;
+    } else {
+        e0 = m.row[c];
+        n = m.row[c + 1] - e0;
+    }
+}
+
 // ===========================================================================
 // a1  GradientPhiGaussDolfyn — GradientKernels.ipp:38-136
 // SWEEP0: dPhidxo is still the zero field of the first sweep, so the
@@ -133,9 +152,9 @@ template <class T> struct GradArgs {
 };
 
 template <class T, bool SWEEP0>
-__device__ __forceinline__ void gradient_terms(const DevMesh<T> &m, const GradArgs<T> &a, int ent, T t[3]) {
-    const int f = ent >> 1;
-    const int2 cn = m.fconn[f];
+__device__ __forceinline__ void gradient_terms(const DevMesh<T> &m, const GradArgs<T> &a, int c, int2 en, T t[3]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
     const V4<T> s = m.fgeoS[f];
     T phiF;
     if (cn.y >= 0) {
@@ -172,10 +191,11 @@ __global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, GradArgs<T> a
     const int c = lb * kBlock + threadIdx.x;
     if (c >= m.C) return;
     T acc[3] = {0, 0, 0};
-    const int e1 = m.row[c + 1];
-    for (int e = m.row[c]; e < e1; e++) {
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
         T t[3];
-        gradient_terms<T, SWEEP0>(m, a, m.ent[e], t);
+        gradient_terms<T, SWEEP0>(m, a, c, m.ent[e], t);
         acc[0] = acc[0] + t[0];
         acc[1] = acc[1] + t[1];
         acc[2] = acc[2] + t[2];
@@ -192,13 +212,10 @@ __global__ __launch_bounds__(kBlock) void k_gradient_vec(DevMesh<T> m, GradArgs<
     const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
     const int c = lb * CPB + cl;
     int e0 = 0, n = 0;
-    if (c < m.C) {
-        e0 = m.row[c];
-        n = m.row[c + 1] - e0;
-    }
+    if (c < m.C) row_range(m, c, e0, n);
     if (j < n) {
         T t[NT];
-        gradient_terms<T, SWEEP0>(m, a, m.ent[e0 + j], t);
+        gradient_terms<T, SWEEP0>(m, a, c, m.ent[e0 + j], t);
         for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
     }
     __syncthreads();
@@ -242,8 +259,9 @@ template <class T> __device__ void mass_interior(const DevMesh<T> &m, const Mass
     const T mf = denf * (uFace * s.x + vFace * s.y + wFace * s.z);
     const v3<T> xnorm = normalise3(v3<T>{s.x, s.y, s.z});
     const V4<T> pa = m.fgeoP[f];
+    const V2<T> na = m.fgeoN[f];
     const v3<T> xpac = xyz(pa);
-    const v3<T> xnac = xyz(m.fgeoN[f]);
+    const v3<T> xnac{pa.w, na.x, na.y};
     const v3<T> gp_ip = ld3(a.dpdx, ip);
     const T pip = a.p[ip] + dot3(gp_ip, sub3(xpac, cip));
     const T pin = a.p[in] + dot3(gp_ip, sub3(xpac, cin));  // dpdx[ip] (MassKernels.ipp:169)
@@ -253,7 +271,7 @@ template <class T> __device__ void mass_interior(const DevMesh<T> &m, const Mass
     const T apv2 = a.den[in] * a.ar[in];
     T apv = apv2 * facn + apv1 * facp;
     const T factv = gin.w * facn + gip.w * facp;
-    apv *= pa.w * factv / dot3(xpn2, xnorm);
+    apv *= xf.w * factv / dot3(xpn2, xnorm);
     const v3<T> gp_in = ld3(a.dpdx, in);
     const T dpx = (gp_in.x * facn + gp_ip.x * facp) * xpn.x;
     const T dpy = (gp_in.y * facn + gp_ip.y * facp) * xpn.y;
@@ -266,12 +284,14 @@ template <class T> __device__ void mass_interior(const DevMesh<T> &m, const Mass
 
 // boundary faces of one owning cell, in face order (MassKernels.ipp:197-305)
 template <class T> __device__ void mass_boundary_cell(const DevMesh<T> &m, const MassArgs<T> &a, int c) {
-    const int e1 = m.row[c + 1];
+    int e0, n;
+    row_range(m, c, e0, n);
     bool touched = false;
     T su = 0;
-    for (int e = m.row[c]; e < e1; e++) {
-        const int f = m.ent[e] >> 1;
-        const int2 cn = m.fconn[f];
+    for (int e = e0; e < e0 + n; e++) {
+        const int2 en = m.ent[e];
+        const int f = en.x >> 1;
+        const int2 cn = entry_cells(c, en);
         if (cn.y >= 0) continue;
         const int ib = ~cn.y;
         const int ir = m.bregion[ib];
@@ -309,12 +329,16 @@ template <class T> __device__ void mass_boundary_cell(const DevMesh<T> &m, const
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, int nblk_interior) {
-    if ((int)blockIdx.x < nblk_interior) {
-        const int f = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, int per, int nblk_interior) {
+    // blocks [0, 8 per): interior faces, XCD-aware; the rest: boundary cells
+    const int b = blockIdx.x;
+    if (b < kXcd * per) {
+        const int lb = xcd_block(b, per);
+        if (lb >= nblk_interior) return;
+        const int f = lb * kBlock + threadIdx.x;
         if (f < m.Fi) mass_interior(m, a, f);
     } else {
-        const int k = (blockIdx.x - nblk_interior) * kBlock + threadIdx.x;
+        const int k = (b - kXcd * per) * kBlock + threadIdx.x;
         if (k < m.nbc) mass_boundary_cell(m, a, m.bcells[k]);
     }
 }
@@ -361,7 +385,7 @@ template <class T> __global__ void k_mass_bl2(DevMesh<T> m, const T *__restrict_
 template <class T> __global__ void k_mass_bl3(DevMesh<T> m, T flowin, T *ratearea) {
     const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
     T areaout = wave_seq_sum<T>(T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET],
-                                [&](int i) { return m.fgeoP[m.bface[lst[i]]].w; });
+                                [&](int i) { return m.fgeoX[m.bface[lst[i]]].w; });
     if (threadIdx.x == 0) *ratearea = -flowin / areaout;
 }
 
@@ -375,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_bl4_faces(DevMesh<T> m, T *mass
     const int ir = m.bregion[ib];
     const int i = m.bface[ib];
     const T split = m.rsplvl[ir];
-    const T area = m.fgeoP[i].w;
+    const T area = m.fgeoX[i].w;
     const T mf = ratearea * area * split;
     massFlux[i] = mf;
     const T faceFlux = mf / denB[ib] / area;
@@ -467,7 +491,7 @@ __device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T> &a, in
     else PhiFace = m.rt[ir];
     const T dn = m.bdist[ib];
     const T Resist = m.rr[ir];
-    const T area = m.fgeoP[f].w;
+    const T area = m.fgeoX[f].w;
     const T cp = a.cpB[ib];
     if (!a.solveTurb) {
         VisFace = a.visLam / a.prandtl / dn;
@@ -510,9 +534,10 @@ template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarAr
 // whether the face updates Au (INLET, enthalpy WALL). Per-face outputs
 // (RFace, PhiB, mesh Q/H/T) are written here by the face's owner.
 template <class T>
-__device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarArgs<T> &a, int ent, T t[4]) {
-    const int f = ent >> 1;
-    const int2 cn = m.fconn[f];
+__device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarArgs<T> &a, int c, int2 en,
+                                             T t[4]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
     const V4<T> s = m.fgeoS[f];
     const v3<T> S{s.x, s.y, s.z};
     const T mf = a.massFlux[f];
@@ -525,14 +550,14 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         Visac = scalar_visac(a, Visac);
         const v3<T> g = lerp3(ld3(a.dPhidx, in), facn, ld3(a.dPhidx, ip), facp);
         const v3<T> Xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-        const T VisFace = Visac * m.fgeoX[f].w;
+        const T VisFace = Visac * m.frl[f];
         const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
         const T fce = mf * PhiFace;
         const T fde1 = Visac * dot3(g, S);
         const T fci = fmin((double)mf, 0.0) * (double)a.phi[in] + fmax((double)mf, 0.0) * (double)a.phi[ip];
         const T fdi = VisFace * dot3(g, Xpn);
         const T blend = a.gammaBlend * (fce - fci);
-        if ((ent & 1) == 0) {
+        if ((en.x & 1) == 0) {
             // Su[ip] = ((Su[ip] - blend) + fde1) - fdi
             t[0] = -blend;
             t[1] = fde1;
@@ -557,7 +582,7 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         const T Visac = scalar_visac(a, a.visEffB[ib]);
         const V4<T> xf = m.fgeoX[f];
         const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
-        const T VisFace = Visac * xf.w;
+        const T VisFace = Visac * m.frl[f];
         const T fde = Visac * dot3(g, S);
         const T fdi = VisFace * dot3(g, Xpn);
         const T fF = -VisFace + fmin((double)mf, 0.0);
@@ -575,7 +600,7 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         const V4<T> xf = m.fgeoX[f];
         const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
         const T PhiFace = a.phi[ip] + dot3(g, Xpn);
-        const T VisFace = Visac * xf.w;
+        const T VisFace = Visac * m.frl[f];
         // Su = (Su + fde) - fdi
         t[0] = Visac * dot3(g, S);
         t[1] = -(VisFace * dot3(g, Xpn));
@@ -595,7 +620,7 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         }
         T Hcoef, VisFace;
         const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
-        const T area = m.fgeoP[f].w;
+        const T area = m.fgeoX[f].w;
         const T cp = a.cpB[ib];
         a.phiB[ib] = PhiFace;
         // Au = Au + Hcoef;  Su = Su + Hcoef * PhiFace
@@ -620,10 +645,11 @@ __global__ __launch_bounds__(kBlock) void k_scalar(DevMesh<T> m, ScalarArgs<T> a
     T su = a.su[c];
     T au = 0;
     bool au_loaded = false;
-    const int e1 = m.row[c + 1];
-    for (int e = m.row[c]; e < e1; e++) {
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
         T t[4];
-        const bool ta = scalar_terms(m, a, m.ent[e], t);
+        const bool ta = scalar_terms(m, a, c, m.ent[e], t);
         su = ((su + t[0]) + t[1]) + t[2];
         if (ta) {
             if (!au_loaded) {
@@ -647,13 +673,10 @@ __global__ __launch_bounds__(kBlock) void k_scalar_vec(DevMesh<T> m, ScalarArgs<
     const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
     const int c = lb * CPB + cl;
     int e0 = 0, n = 0;
-    if (c < m.C) {
-        e0 = m.row[c];
-        n = m.row[c + 1] - e0;
-    }
+    if (c < m.C) row_range(m, c, e0, n);
     if (j < n) {
         T t[NT];
-        flag[cl * P + j] = scalar_terms(m, a, m.ent[e0 + j], t) ? 1 : 0;
+        flag[cl * P + j] = scalar_terms(m, a, c, m.ent[e0 + j], t) ? 1 : 0;
         for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
     }
     __syncthreads();
@@ -701,7 +724,7 @@ template <class T> __device__ T uvw_visface_carry(const DevMesh<T> &m, const Uvw
     const int g = m.carry_src[f];
     if (g < 0) return m.carry_init;
     const int2 cg = m.fconn[g];
-    const T rl = m.fgeoX[g].w;
+    const T rl = m.frl[g];
     if (cg.y >= 0) {
         const T facn = m.fgeoS[g].w;
         const T facp = 1.0 - facn;
@@ -720,9 +743,9 @@ template <class T> __device__ T uvw_visface_carry(const DevMesh<T> &m, const Uvw
 // Per-face outputs (rFace, uB/vB/wB, the OUTLET massFlux clamp, boundary
 // shear) are written here by the face's owner.
 template <class T>
-__device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> &a, int ent, T t[10]) {
-    const int f = ent >> 1;
-    const int2 cn = m.fconn[f];
+__device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> &a, int c, int2 en, T t[10]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
     const V4<T> s = m.fgeoS[f];
     const T sx = s.x, sy = s.y, sz = s.z;
     for (int k = 0; k < 10; k++) t[k] = nzero<T>();
@@ -735,7 +758,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
         const v3<T> dw = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
         const T visac = a.visEff[in] * facn + a.visEff[ip] * facp;
         const v3<T> xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-        const T visFace = visac * m.fgeoX[f].w;
+        const T visFace = visac * m.frl[f];
         const T mf = a.massFlux[f];
         const v3<T> uf = uvw_carry(m, a, f);
         const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
@@ -753,7 +776,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
         const T blendU = a.gammaBlend * (fuce - fuci);
         const T blendV = a.gammaBlend * (fvce - fvci);
         const T blendW = a.gammaBlend * (fwce - fwci);
-        if ((ent & 1) == 0) {
+        if ((en.x & 1) == 0) {
             // su[ip] = ((su[ip] - blendU) + fude) - fudi   (sv, sw alike)
             t[0] = -blendU, t[1] = fude, t[2] = -fudi;
             t[3] = -blendV, t[4] = fvde, t[5] = -fvdi;
@@ -788,7 +811,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
             visac = a.visEff[ip];
         }
         const v3<T> xpn = sub3(xyz(xf), cp);
-        const T visFace = visac * xf.w;
+        const T visFace = visac * m.frl[f];
         T fude = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
         T fvde = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
         T fwde = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
@@ -826,7 +849,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
         const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
         const T dn = m.bdist[ib];
         const T twoVis = (T)(2.0 * visac);
-        const T area = m.fgeoP[f].w;
+        const T area = m.fgeoX[f].w;
         const v3<T> force{((twoVis * un.x) / dn) * area, ((twoVis * un.y) / dn) * area,
                           ((twoVis * un.z) / dn) * area};
         const T visFace = uvw_visface_carry(m, a, f);
@@ -845,7 +868,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
     const T visac = a.visEffB[ib];
     const v3<T> xpn = sub3(xyz(xf), cp);
     const T xpn_length = (T)length3(xpn);
-    const T area = m.fgeoP[f].w;
+    const T area = m.fgeoX[f].w;
     const T coef = visac * area / xpn_length;
     const v3<T> xn = normalise3(v3<T>{sx, sy, sz});
     const v3<T> upr = sub3(v3<T>{up, vp, wp}, uw);
@@ -879,10 +902,11 @@ __global__ __launch_bounds__(kBlock) void k_uvw(DevMesh<T> m, UvwArgs<T> a, int 
     T s[3] = {a.su[c], a.sv[c], a.sw[c]};
     T aa[3] = {0, 0, 0};
     bool a_loaded = false;
-    const int e1 = m.row[c + 1];
-    for (int e = m.row[c]; e < e1; e++) {
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
         T t[10];
-        const bool ta = uvw_terms(m, a, m.ent[e], t);
+        const bool ta = uvw_terms(m, a, c, m.ent[e], t);
         for (int k = 0; k < 3; k++) s[k] = ((s[k] + t[3 * k]) + t[3 * k + 1]) + t[3 * k + 2];
         if (ta) {
             if (!a_loaded) {
@@ -914,13 +938,10 @@ __global__ __launch_bounds__(kBlock) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a, 
     const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
     const int c = lb * CPB + cl;
     int e0 = 0, n = 0;
-    if (c < m.C) {
-        e0 = m.row[c];
-        n = m.row[c + 1] - e0;
-    }
+    if (c < m.C) row_range(m, c, e0, n);
     if (j < n) {
         T t[NT];
-        flag[cl * P + j] = uvw_terms(m, a, m.ent[e0 + j], t) ? 1 : 0;
+        flag[cl * P + j] = uvw_terms(m, a, c, m.ent[e0 + j], t) ? 1 : 0;
         for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
     }
     __syncthreads();
@@ -1181,9 +1202,11 @@ int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, co
     const DevMesh<T> &d = M->d;
     MassArgs<T> a{dudx, dvdx, dwdx, dpdx, den, u, v, w, p, ar, denB, massFlux, su, rface, small, sTE, sTD, sV, sE,
                   teC, edC, vC, tC, teB, edB, vB, tB};
-    const int nbi = blocks(d.Fi), nbb = blocks(d.nbc);
-    if (nbi + nbb > 0)
-        hipLaunchKernelGGL(k_mass<T>, dim3(nbi + nbb), dim3(kBlock), 0, (hipStream_t)stream, d, a, nbi);
+    const Grid gi = xcd_grid(d.Fi, kBlock);
+    const int nbb = blocks(d.nbc);
+    if (gi.blocks + nbb > 0)
+        hipLaunchKernelGGL(k_mass<T>, dim3(gi.blocks + nbb), dim3(kBlock), 0, (hipStream_t)stream, d, a, gi.per,
+                           gi.nblk);
     // the face loop's counters count boundary faces by region type
     if (icinl) *icinl += M->ntype[CUPCFD_RTYPE_INLET];
     if (icout) *icout += M->ntype[CUPCFD_RTYPE_OUTLET];
