@@ -537,7 +537,8 @@ extern "C" int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh) {
 
 extern "C" int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
-    if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 8)
+    if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 6 &&
+        lanes_per_cell != 8)
         return CUPCFD_E_CONFIG_INVALID_VALUE;
     if (mesh->precision == 64) mesh->m64->schedule_lpc = lanes_per_cell;
     else mesh->m32->schedule_lpc = lanes_per_cell;

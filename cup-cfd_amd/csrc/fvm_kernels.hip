@@ -117,6 +117,8 @@ template <class T, class Get> __device__ T wave_seq_sum(T acc, int n, Get get) {
 // private L2) one contiguous range of the locality-ordered cells.
 // ===========================================================================
 constexpr int kXcd = 8;
+// cells per block of the lane-per-face gathers (block = lanes per cell x 32)
+constexpr int kCellsPerBlock = 32;
 __device__ __forceinline__ int xcd_block(int b, int per) { return (b % kXcd) * per + b / kXcd; }
 
 template <class T> __device__ __forceinline__ T nzero() { return (T)(-0.0); }
@@ -204,8 +206,8 @@ __global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, GradArgs<T> a
 }
 
 template <class T, bool SWEEP0, int LPC>
-__global__ __launch_bounds__(kBlock) void k_gradient_vec(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
-    constexpr int CPB = kBlock / LPC, NT = 3, P = LPC + 1;
+__global__ __launch_bounds__(LPC * 32) void k_gradient_vec(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kCellsPerBlock, NT = 3, P = LPC + 1;
     __shared__ T lds[NT * CPB * P];
     const int lb = xcd_block(blockIdx.x, per);
     if (lb >= nblk) return;
@@ -664,8 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar(DevMesh<T> m, ScalarArgs<T> a
 }
 
 template <class T, int LPC>
-__global__ __launch_bounds__(kBlock) void k_scalar_vec(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
-    constexpr int CPB = kBlock / LPC, NT = 4, P = LPC + 1;
+__global__ __launch_bounds__(LPC * 32) void k_scalar_vec(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kCellsPerBlock, NT = 4, P = LPC + 1;
     __shared__ T lds[NT * CPB * P];
     __shared__ uint8_t flag[CPB * P];
     const int lb = xcd_block(blockIdx.x, per);
@@ -929,8 +931,8 @@ __global__ __launch_bounds__(kBlock) void k_uvw(DevMesh<T> m, UvwArgs<T> a, int 
 }
 
 template <class T, int LPC>
-__global__ __launch_bounds__(kBlock) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
-    constexpr int CPB = kBlock / LPC, NT = 10, P = LPC + 1;
+__global__ __launch_bounds__(LPC * 32) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kCellsPerBlock, NT = 10, P = LPC + 1;
     __shared__ T lds[NT * CPB * P];
     __shared__ uint8_t flag[CPB * P];
     const int lb = xcd_block(blockIdx.x, per);
@@ -1135,12 +1137,17 @@ inline Grid xcd_grid(int64_t items, int per_block) {
 }
 
 // Lanes per cell for the gathers: the mesh's schedule if set (1 = one thread
-// per cell), else the smallest of 4 / 8 that holds every row, else 1.
+// per cell); else the row length of uniform 4- or 6-face meshes (every lane
+// busy), else 8 if every row fits, else one thread per cell. Returns 0 for
+// the one-thread-per-cell kernels.
 template <class T> int lanes_per_cell(const Mesh<T> &M) {
     int l = M.schedule_lpc;
-    if (l == 0) l = M.d.maxrow <= 4 ? 4 : (M.d.maxrow <= 8 ? 8 : 1);
+    if (l == 0) {
+        if (M.d.rowlen == 4 || M.d.rowlen == 6) l = M.d.rowlen;
+        else l = M.d.maxrow <= 4 ? 4 : (M.d.maxrow <= 8 ? 8 : 1);
+    }
     if (l != 1 && M.d.maxrow > l) l = 1;
-    return l == 4 || l == 8 ? l : 0;
+    return l == 4 || l == 6 || l == 8 ? l : 0;
 }
 
 template <class T> Mesh<T> *get(cupcfd_hip_mesh *h);
@@ -1169,7 +1176,8 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
     const int lpc = lanes_per_cell(*M);
-    const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
+    const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
         const bool last = (it == nGradient - 1);
@@ -1177,11 +1185,14 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
         const bool s0 = (it == 0);
         if (lpc == 8) {
-            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 8>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
-            else hipLaunchKernelGGL((k_gradient_vec<T, false, 8>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 8>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_vec<T, false, 8>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
+        } else if (lpc == 6) {
+            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 6>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_vec<T, false, 6>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
         } else if (lpc == 4) {
-            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 4>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
-            else hipLaunchKernelGGL((k_gradient_vec<T, false, 4>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
+            if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 4>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_vec<T, false, 4>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
         } else {
             if (s0) hipLaunchKernelGGL((k_gradient<T, true>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
             else hipLaunchKernelGGL((k_gradient<T, false>), gr.blocks, kBlock, 0, s, d, ga, gr.per, gr.nblk);
@@ -1264,10 +1275,12 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
     }
     if (d.C) {
         const int lpc = lanes_per_cell(*M);
-        const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
+        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
-        else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 6) hipLaunchKernelGGL((k_scalar_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else hipLaunchKernelGGL(k_scalar<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
     }
     return launch_ok();
@@ -1284,10 +1297,12 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
                  gammaBlend, small};
     if (d.C) {
         const int lpc = lanes_per_cell(*M);
-        const Grid gr = xcd_grid(d.C, lpc ? kBlock / lpc : kBlock);
+        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
-        else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 6) hipLaunchKernelGGL((k_uvw_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else hipLaunchKernelGGL(k_uvw<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
     }
     return launch_ok();

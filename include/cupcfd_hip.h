@@ -156,8 +156,9 @@ int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
 /* Gather schedule of the per-cell face loops (gradient, scalar, UVW): lanes
- * evaluating one cell's faces in parallel, 4 or 8, or 1 for one thread per
- * cell; 0 (default) picks the smallest of 4 / 8 that holds the longest row.
+ * evaluating one cell's faces in parallel (4, 6 or 8; a row longer than that
+ * falls back to one thread per cell), or 1 for one thread per cell; 0
+ * (default) uses the row length of all-tet / all-hex meshes, else 8.
  * Every schedule gives bit-identical results. */
 int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
 /* copy mutable mesh state view → device (push) or device → view (pull) */
