@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--schedule", type=int, default=0,
+                    help="gather schedule: 0 automatic, 1 thread per cell, 4/6/8 lanes per cell, 64 cell tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -128,6 +130,7 @@ def main():
     t0 = time.time()
     host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
     dev = ch.DeviceMesh(host, 64, reorder=True)
+    dev.set_schedule(args.schedule)
     t_mesh = time.time() - t0
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
     Fi = int((host.a["faceBnd"] < 0).sum())

@@ -342,6 +342,80 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     d.orow = up.put(orow.data(), orow.size());
     d.oent = up.put(oent.data(), oent.size());
     d.noc = (int)ocells.size();
+
+    // ---- cell tiles ---------------------------------------------------------
+    {
+        const int TC = 64;
+        const int NT = (C + TC - 1) / TC;
+        std::vector<int> istart(C + 1, 0), bstart(C + 1, 0);  // first owned interior / boundary face per cell
+        for (int k = 0; k < Fi; k++) istart[fconn[k].x + 1]++;
+        for (int k = Fi; k < F; k++) bstart[fconn[k].x + 1]++;
+        for (int c = 0; c < C; c++) {
+            istart[c + 1] += istart[c];
+            bstart[c + 1] += bstart[c];
+        }
+        std::vector<int4> tinfo(NT);
+        for (int t = 0; t < NT; t++) {
+            const int c0 = t * TC, c1 = std::min(C, c0 + TC);
+            tinfo[t] = make_int4(istart[c0], istart[c1] - istart[c0], Fi + bstart[c0], bstart[c1] - bstart[c0]);
+        }
+        std::vector<int> thp(NT + 1, 0), thf;
+        for (int k = 0; k < Fi; k++)
+            if (fconn[k].y / TC != fconn[k].x / TC) thp[fconn[k].y / TC + 1]++;
+        for (int t = 0; t < NT; t++) thp[t + 1] += thp[t];
+        thf.assign(thp[NT], 0);
+        {
+            std::vector<int> hf(thp.begin(), thp.end() - 1);
+            for (int k = 0; k < Fi; k++)  // ascending k within each tile
+                if (fconn[k].y / TC != fconn[k].x / TC) thf[hf[fconn[k].y / TC]++] = k;
+        }
+        std::vector<int2> thc(thf.size());
+        for (size_t h = 0; h < thf.size(); h++) thc[h] = fconn[thf[h]];
+        int tslots = 0, tnnz = 0;
+        bool fits = true;
+        std::vector<uint16_t> eslot(nnz, 0);
+        for (int t = 0; t < NT && fits; t++) {
+            const int4 ti = tinfo[t];
+            const int nslot = ti.y + ti.w + (thp[t + 1] - thp[t]);
+            tslots = std::max(tslots, nslot);
+            if (nslot > 32767) fits = false;
+            const int c0 = t * TC, c1 = std::min(C, c0 + TC);
+            tnnz = std::max(tnnz, row[c1] - row[c0]);
+            // owned faces (whose per-face outputs this tile writes) have their owner in the tile
+            for (int k = ti.x; k < ti.x + ti.y; k++)
+                if (fconn[k].x < c0 || fconn[k].x >= c1) fits = false;
+            for (int k = ti.z; k < ti.z + ti.w; k++)
+                if (fconn[k].x < c0 || fconn[k].x >= c1) fits = false;
+            for (int c = c0; c < c1 && fits; c++)
+                for (int e = row[c]; e < row[c + 1]; e++) {
+                    const int k = ent[e].x >> 1, role = ent[e].x & 1;
+                    int slot;
+                    if (k >= Fi) slot = ti.y + (k - ti.z);
+                    else if (fconn[k].x / TC == t) slot = k - ti.x;
+                    else {
+                        const int *h0 = thf.data() + thp[t], *h1 = thf.data() + thp[t + 1];
+                        slot = ti.y + ti.w + (int)(std::lower_bound(h0, h1, k) - h0);
+                    }
+                    // the slot must name this entry's face the way the kernels decode it
+                    int back;
+                    if (slot < ti.y) back = ti.x + slot;
+                    else if (slot < ti.y + ti.w) back = ti.z + (slot - ti.y);
+                    else if (slot < nslot) back = thf[thp[t] + (slot - ti.y - ti.w)];
+                    else back = -1;
+                    if (back != k) fits = false;
+                    eslot[e] = (uint16_t)(slot | (role << 15));
+                }
+        }
+        d.tcells = TC;
+        d.ntiles = fits ? NT : 0;
+        d.tslots = tslots;
+        d.tnnz = tnnz;
+        d.thc = up.put(thc.data(), thc.size());
+        d.tinfo = up.put(tinfo.data(), tinfo.size());
+        d.thp = up.put(thp.data(), thp.size());
+        d.thf = up.put(thf.data(), thf.size());
+        d.eslot = up.put(eslot.data(), eslot.size());
+    }
     if (!up.ok) return CUPCFD_E_ERROR;
     M.radiab_h = radiab;
     return CUPCFD_E_SUCCESS;
@@ -538,7 +612,7 @@ extern "C" int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh) {
 extern "C" int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
     if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 6 &&
-        lanes_per_cell != 8)
+        lanes_per_cell != 8 && lanes_per_cell != 64)
         return CUPCFD_E_CONFIG_INVALID_VALUE;
     if (mesh->precision == 64) mesh->m64->schedule_lpc = lanes_per_cell;
     else mesh->m32->schedule_lpc = lanes_per_cell;

@@ -71,6 +71,20 @@ template <class T> struct DevMesh {
     // cells owning OUTLET boundaries, with those boundaries in ascending id
     const int *ocells, *orow, *oent;
     int noc;
+    // cell tiles for the LDS-staged gathers: tile t holds cells
+    // [t*tcells, (t+1)*tcells); its face slots are its owned interior faces,
+    // its owned boundary faces, then its halo faces (interior faces whose
+    // cell2 is in the tile and cell1 is not). tinfo[t] = (first owned interior
+    // face, count, first owned boundary face, count); halo faces of tile t are
+    // thf[thp[t] .. thp[t+1]) with their (cell1, cell2) in thc; eslot[e] is
+    // the slot of CSR entry e, with bit 15 set when the entry's cell is the
+    // face's cell2.
+    int tcells, ntiles, tslots;  // tslots: most slots of any tile
+    int tnnz;                    // most CSR entries of any tile
+    const int4 *tinfo;
+    const int *thp, *thf;
+    const int2 *thc;
+    const uint16_t *eslot;
 };
 
 // Host-side owner of a device mesh.
@@ -87,7 +101,7 @@ template <class T> struct Mesh {
     std::vector<int32_t> fconn_h;                     // 2F, host copy for carry precompute
     std::vector<int32_t> btype_h;                     // type per boundary
     int32_t ntype[5] = {0, 0, 0, 0, 0};               // boundary faces per region type
-    // gather schedule: lanes per cell (0 = automatic, 1 = one thread per cell, 4, 8)
+    // gather schedule: lanes per cell (0 = automatic, 1 = one thread per cell, 4, 6, 8; 64 = cell tiles)
     int schedule_lpc = 0;
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -153,33 +153,61 @@ template <class T> struct GradArgs {
     T *out1, *out2;
 };
 
+// the cell values an interior face of the gradient reads
+template <class T> struct GradCell {
+    T phi;
+    v3<T> go, cc;
+};
 template <class T, bool SWEEP0>
-__device__ __forceinline__ void gradient_terms(const DevMesh<T> &m, const GradArgs<T> &a, int c, int2 en, T t[3]) {
-    const int f = en.x >> 1;
-    const int2 cn = entry_cells(c, en);
-    const V4<T> s = m.fgeoS[f];
-    T phiF;
-    if (cn.y >= 0) {
-        const int ip = cn.x, in = cn.y;
-        const T facn = s.w;
-        const T facp = 1.0 - facn;
-        phiF = (a.phi[in] * facn) + (a.phi[ip] * facp);
-        if (!SWEEP0) {
-            const v3<T> xac = lerp3(xyz(m.cgeo[in]), facn, xyz(m.cgeo[ip]), facp);
-            const v3<T> g = lerp3(ld3(a.grado, in), facn, ld3(a.grado, ip), facp);
-            phiF += dot3(g, sub3(xyz(m.fgeoX[f]), xac));
-        }
-    } else {
-        phiF = a.phiB[~cn.y];
+__device__ __forceinline__ GradCell<T> grad_cell(const DevMesh<T> &m, const GradArgs<T> &a, int c) {
+    GradCell<T> r;
+    r.phi = a.phi[c];
+    if (!SWEEP0) {
+        r.go = ld3(a.grado, c);
+        r.cc = xyz(m.cgeo[c]);
+    }
+    return r;
+}
+
+// phiF * S of interior face f, the same for both of its cells
+template <class T, bool SWEEP0>
+__device__ __forceinline__ void gradient_interior(const V4<T> &xf, const V4<T> &s, const GradCell<T> &P,
+                                                  const GradCell<T> &N, T t[3]) {
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    T phiF = (N.phi * facn) + (P.phi * facp);
+    if (!SWEEP0) {
+        const v3<T> xac = lerp3(N.cc, facn, P.cc, facp);
+        const v3<T> g = lerp3(N.go, facn, P.go, facp);
+        phiF += dot3(g, sub3(xyz(xf), xac));
     }
     t[0] = phiF * s.x;
     t[1] = phiF * s.y;
     t[2] = phiF * s.z;
 }
 
-template <class T> __device__ __forceinline__ void gradient_store(const DevMesh<T> &m, const GradArgs<T> &a, int c,
-                                                                  int k, T acc) {
-    const T vol = m.cvol[c];
+template <class T>
+__device__ __forceinline__ void gradient_boundary(const GradArgs<T> &a, int ib, const V4<T> &s, T t[3]) {
+    const T phiF = a.phiB[ib];
+    t[0] = phiF * s.x;
+    t[1] = phiF * s.y;
+    t[2] = phiF * s.z;
+}
+
+template <class T, bool SWEEP0>
+__device__ __forceinline__ void gradient_terms(const DevMesh<T> &m, const GradArgs<T> &a, int c, int2 en, T t[3]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
+    const V4<T> s = m.fgeoS[f];
+    if (cn.y >= 0)
+        gradient_interior<T, SWEEP0>(SWEEP0 ? s : m.fgeoX[f], s, grad_cell<T, SWEEP0>(m, a, cn.x),
+                                     grad_cell<T, SWEEP0>(m, a, cn.y), t);
+    else
+        gradient_boundary(a, ~cn.y, s, t);
+}
+
+template <class T>
+__device__ __forceinline__ void gradient_store(const GradArgs<T> &a, int c, int k, T acc, T vol) {
     const T fact = 1.0 / vol;
     const T r = acc * fact;
     a.out1[3 * (size_t)c + k] = r;
@@ -202,7 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_gradient(DevMesh<T> m, GradArgs<T> a
         acc[1] = acc[1] + t[1];
         acc[2] = acc[2] + t[2];
     }
-    for (int k = 0; k < 3; k++) gradient_store(m, a, c, k, acc[k]);
+    const T vol = m.cvol[c];
+    for (int k = 0; k < 3; k++) gradient_store(a, c, k, acc[k], vol);
 }
 
 template <class T, bool SWEEP0, int LPC>
@@ -224,7 +253,163 @@ __global__ __launch_bounds__(LPC * 32) void k_gradient_vec(DevMesh<T> m, GradArg
     if (c < m.C && j < 3) {
         T acc = 0;
         for (int i = 0; i < n; i++) acc = acc + lds[(j * CPB + cl) * P + i];
-        gradient_store(m, a, c, j, acc);
+        gradient_store(a, c, j, acc, m.cvol[c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile schedule (all three gathers): block b works on tile t = xcd_block(b),
+// the cells [64t, 64t + 64), in three phases separated by barriers:
+//  A. every global load of the block that does not depend on another block:
+//     the tile's cell values (coalesced, into LDS), the CSR slots of its rows
+//     (into LDS), each thread's first face slot with that face's values and
+//     the values of the face's cell outside the tile (held in registers), and
+//     each fold item's starting value;
+//  B. each face slot of the tile (owned interior faces, owned boundary faces,
+//     halo faces) is evaluated once, its terms in the cell1 view kept in LDS;
+//     per-face outputs are written by the owning tile only;
+//  C. each (cell, output component) folds its row in face order, reading the
+//     terms of slot eslot[e] (negated for the cell2 side where the reference's
+//     cell2 update is the cell1 update with every term's sign flipped).
+// ---------------------------------------------------------------------------
+constexpr int kTileCells = 64;
+static_assert(kBlock >= 4 * kTileCells, "phase A gives each thread at most one element of a cell field");
+
+struct TileSlots {
+    int4 ti;  // first owned interior face, count, first owned boundary face, count
+    int h0, nh, nslot;
+};
+template <class T> __device__ __forceinline__ TileSlots tile_slots(const DevMesh<T> &m, int t) {
+    TileSlots r;
+    r.ti = m.tinfo[t];
+    r.h0 = m.thp[t];
+    r.nh = m.thp[t + 1] - r.h0;
+    r.nslot = r.ti.y + r.ti.w + r.nh;
+    return r;
+}
+// face of slot sl with its (cell1, cell2 or ~boundary); `owned` is false for halo slots
+template <class T>
+__device__ __forceinline__ int slot_conn(const DevMesh<T> &m, const TileSlots &ts, int sl, int2 &cn, bool &owned) {
+    owned = true;
+    int f;
+    if (sl < ts.ti.y) {
+        f = ts.ti.x + sl;
+    } else if (sl < ts.ti.y + ts.ti.w) {
+        f = ts.ti.z + (sl - ts.ti.y);
+    } else {
+        const int h = ts.h0 + (sl - ts.ti.y - ts.ti.w);
+        owned = false;
+        cn = m.thc[h];
+        return m.thf[h];
+    }
+    cn = m.fconn[f];
+    return f;
+}
+__device__ __forceinline__ bool in_tile(int c, int c0, int nc) { return (unsigned)(c - c0) < (unsigned)nc; }
+// the cell of a slot's face outside the tile, or -1 (a slot has at most one)
+__device__ __forceinline__ int outside_cell(int2 cn, int c0, int nc) {
+    if (!in_tile(cn.x, c0, nc)) return cn.x;
+    if (cn.y >= 0 && !in_tile(cn.y, c0, nc)) return cn.y;
+    return -1;
+}
+// phase-A staging of one cell field, of a packed 3-vector field (3 LDS
+// fields) and of the cell centres (3 LDS fields): one element per thread
+template <class T> __device__ __forceinline__ void stage1(T *l, const T *p, int c0, int nc) {
+    const int i = threadIdx.x;
+    if (i < nc) l[i] = p[c0 + i];
+}
+template <class T> __device__ __forceinline__ void stage3(T *l, const T *p, int c0, int nc) {
+    const int i = threadIdx.x;
+    if (i < 3 * nc) l[(i % 3) * kTileCells + i / 3] = p[3 * (size_t)c0 + i];
+}
+template <class T> __device__ __forceinline__ void stage_centre(T *l, const V4<T> *g, int c0, int nc) {
+    const int i = threadIdx.x;
+    if (i < 4 * nc && (i & 3) != 3) l[(i & 3) * kTileCells + (i >> 2)] = ((const T *)(g + c0))[i];
+}
+// the CSR slots of the tile's rows into LDS; returns the tile's first entry
+template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> &m, uint16_t *le, int c0, int nc) {
+    int e0, e1;
+    if (m.rowlen) {
+        e0 = c0 * m.rowlen;
+        e1 = (c0 + nc) * m.rowlen;
+    } else {
+        e0 = m.row[c0];
+        e1 = m.row[c0 + nc];
+    }
+    for (int i = threadIdx.x; i < e1 - e0; i += kBlock) le[i] = m.eslot[e0 + i];
+    return e0;
+}
+
+template <class T, bool SWEEP0>
+__global__ __launch_bounds__(kBlock) void k_gradient_tile(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TC = kTileCells, NF = SWEEP0 ? 1 : 7;  // phi (+ dPhidxo xyz, centre xyz)
+    const int S = m.tslots;
+    T *lc = (T *)smem;                        // [NF][TC]
+    T *lt = lc + NF * TC;                     // [3][S]
+    uint16_t *le = (uint16_t *)(lt + 3 * S);  // [tnnz]
+    const int t = xcd_block(blockIdx.x, per);
+    if (t >= nblk) return;
+    const int tid = threadIdx.x;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    // A
+    stage1(lc, a.phi, c0, nc);
+    if (!SWEEP0) {
+        stage3(lc + TC, a.grado, c0, nc);
+        stage_centre(lc + 4 * TC, m.cgeo, c0, nc);
+    }
+    const int eb = stage_eslots(m, le, c0, nc);
+    int f = 0, out = -1;
+    int2 cn = make_int2(0, 0);
+    bool owned;
+    V4<T> s{}, xf{};
+    GradCell<T> o{};
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        s = m.fgeoS[f];
+        out = -1;
+        if (cn.y >= 0) {
+            if (!SWEEP0) xf = m.fgeoX[f];
+            out = outside_cell(cn, c0, nc);
+            if (out >= 0) o = grad_cell<T, SWEEP0>(m, a, out);
+        }
+    };
+    if (tid < ts.nslot) load(tid);
+    const int k = tid / TC, cl = tid % TC;
+    const bool fold = tid < 3 * TC && cl < nc;
+    int e0 = 0, n = 0;
+    T vol = 1;
+    if (fold) {
+        row_range(m, c0 + cl, e0, n);
+        vol = m.cvol[c0 + cl];
+    }
+    __syncthreads();
+    // B
+    auto cell = [&](int c) -> GradCell<T> {
+        if (c == out) return o;
+        const int l = c - c0;
+        GradCell<T> r;
+        r.phi = lc[l];
+        if (!SWEEP0) {
+            r.go = v3<T>{lc[1 * TC + l], lc[2 * TC + l], lc[3 * TC + l]};
+            r.cc = v3<T>{lc[4 * TC + l], lc[5 * TC + l], lc[6 * TC + l]};
+        }
+        return r;
+    };
+    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+        if (sl != tid) load(sl);
+        T tt[3];
+        if (cn.y >= 0) gradient_interior<T, SWEEP0>(xf, s, cell(cn.x), cell(cn.y), tt);
+        else gradient_boundary(a, ~cn.y, s, tt);
+        for (int q = 0; q < 3; q++) lt[q * S + sl] = tt[q];
+    }
+    __syncthreads();
+    // C
+    if (fold) {
+        T acc = 0;
+        for (int e = e0; e < e0 + n; e++) acc = acc + lt[k * S + (le[e - eb] & 0x7fff)];
+        gradient_store(a, c0 + cl, k, acc, vol);
     }
 }
 
@@ -531,62 +716,65 @@ template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarAr
     return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
 }
 
-// Terms of one face of the scalar loop for the cell the entry belongs to:
-// t[0..2] the three additions to Su, t[3] the addition to Au. Returns
-// whether the face updates Au (INLET, enthalpy WALL). Per-face outputs
-// (RFace, PhiB, mesh Q/H/T) are written here by the face's owner.
+// the cell values an interior face of the scalar loop reads
+template <class T> struct ScalarCell {
+    T phi, vis;
+    v3<T> g, cc;
+};
 template <class T>
-__device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarArgs<T> &a, int c, int2 en,
-                                             T t[4]) {
-    const int f = en.x >> 1;
-    const int2 cn = entry_cells(c, en);
-    const V4<T> s = m.fgeoS[f];
+__device__ __forceinline__ ScalarCell<T> scalar_cell(const DevMesh<T> &m, const ScalarArgs<T> &a, int c) {
+    return ScalarCell<T>{a.phi[c], a.visEff[c], ld3(a.dPhidx, c), xyz(m.cgeo[c])};
+}
+
+// Interior face f of the scalar loop (ScalarKernels.ipp:135-217): terms of
+// the cell1 update Su[ip] = ((Su[ip] - blend) + fde1) - fdi; the cell2 update
+// is the same three terms negated. Writes RFace when `owner`.
+template <class T>
+__device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, const V4<T> &s,
+                                                T mf, T rl,
+                                                const ScalarCell<T> &P, const ScalarCell<T> &N, bool owner,
+                                                T t[3]) {
     const v3<T> S{s.x, s.y, s.z};
-    const T mf = a.massFlux[f];
-    t[0] = t[1] = t[2] = t[3] = nzero<T>();
-    if (cn.y >= 0) {
-        const int ip = cn.x, in = cn.y;
-        const T facn = s.w;
-        const T facp = 1.0 - facn;
-        T Visac = a.visEff[in] * facn + a.visEff[ip] * facp;
-        Visac = scalar_visac(a, Visac);
-        const v3<T> g = lerp3(ld3(a.dPhidx, in), facn, ld3(a.dPhidx, ip), facp);
-        const v3<T> Xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-        const T VisFace = Visac * m.frl[f];
-        const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
-        const T fce = mf * PhiFace;
-        const T fde1 = Visac * dot3(g, S);
-        const T fci = fmin((double)mf, 0.0) * (double)a.phi[in] + fmax((double)mf, 0.0) * (double)a.phi[ip];
-        const T fdi = VisFace * dot3(g, Xpn);
-        const T blend = a.gammaBlend * (fce - fci);
-        if ((en.x & 1) == 0) {
-            // Su[ip] = ((Su[ip] - blend) + fde1) - fdi
-            t[0] = -blend;
-            t[1] = fde1;
-            t[2] = -fdi;
-            a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
-            a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
-        } else {
-            // Su[in] = ((Su[in] + blend) - fde1) + fdi
-            t[0] = blend;
-            t[1] = -fde1;
-            t[2] = fdi;
-        }
-        return false;
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    T Visac = N.vis * facn + P.vis * facp;
+    Visac = scalar_visac(a, Visac);
+    const v3<T> g = lerp3(N.g, facn, P.g, facp);
+    const v3<T> Xpn = sub3(N.cc, P.cc);
+    const T VisFace = Visac * rl;
+    const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
+    const T fce = mf * PhiFace;
+    const T fde1 = Visac * dot3(g, S);
+    const T fci = fmin((double)mf, 0.0) * (double)N.phi + fmax((double)mf, 0.0) * (double)P.phi;
+    const T fdi = VisFace * dot3(g, Xpn);
+    const T blend = a.gammaBlend * (fce - fci);
+    t[0] = -blend;
+    t[1] = fde1;
+    t[2] = -fdi;
+    if (owner) {
+        a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
+        a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
     }
-    const int ip = cn.x;
-    const int ib = ~cn.y;
+}
+
+// Boundary face f of cell P = cell1 (ScalarKernels.ipp:219-447): t[0..2] the
+// additions to Su, t[3] to Au; returns whether Au is updated (INLET, enthalpy
+// WALL). Writes PhiB and the mesh's Q/H/T.
+template <class T>
+__device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, const V4<T> &s,
+                                                int ip, const ScalarCell<T> &P, int ib, T t[4]) {
+    const v3<T> S{s.x, s.y, s.z};
+    t[0] = t[1] = t[2] = t[3] = nzero<T>();
     const int ir = m.bregion[ib];
     const int it = m.rtype[ir];
     if (it == CUPCFD_RTYPE_INLET) {
-        const v3<T> g = ld3(a.dPhidx, ip);
+        const T mf = a.massFlux[f];
         const T PhiFace = a.inletAssigns ? (T)(0) : scalar_carry(m, a, f);
         const T Visac = scalar_visac(a, a.visEffB[ib]);
-        const V4<T> xf = m.fgeoX[f];
-        const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
+        const v3<T> Xpn = sub3(xyz(m.fgeoX[f]), P.cc);
         const T VisFace = Visac * m.frl[f];
-        const T fde = Visac * dot3(g, S);
-        const T fdi = VisFace * dot3(g, Xpn);
+        const T fde = Visac * dot3(P.g, S);
+        const T fdi = VisFace * dot3(P.g, Xpn);
         const T fF = -VisFace + fmin((double)mf, 0.0);
         // Au = Au - f;  Su = ((Su + fde) - fdi) - f * PhiFace
         t[0] = fde;
@@ -597,26 +785,24 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         return true;
     }
     if (it == CUPCFD_RTYPE_OUTLET) {
-        const v3<T> g = ld3(a.dPhidx, ip);
-        const T Visac = scalar_visac(a, a.visEff[ip]);
-        const V4<T> xf = m.fgeoX[f];
-        const v3<T> Xpn = sub3(xyz(xf), xyz(m.cgeo[ip]));
-        const T PhiFace = a.phi[ip] + dot3(g, Xpn);
+        const T Visac = scalar_visac(a, P.vis);
+        const v3<T> Xpn = sub3(xyz(m.fgeoX[f]), P.cc);
+        const T PhiFace = P.phi + dot3(P.g, Xpn);
         const T VisFace = Visac * m.frl[f];
         // Su = (Su + fde) - fdi
-        t[0] = Visac * dot3(g, S);
-        t[1] = -(VisFace * dot3(g, Xpn));
+        t[0] = Visac * dot3(P.g, S);
+        t[1] = -(VisFace * dot3(P.g, Xpn));
         a.phiB[ib] = PhiFace;
         return false;
     }
     if (it == CUPCFD_RTYPE_SYMP) {
-        const v3<T> ds = sub3(xyz(m.fgeoX[f]), xyz(m.cgeo[ip]));
-        a.phiB[ib] = a.phi[ip] + dot3(ld3(a.dPhidx, ip), ds);
+        const v3<T> ds = sub3(xyz(m.fgeoX[f]), P.cc);
+        a.phiB[ib] = P.phi + dot3(P.g, ds);
         return false;
     }
     if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
         if (m.radiab[ir]) {
-            a.phiB[ib] = a.phi[ip];
+            a.phiB[ib] = P.phi;
             m.bq[ib] = 0.0;
             return false;
         }
@@ -628,7 +814,7 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         // Au = Au + Hcoef;  Su = Su + Hcoef * PhiFace
         t[0] = Hcoef * PhiFace;
         t[3] = Hcoef;
-        const T Tdif = (PhiFace - a.phi[ip]);
+        const T Tdif = (PhiFace - P.phi);
         const T h1 = Hcoef * cp / area;
         m.bq[ib] = h1 * Tdif;
         m.bh[ib] = VisFace * cp;
@@ -636,6 +822,27 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
         return true;
     }
     return false;
+}
+
+// Terms of CSR entry en of cell c (serial and lane-per-face schedules)
+template <class T>
+__device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarArgs<T> &a, int c, int2 en, T t[4]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
+    const V4<T> s = m.fgeoS[f];
+    if (cn.y >= 0) {
+        const bool in = (en.x & 1) != 0;
+        scalar_interior(m, a, f, s, a.massFlux[f], m.frl[f], scalar_cell(m, a, cn.x), scalar_cell(m, a, cn.y), !in,
+                        t);
+        if (in) {
+            t[0] = -t[0];
+            t[1] = -t[1];
+            t[2] = -t[2];
+        }
+        t[3] = nzero<T>();
+        return false;
+    }
+    return scalar_boundary(m, a, f, s, cn.x, scalar_cell(m, a, cn.x), ~cn.y, t);
 }
 
 template <class T>
@@ -700,6 +907,101 @@ __global__ __launch_bounds__(LPC * 32) void k_scalar_vec(DevMesh<T> m, ScalarArg
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TC = kTileCells, NF = 8;  // phi, VisEff, dPhidx xyz, centre xyz
+    const int S = m.tslots;
+    T *lc = (T *)smem;                        // [NF][TC]
+    T *lt = lc + NF * TC;                     // [4][S]
+    uint16_t *le = (uint16_t *)(lt + 4 * S);  // [tnnz]
+    uint8_t *lf = (uint8_t *)(le + m.tnnz);   // [S]
+    const int t = xcd_block(blockIdx.x, per);
+    if (t >= nblk) return;
+    const int tid = threadIdx.x;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    // A
+    stage1(lc, a.phi, c0, nc);
+    stage1(lc + TC, a.visEff, c0, nc);
+    stage3(lc + 2 * TC, a.dPhidx, c0, nc);
+    stage_centre(lc + 5 * TC, m.cgeo, c0, nc);
+    const int eb = stage_eslots(m, le, c0, nc);
+    int f = 0, out = -1;
+    int2 cn = make_int2(0, 0);
+    bool owned;
+    V4<T> s{};
+    T mf = 0, rl = 0;
+    ScalarCell<T> o{};
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        s = m.fgeoS[f];
+        out = -1;
+        if (cn.y >= 0) {
+            mf = a.massFlux[f];
+            rl = m.frl[f];
+            out = outside_cell(cn, c0, nc);
+            if (out >= 0) o = scalar_cell(m, a, out);
+        }
+    };
+    if (tid < ts.nslot) load(tid);
+    const bool fold = tid < nc;
+    int e0 = 0, n = 0;
+    T su = 0;
+    if (fold) {
+        row_range(m, c0 + tid, e0, n);
+        su = a.su[c0 + tid];
+    }
+    __syncthreads();
+    // B
+    auto cell = [&](int c) -> ScalarCell<T> {
+        if (c == out) return o;
+        const int l = c - c0;
+        return ScalarCell<T>{lc[l], lc[TC + l], v3<T>{lc[2 * TC + l], lc[3 * TC + l], lc[4 * TC + l]},
+                             v3<T>{lc[5 * TC + l], lc[6 * TC + l], lc[7 * TC + l]}};
+    };
+    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+        if (sl != tid) load(sl);
+        T tt[4];
+        bool fl = false;
+        if (cn.y >= 0) {
+            scalar_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, tt);
+            tt[3] = nzero<T>();
+        } else {
+            fl = scalar_boundary(m, a, f, s, cn.x, cell(cn.x), ~cn.y, tt);
+        }
+        for (int q = 0; q < 4; q++) lt[q * S + sl] = tt[q];
+        lf[sl] = fl ? 1 : 0;
+    }
+    __syncthreads();
+    // C: Su, then Au for cells with an Au-updating face
+    if (fold) {
+        const int c = c0 + tid;
+        bool any = false;
+        for (int e = e0; e < e0 + n; e++) {
+            const int es = le[e - eb];
+            const int sl = es & 0x7fff;
+            T t0 = lt[sl], t1 = lt[S + sl], t2 = lt[2 * S + sl];
+            if (es & 0x8000) {
+                t0 = -t0;
+                t1 = -t1;
+                t2 = -t2;
+            }
+            su = ((su + t0) + t1) + t2;
+            any |= lf[sl] != 0;
+        }
+        a.su[c] = su;
+        if (any) {
+            T au = a.au[c];
+            for (int e = e0; e < e0 + n; e++) {
+                const int sl = le[e - eb] & 0x7fff;
+                if (lf[sl]) au = au + lt[3 * S + sl];
+            }
+            a.au[c] = au;
+        }
+    }
+}
+
 // ===========================================================================
 // a10 FluxUVWDolfynFaceLoop1 — UVWKernels.ipp:38-515
 // ===========================================================================
@@ -739,70 +1041,76 @@ template <class T> __device__ T uvw_visface_carry(const DevMesh<T> &m, const Uvw
     return visac * rl;
 }
 
-// Terms of one face of the UVW loop for the cell the entry belongs to:
-// t[3c..3c+2] the three additions to su/sv/sw (component c), t[9] the addition
-// to au, av and aw alike. Returns whether the face updates au/av/aw.
-// Per-face outputs (rFace, uB/vB/wB, the OUTLET massFlux clamp, boundary
-// shear) are written here by the face's owner.
+// the cell values an interior face of the UVW loop reads
+template <class T> struct UvwCell {
+    T u, v, w, vis;
+    v3<T> du, dv, dw, cc;
+};
 template <class T>
-__device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> &a, int c, int2 en, T t[10]) {
-    const int f = en.x >> 1;
-    const int2 cn = entry_cells(c, en);
-    const V4<T> s = m.fgeoS[f];
+__device__ __forceinline__ UvwCell<T> uvw_cell(const DevMesh<T> &m, const UvwArgs<T> &a, int c) {
+    return UvwCell<T>{a.u[c], a.v[c], a.w[c], a.visEff[c],
+                      ld3(a.dudx, c), ld3(a.dvdx, c), ld3(a.dwdx, c), xyz(m.cgeo[c])};
+}
+
+// Interior face f of the UVW loop (UVWKernels.ipp:128-230): t[3k..3k+2] the
+// terms of the cell1 update su[ip] = ((su[ip] - blendU) + fude) - fudi (sv,
+// sw alike); the cell2 update is the same terms negated. Writes rFace when
+// `owner`.
+template <class T>
+__device__ __forceinline__ void uvw_interior(const DevMesh<T> &m, const UvwArgs<T> &a, int f, const V4<T> &s, T mf,
+                                             T rl,
+                                             const UvwCell<T> &P, const UvwCell<T> &N, bool owner, T t[9]) {
+    const T sx = s.x, sy = s.y, sz = s.z;
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    const v3<T> du = lerp3(N.du, facn, P.du, facp);
+    const v3<T> dv = lerp3(N.dv, facn, P.dv, facp);
+    const v3<T> dw = lerp3(N.dw, facn, P.dw, facp);
+    const T visac = N.vis * facn + P.vis * facp;
+    const v3<T> xpn = sub3(N.cc, P.cc);
+    const T visFace = visac * rl;
+    const v3<T> uf = uvw_carry(m, a, f);
+    const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
+    const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
+    const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
+    const T fwde1 = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
+    const T fude = visac * fude1, fvde = visac * fvde1, fwde = visac * fwde1;
+    const T fmn = smin(mf, (T)(0.0)), fmx = smax(mf, (T)(0.0));
+    const T fuci = fmn * N.u + fmx * P.u;
+    const T fvci = fmn * N.v + fmx * P.v;
+    const T fwci = fmn * N.w + fmx * P.w;
+    const T fudi = visFace * dot3(du, xpn);
+    const T fvdi = visFace * dot3(dv, xpn);
+    const T fwdi = visFace * dot3(dw, xpn);
+    const T blendU = a.gammaBlend * (fuce - fuci);
+    const T blendV = a.gammaBlend * (fvce - fvci);
+    const T blendW = a.gammaBlend * (fwce - fwci);
+    t[0] = -blendU, t[1] = fude, t[2] = -fudi;
+    t[3] = -blendV, t[4] = fvde, t[5] = -fvdi;
+    t[6] = -blendW, t[7] = fwde, t[8] = -fwdi;
+    if (owner) {
+        a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
+        a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
+    }
+}
+
+// Boundary face f of cell P = cell1 (UVWKernels.ipp:232-515): t[0..8] the
+// additions to su/sv/sw, t[9] to au, av and aw alike; returns whether the face
+// updates au/av/aw. Writes uB/vB/wB, the OUTLET massFlux clamp and the wall
+// shear.
+template <class T>
+__device__ __forceinline__ bool uvw_boundary(const DevMesh<T> &m, const UvwArgs<T> &a, int f, const V4<T> &s,
+                                             const UvwCell<T> &P, int ib, T t[10]) {
     const T sx = s.x, sy = s.y, sz = s.z;
     for (int k = 0; k < 10; k++) t[k] = nzero<T>();
-    if (cn.y >= 0) {
-        const int ip = cn.x, in = cn.y;
-        const T facn = s.w;
-        const T facp = 1.0 - facn;
-        const v3<T> du = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
-        const v3<T> dv = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
-        const v3<T> dw = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
-        const T visac = a.visEff[in] * facn + a.visEff[ip] * facp;
-        const v3<T> xpn = sub3(xyz(m.cgeo[in]), xyz(m.cgeo[ip]));
-        const T visFace = visac * m.frl[f];
-        const T mf = a.massFlux[f];
-        const v3<T> uf = uvw_carry(m, a, f);
-        const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
-        const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
-        const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
-        const T fwde1 = (du.z + dw.x) * sx + (dw.y + dv.z) * sy + (dw.z + dw.z) * sz;
-        const T fude = visac * fude1, fvde = visac * fvde1, fwde = visac * fwde1;
-        const T fmn = smin(mf, (T)(0.0)), fmx = smax(mf, (T)(0.0));
-        const T fuci = fmn * a.u[in] + fmx * a.u[ip];
-        const T fvci = fmn * a.v[in] + fmx * a.v[ip];
-        const T fwci = fmn * a.w[in] + fmx * a.w[ip];
-        const T fudi = visFace * dot3(du, xpn);
-        const T fvdi = visFace * dot3(dv, xpn);
-        const T fwdi = visFace * dot3(dw, xpn);
-        const T blendU = a.gammaBlend * (fuce - fuci);
-        const T blendV = a.gammaBlend * (fvce - fvci);
-        const T blendW = a.gammaBlend * (fwce - fwci);
-        if ((en.x & 1) == 0) {
-            // su[ip] = ((su[ip] - blendU) + fude) - fudi   (sv, sw alike)
-            t[0] = -blendU, t[1] = fude, t[2] = -fudi;
-            t[3] = -blendV, t[4] = fvde, t[5] = -fvdi;
-            t[6] = -blendW, t[7] = fwde, t[8] = -fwdi;
-            a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
-            a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
-        } else {
-            // su[in] = ((su[in] + blendU) - fude) + fudi
-            t[0] = blendU, t[1] = -fude, t[2] = fudi;
-            t[3] = blendV, t[4] = -fvde, t[5] = fvdi;
-            t[6] = blendW, t[7] = -fwde, t[8] = fwdi;
-        }
-        return false;
-    }
-    const int ip = cn.x;
-    const int ib = ~cn.y;
     const int ir = m.bregion[ib];
     const int it = m.rtype[ir];
     if (it == CUPCFD_RTYPE_DEFAULT) return false;
     const V4<T> xf = m.fgeoX[f];
-    const v3<T> cp = xyz(m.cgeo[ip]);
-    const T up = a.u[ip], vp = a.v[ip], wp = a.w[ip];
+    const v3<T> cp = P.cc;
+    const T up = P.u, vp = P.v, wp = P.w;
     if (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) {
-        const v3<T> du = ld3(a.dudx, ip), dv = ld3(a.dvdx, ip), dw = ld3(a.dwdx, ip);
+        const v3<T> du = P.du, dv = P.dv, dw = P.dw;
         v3<T> uf;
         T visac;
         if (it == CUPCFD_RTYPE_INLET) {
@@ -810,7 +1118,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
             visac = a.visEffB[ib];
         } else {
             uf = v3<T>{up, vp, wp};
-            visac = a.visEff[ip];
+            visac = P.vis;
         }
         const v3<T> xpn = sub3(xyz(xf), cp);
         const T visFace = visac * m.frl[f];
@@ -841,17 +1149,17 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
     }
     if (it == CUPCFD_RTYPE_SYMP) {
         const v3<T> xpn = sub3(xyz(xf), cp);
-        const T visac = a.visEff[ip];
-        const T du = dot3(ld3(a.dudx, ip), xpn);
-        const T dv = dot3(ld3(a.dvdx, ip), xpn);
-        const T dw = dot3(ld3(a.dwdx, ip), xpn);
+        const T visac = P.vis;
+        const T du = dot3(P.du, xpn);
+        const T dv = dot3(P.dv, xpn);
+        const T dw = dot3(P.dw, xpn);
         const v3<T> us{up + du, vp + dv, wp + dw};
         const v3<T> xn = normalise3(v3<T>{(T)(0.0 - sx), (T)(0.0 - sy), (T)(0.0 - sz)});
         const T dp = dot3(us, xn);
         const v3<T> un{dp * xn.x, dp * xn.y, dp * xn.z};
         const T dn = m.bdist[ib];
         const T twoVis = (T)(2.0 * visac);
-        const T area = m.fgeoX[f].w;
+        const T area = xf.w;
         const v3<T> force{((twoVis * un.x) / dn) * area, ((twoVis * un.y) / dn) * area,
                           ((twoVis * un.z) / dn) * area};
         const T visFace = uvw_visface_carry(m, a, f);
@@ -870,7 +1178,7 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
     const T visac = a.visEffB[ib];
     const v3<T> xpn = sub3(xyz(xf), cp);
     const T xpn_length = (T)length3(xpn);
-    const T area = m.fgeoX[f].w;
+    const T area = xf.w;
     const T coef = visac * area / xpn_length;
     const v3<T> xn = normalise3(v3<T>{sx, sy, sz});
     const v3<T> upr = sub3(v3<T>{up, vp, wp}, uw);
@@ -893,6 +1201,23 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
     a.vB[ib] = uw.y;
     a.wB[ib] = uw.z;
     return true;
+}
+
+// Terms of CSR entry en of cell c (serial and lane-per-face schedules)
+template <class T>
+__device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> &a, int c, int2 en, T t[10]) {
+    const int f = en.x >> 1;
+    const int2 cn = entry_cells(c, en);
+    const V4<T> s = m.fgeoS[f];
+    if (cn.y >= 0) {
+        const bool in = (en.x & 1) != 0;
+        uvw_interior(m, a, f, s, a.massFlux[f], m.frl[f], uvw_cell(m, a, cn.x), uvw_cell(m, a, cn.y), !in, t);
+        if (in)
+            for (int k = 0; k < 9; k++) t[k] = -t[k];
+        t[9] = nzero<T>();
+        return false;
+    }
+    return uvw_boundary(m, a, f, s, uvw_cell(m, a, cn.x), ~cn.y, t);
 }
 
 template <class T>
@@ -968,6 +1293,117 @@ __global__ __launch_bounds__(LPC * 32) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a
                     if (flag[cl * P + i]) acc = acc + lds[(9 * CPB + cl) * P + i];
                 dst[c] = acc;
             }
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TC = kTileCells, NF = 16;  // u v w VisEff, dudx dvdx dwdx centre (xyz each)
+    const int S = m.tslots;
+    T *lc = (T *)smem;                         // [NF][TC]
+    T *lt = lc + NF * TC;                      // [10][S]
+    uint16_t *le = (uint16_t *)(lt + 10 * S);  // [tnnz]
+    uint8_t *lf = (uint8_t *)(le + m.tnnz);    // [S]
+    const int t = xcd_block(blockIdx.x, per);
+    if (t >= nblk) return;
+    const int tid = threadIdx.x;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    // A
+    stage1(lc, a.u, c0, nc);
+    stage1(lc + TC, a.v, c0, nc);
+    stage1(lc + 2 * TC, a.w, c0, nc);
+    stage1(lc + 3 * TC, a.visEff, c0, nc);
+    stage3(lc + 4 * TC, a.dudx, c0, nc);
+    stage3(lc + 7 * TC, a.dvdx, c0, nc);
+    stage3(lc + 10 * TC, a.dwdx, c0, nc);
+    stage_centre(lc + 13 * TC, m.cgeo, c0, nc);
+    const int eb = stage_eslots(m, le, c0, nc);
+    int f = 0, out = -1;
+    int2 cn = make_int2(0, 0);
+    bool owned;
+    V4<T> s{};
+    T mf = 0, rl = 0;
+    UvwCell<T> o{};
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        s = m.fgeoS[f];
+        out = -1;
+        if (cn.y >= 0) {
+            mf = a.massFlux[f];
+            rl = m.frl[f];
+            out = outside_cell(cn, c0, nc);
+            if (out >= 0) o = uvw_cell(m, a, out);
+        }
+    };
+    if (tid < ts.nslot) load(tid);
+    // fold item tid < 3 * TC: component k of cell cl
+    const int k = tid / TC, cl = tid % TC;
+    const bool fold = tid < 3 * TC && cl < nc;
+    T *dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
+    int e0 = 0, n = 0;
+    T acc = 0;
+    if (fold) {
+        row_range(m, c0 + cl, e0, n);
+        acc = dst[c0 + cl];
+    }
+    __syncthreads();
+    // B
+    auto cell = [&](int c) -> UvwCell<T> {
+        if (c == out) return o;
+        const T *q = lc + (c - c0);
+        return UvwCell<T>{q[0],
+                          q[TC],
+                          q[2 * TC],
+                          q[3 * TC],
+                          v3<T>{q[4 * TC], q[5 * TC], q[6 * TC]},
+                          v3<T>{q[7 * TC], q[8 * TC], q[9 * TC]},
+                          v3<T>{q[10 * TC], q[11 * TC], q[12 * TC]},
+                          v3<T>{q[13 * TC], q[14 * TC], q[15 * TC]}};
+    };
+    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+        if (sl != tid) load(sl);
+        T tt[10];
+        bool fl = false;
+        if (cn.y >= 0) {
+            uvw_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, tt);
+            tt[9] = nzero<T>();
+        } else {
+            fl = uvw_boundary(m, a, f, s, cell(cn.x), ~cn.y, tt);
+        }
+        for (int q = 0; q < 10; q++) lt[q * S + sl] = tt[q];
+        lf[sl] = fl ? 1 : 0;
+    }
+    __syncthreads();
+    // C: su/sv/sw component k, then au/av/aw component k for cells with an
+    // a-updating face
+    if (fold) {
+        const int c = c0 + cl;
+        const T *l0 = lt + (3 * k) * S, *l1 = l0 + S, *l2 = l1 + S;
+        bool any = false;
+        for (int e = e0; e < e0 + n; e++) {
+            const int es = le[e - eb];
+            const int sl = es & 0x7fff;
+            T t0 = l0[sl], t1 = l1[sl], t2 = l2[sl];
+            if (es & 0x8000) {
+                t0 = -t0;
+                t1 = -t1;
+                t2 = -t2;
+            }
+            acc = ((acc + t0) + t1) + t2;
+            any |= lf[sl] != 0;
+        }
+        dst[c] = acc;
+        if (any) {
+            T *dsta = k == 0 ? a.au : (k == 1 ? a.av : a.aw);
+            T aa = dsta[c];
+            for (int e = e0; e < e0 + n; e++) {
+                const int sl = le[e - eb] & 0x7fff;
+                if (lf[sl]) aa = aa + lt[9 * S + sl];
+            }
+            dsta[c] = aa;
         }
     }
 }
@@ -1136,12 +1572,30 @@ inline Grid xcd_grid(int64_t items, int per_block) {
     return Grid{per * kXcd, per, nblk};
 }
 
-// Lanes per cell for the gathers: the mesh's schedule if set (1 = one thread
-// per cell); else the row length of uniform 4- or 6-face meshes (every lane
-// busy), else 8 if every row fits, else one thread per cell. Returns 0 for
-// the one-thread-per-cell kernels.
-template <class T> int lanes_per_cell(const Mesh<T> &M) {
+// Dynamic LDS of a tile kernel: nf cell fields, nt term arrays of tslots
+// entries, the tile's CSR slots and (flags) one byte per slot; 0 if above a
+// workgroup's 64 KiB
+template <class T> size_t tile_lds(const DevMesh<T> &d, int nf, int nt, bool flags) {
+    const size_t b = ((size_t)nf * kTileCells + (size_t)nt * d.tslots) * sizeof(T) + 2 * (size_t)d.tnnz +
+                     (flags ? (size_t)d.tslots : 0);
+    return b <= 65536 ? b : 0;
+}
+
+// Gather schedule of gradient, scalar and UVW. Automatic (mesh schedule 0)
+// and 64 take the cell-tile kernels when the mesh's tiles fit their slot
+// index and `lds` bytes of LDS (set here); otherwise lanes per cell: the mesh's
+// schedule if set (1 = one thread per cell), else the row length of uniform 4-
+// or 6-face meshes (every lane busy), else 8 if every row fits, else one
+// thread per cell. Returns kTile, 4, 6, 8, or 0 for one thread per cell.
+constexpr int kTile = kTileCells;
+template <class T> int gather_schedule(const Mesh<T> &M, int nf, int nt, bool flags, size_t &lds) {
     int l = M.schedule_lpc;
+    lds = 0;
+    if (l == kTile || l == 0) {
+        if (M.d.ntiles > 0) lds = tile_lds(M.d, nf, nt, flags);
+        if (lds) return kTile;
+        l = 0;
+    }
     if (l == 0) {
         if (M.d.rowlen == 4 || M.d.rowlen == 6) l = M.d.rowlen;
         else l = M.d.maxrow <= 4 ? 4 : (M.d.maxrow <= 8 ? 8 : 1);
@@ -1175,16 +1629,20 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         t[1] = (T *)M->ws_get(43, 3 * (size_t)d.C * sizeof(T));
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
-    const int lpc = lanes_per_cell(*M);
-    const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
+    size_t lds;
+    const int lpc = gather_schedule(*M, 7, 3, false, lds);
+    const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+    const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
         const bool last = (it == nGradient - 1);
         T *dst = last ? g : t[it & 1];
         GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
         const bool s0 = (it == 0);
-        if (lpc == 8) {
+        if (lpc == kTile) {
+            if (s0) hipLaunchKernelGGL((k_gradient_tile<T, true>), gr.blocks, nthr, lds, s, d, ga, gr.per, gr.nblk);
+            else hipLaunchKernelGGL((k_gradient_tile<T, false>), gr.blocks, nthr, lds, s, d, ga, gr.per, gr.nblk);
+        } else if (lpc == 8) {
             if (s0) hipLaunchKernelGGL((k_gradient_vec<T, true, 8>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
             else hipLaunchKernelGGL((k_gradient_vec<T, false, 8>), gr.blocks, nthr, 0, s, d, ga, gr.per, gr.nblk);
         } else if (lpc == 6) {
@@ -1274,11 +1732,13 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         if (!a.src) return CUPCFD_E_ERROR;
     }
     if (d.C) {
-        const int lpc = lanes_per_cell(*M);
-        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
+        size_t lds;
+        const int lpc = gather_schedule(*M, 8, 4, true, lds);
+        const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+        const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        if (lpc == kTile) hipLaunchKernelGGL(k_scalar_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 6) hipLaunchKernelGGL((k_scalar_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else hipLaunchKernelGGL(k_scalar<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
@@ -1296,11 +1756,13 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
     UvwArgs<T> a{u, v, w, visEff, visEffB, dudx, dvdx, dwdx, uB, vB, wB, massFlux, rFace, su, sv, sw, au, av, aw,
                  gammaBlend, small};
     if (d.C) {
-        const int lpc = lanes_per_cell(*M);
-        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
+        size_t lds;
+        const int lpc = gather_schedule(*M, 16, 10, true, lds);
+        const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+        const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        if (lpc == kTile) hipLaunchKernelGGL(k_uvw_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
+        else if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 6) hipLaunchKernelGGL((k_uvw_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else hipLaunchKernelGGL(k_uvw<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);

@@ -322,7 +322,7 @@ class DeviceMesh:
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
 
     def set_schedule(self, lanes_per_cell):
-        """0 = automatic, 1 = one thread per cell, 4 or 8 lanes per cell."""
+        """0 = automatic, 1 = one thread per cell, 4, 6 or 8 lanes per cell, 64 = 64-cell tiles."""
         check("set_schedule", lib().cupcfd_hip_mesh_set_schedule(self._h, int(lanes_per_cell)))
 
     def push_state(self):

@@ -155,11 +155,14 @@ int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t
 int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
-/* Gather schedule of the per-cell face loops (gradient, scalar, UVW): lanes
- * evaluating one cell's faces in parallel (4, 6 or 8; a row longer than that
- * falls back to one thread per cell), or 1 for one thread per cell; 0
- * (default) uses the row length of all-tet / all-hex meshes, else 8.
- * Every schedule gives bit-identical results. */
+/* Gather schedule of the per-cell face loops (gradient, scalar, UVW): 64 for
+ * tiles of 64 cells whose values are staged in LDS and whose faces are each
+ * evaluated once; lanes evaluating one cell's faces in parallel (4, 6 or 8; a
+ * row longer than that falls back to one thread per cell); or 1 for one
+ * thread per cell. 0 (default) takes tiles when the mesh's tiles fit (at most
+ * 32767 face slots per tile and 64 KiB of LDS), else the row length of
+ * all-tet / all-hex meshes, else 8. Every schedule gives bit-identical
+ * results. */
 int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
 /* copy mutable mesh state view → device (push) or device → view (pull) */
 int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh);

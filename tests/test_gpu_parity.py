@@ -150,10 +150,10 @@ def test_size_errors_are_reported():
 @pytest.mark.gpu
 @pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32"])
 def test_schedules_bit_identical(meshname):
-    """One thread per cell and 6 or 8 lanes per cell give the same bits."""
+    """One thread per cell, 6 or 8 lanes per cell and 64-cell tiles give the same bits."""
     host, _ = _setup(meshname, 64)
     outs = {}
-    for lpc in (1, 6, 8):
+    for lpc in (1, 6, 8, 64):
         dev = ch.DeviceMesh(host, 64)
         dev.set_schedule(lpc)
         for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
@@ -162,5 +162,5 @@ def test_schedules_bit_identical(meshname):
             outs[(lpc, kernel)] = run(kernel, fvm, dev, f, lambda: host.state(64))[1]
     for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
         for k in outs[(1, kernel)]:
-            for lpc in (6, 8):
+            for lpc in (6, 8, 64):
                 assert compare(outs[(1, kernel)][k], outs[(lpc, kernel)][k])[0], (kernel, k, lpc)
