@@ -104,6 +104,108 @@ def cpu_baseline(host, fields, budget_s=20.0):
     }
 
 
+def batched_fields(host, K, seed):
+    """K scalar fields phi_k = sum_{m=1..8} a_m sin(k_m . x + theta_m) + 0.05 U(-1,1) at cell centres (and
+    boundary face centres), with seeded wave vectors, phases and amplitudes; returned scalar-innermost."""
+    rng = np.random.default_rng(seed)
+    cc = host.a["cellCenter"]
+    fc = host.a["faceCenter"][host.a["bndFace"]]
+    phi = np.empty((host.nCells, K))
+    phiB = np.empty((host.nBoundaries, K))
+    for k in range(K):
+        kv = rng.uniform(-4 * np.pi, 4 * np.pi, (8, 3))
+        th = rng.uniform(0, 2 * np.pi, 8)
+        am = rng.uniform(0.1, 1.0, 8)
+        phi[:, k] = np.sin(cc @ kv.T + th) @ am + 0.05 * rng.uniform(-1, 1, host.nCells)
+        phiB[:, k] = np.sin(fc @ kv.T + th) @ am + 0.05 * rng.uniform(-1, 1, host.nBoundaries)
+    return phi, phiB
+
+
+def bench_batched(torch, n, K, steps, warmup):
+    """K scalars through gradient + scalar flux on a relabelled StructGen n^3 mesh (FP64): one batched call of
+    each against K single-scalar calls on K separate arrays. The scalars share the mass flux of one
+    FluxMassDolfynFaceLoop call. Returns per-scalar cell throughputs and their ratio."""
+    import cupcfd_hip as ch
+    from cupcfd_hip import workload as W
+    from cupcfd_hip.fvm import resident as R
+
+    host = ch.HostMesh(n, seed=4242, permute_cells=True, permute_faces=True)
+    dev = ch.DeviceMesh(host, 64, reorder=True)
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    f = W.Fields(host, seed=99)
+    co, fo = dev.cell_order, dev.face_order
+    cuda = torch.device("cuda:0")
+    dt = torch.float64
+
+    def cells(a):
+        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda)
+
+    # shared mass flux from one mass face loop
+    mf = torch.from_numpy(np.ascontiguousarray(f.massFlux[fo])).to(cuda)
+    su0 = cells(f.su)
+    rf = torch.zeros(2 * F, dtype=dt, device=cuda)
+    R.mass(dev, cells(f.dudx), cells(f.dvdx), cells(f.dwdx), cells(f.dpdx), cells(f.den),
+           torch.from_numpy(f.denB).to(cuda), cells(f.u), cells(f.v), cells(f.w), mf, cells(f.p), cells(f.ar), su0, rf)
+    phi, phiB = batched_fields(host, K, seed=2024)
+    phi_b = cells(phi)                                   # (C, K)
+    phiB_b = torch.from_numpy(phiB).to(cuda)             # (B, K)
+    g_b = torch.zeros((C, 3, K), dtype=dt, device=cuda)
+    go_b = torch.zeros_like(g_b)
+    su_b = torch.zeros((C, K), dtype=dt, device=cuda)
+    au_b = torch.ones((C, K), dtype=dt, device=cuda)
+    pB_b = phiB_b.clone()
+    sh = dict(visEff=cells(f.visEff), den=cells(f.den), te=cells(f.te),
+              cpB=torch.from_numpy(f.cpB).to(cuda), visEffB=torch.from_numpy(f.visEffB).to(cuda))
+    rface = torch.zeros(2 * F, dtype=dt, device=cuda)
+    # K separate single-scalar arrays (no shared cache footprint between calls)
+    one = [dict(phi=phi_b[:, k].contiguous(), phiB=phiB_b[:, k].contiguous(),
+                g=torch.zeros((C, 3), dtype=dt, device=cuda), go=torch.zeros((C, 3), dtype=dt, device=cuda),
+                su=torch.zeros(C, dtype=dt, device=cuda), au=torch.ones(C, dtype=dt, device=cuda),
+                pB=phiB_b[:, k].contiguous()) for k in range(K)]
+    stream = torch.cuda.current_stream()
+
+    def run_batched():
+        R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream)
+        R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"], sh["cpB"],
+                         sh["visEffB"], rface, g_b, W.SCALAR, stream)
+
+    def run_plain():
+        for o in one:
+            R.gradient(dev, 1, o["phi"], o["phiB"], o["g"], o["go"], stream)
+            R.scalar(dev, o["phi"], o["pB"], sh["visEff"], o["au"], o["su"], sh["den"], mf, sh["te"], sh["cpB"],
+                     sh["visEffB"], rface, o["g"], W.SCALAR, stream)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    tb = timed(run_batched)
+    tp = timed(run_plain)
+    # the batched results are those of the single-scalar calls, bit for bit (both ran the same number of
+    # accumulating calls)
+    same = all(torch.equal(g_b[:, :, k], one[k]["g"]) and torch.equal(su_b[:, k], one[k]["su"])
+               for k in (0, K // 2, K - 1))
+    tbg = timed(lambda: R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream))
+    tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
+                                         sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
+    return {
+        "workload": f"gradient + scalar flux of K={K} scalars, StructGen {n}^3 relabelled, FP64, shared mass flux",
+        "cells": C, "K": K,
+        "batched_ms": tb * 1e3, "plain_ms": tp * 1e3,
+        "batched_gcell_scalar_per_s": C * K / tb / 1e9, "plain_gcell_scalar_per_s": C * K / tp / 1e9,
+        "speedup": tp / tb, "bit_identical_sample": bool(same),
+        "batched_gradient_ms": tbg * 1e3, "batched_scalar_ms": tbs * 1e3,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +214,10 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched K-scalar comparison")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched K-scalar comparison")
+    ap.add_argument("--batch-n", type=int, default=96)
+    ap.add_argument("--batch-k", type=int, default=64)
     ap.add_argument("--schedule", type=int, default=0,
                     help="gather schedule: 0 automatic, 1 thread per cell, 4/6/8 lanes per cell, 64 cell tiles")
     args = ap.parse_args()
@@ -126,6 +232,10 @@ def main():
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(torch, args.batch_n, args.batch_k, steps=args.steps,
+                                                   warmup=args.warmup)}))
+        return
     n = args.n
     t0 = time.time()
     host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
@@ -244,6 +354,9 @@ def main():
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(torch, args.batch_n, args.batch_k, steps=max(2, min(args.steps, 10)),
+                                       warmup=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, f, args.cpu_budget)
         out["cpu_baseline"]["cores"] = 1

@@ -36,6 +36,12 @@ template <class T> struct v3 { T x, y, z; };
 template <class T> __device__ __forceinline__ v3<T> ld3(const T *__restrict__ p, int i) {
     return v3<T>{p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]};
 }
+// 3-vector i of an array of vectors whose components are `ks` apart
+// (components of one scalar in a scalar-innermost batch of ks scalars)
+template <class T> __device__ __forceinline__ v3<T> ld3s(const T *p, int i, int ks) {
+    const size_t b = (size_t)3 * i * ks;
+    return v3<T>{p[b], p[b + ks], p[b + 2 * (size_t)ks]};
+}
 template <class T> __device__ __forceinline__ void st3(T *__restrict__ p, int i, v3<T> v) {
     p[3 * (size_t)i] = v.x;
     p[3 * (size_t)i + 1] = v.y;
@@ -649,22 +655,26 @@ template <class T> struct ScalarArgs {
     const T *phi, *visEff, *den, *massFlux, *te, *cpB, *visEffB, *dPhidx;
     T *phiB, *au, *su, *rface;
     const int *src;  // PhiFace source per face (typed meshes), else null
+    // stride of the per-scalar arrays (phi, PhiB, Au, Su, dPhidx): 1 for one
+    // scalar, K for scalar-innermost batches; write_state: this lane writes
+    // the mesh's boundary Q/H/T
+    int ks, write_state;
     int solveTurb, solveEnthalpy, ivar, varT, varTE, varED, inletAssigns, wallActive;
     T visLam, sigmaT, prandtl, sigmaK, sigmaE, sigmaS, schmidt, gammaBlend, small;
     T slres, cmu25;  // wall-function constants, evaluated on the host
+    // the face diffusivity's divisors for this call's SolveTurb and ivar
+    // (ScalarKernels.ipp:143-167): Visac = visLam + (Visac - visLam) / visD1
+    // when SolveTurb, then Visac / visD2 when visHasD2
+    T visD1, visD2;
+    int visHasD2;
 };
 
 template <class T> __device__ __forceinline__ T scalar_visac(const ScalarArgs<T> &a, T Visac) {
     if (a.solveTurb) {
         Visac = Visac - a.visLam;
-        if (a.ivar == a.varT) Visac = (a.visLam + Visac / a.sigmaT) / a.prandtl;
-        else if (a.ivar == a.varTE) Visac = a.visLam + Visac / a.sigmaK;
-        else if (a.ivar == a.varED) Visac = a.visLam + Visac / a.sigmaE;
-        else Visac = (a.visLam + Visac / a.sigmaS) / a.schmidt;
-    } else {
-        if (a.ivar == a.varT) Visac = Visac / a.prandtl;
-        else Visac = Visac / a.schmidt;
+        Visac = a.visLam + Visac / a.visD1;
     }
+    if (a.visHasD2) Visac = Visac / a.visD2;
     return Visac;
 }
 
@@ -694,7 +704,7 @@ __device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T> &a, in
             Hcoef = 1.0 / (1.0 / VisFace + Resist * cp) * area;
         }
     }
-    if (m.rflux[ir]) PhiFace = a.phi[ip] + PhiFlux / (Hcoef * cp / area);
+    if (m.rflux[ir]) PhiFace = a.phi[(size_t)ip * a.ks] + PhiFlux / (Hcoef * cp / area);
     return PhiFace;
 }
 
@@ -710,7 +720,7 @@ template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarAr
     if (it == CUPCFD_RTYPE_INLET) return (T)(0);
     if (it == CUPCFD_RTYPE_OUTLET) {
         const v3<T> Xpn = sub3(xyz(m.fgeoX[g]), xyz(m.cgeo[ip]));
-        return a.phi[ip] + dot3(ld3(a.dPhidx, ip), Xpn);
+        return a.phi[(size_t)ip * a.ks] + dot3(ld3s(a.dPhidx, ip, a.ks), Xpn);
     }
     T H, V;
     return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
@@ -781,7 +791,7 @@ __device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const Scala
         t[1] = -fdi;
         t[2] = -(fF * PhiFace);
         t[3] = -fF;
-        a.phiB[ib] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = PhiFace;
         return true;
     }
     if (it == CUPCFD_RTYPE_OUTLET) {
@@ -792,33 +802,35 @@ __device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const Scala
         // Su = (Su + fde) - fdi
         t[0] = Visac * dot3(P.g, S);
         t[1] = -(VisFace * dot3(P.g, Xpn));
-        a.phiB[ib] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = PhiFace;
         return false;
     }
     if (it == CUPCFD_RTYPE_SYMP) {
         const v3<T> ds = sub3(xyz(m.fgeoX[f]), P.cc);
-        a.phiB[ib] = P.phi + dot3(P.g, ds);
+        a.phiB[(size_t)ib * a.ks] = P.phi + dot3(P.g, ds);
         return false;
     }
     if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
         if (m.radiab[ir]) {
-            a.phiB[ib] = P.phi;
-            m.bq[ib] = 0.0;
+            a.phiB[(size_t)ib * a.ks] = P.phi;
+            if (a.write_state) m.bq[ib] = 0.0;
             return false;
         }
         T Hcoef, VisFace;
         const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
         const T area = m.fgeoX[f].w;
         const T cp = a.cpB[ib];
-        a.phiB[ib] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = PhiFace;
         // Au = Au + Hcoef;  Su = Su + Hcoef * PhiFace
         t[0] = Hcoef * PhiFace;
         t[3] = Hcoef;
         const T Tdif = (PhiFace - P.phi);
         const T h1 = Hcoef * cp / area;
-        m.bq[ib] = h1 * Tdif;
-        m.bh[ib] = VisFace * cp;
-        m.bt[ib] = PhiFace;
+        if (a.write_state) {
+            m.bq[ib] = h1 * Tdif;
+            m.bh[ib] = VisFace * cp;
+            m.bt[ib] = PhiFace;
+        }
         return true;
     }
     return false;
@@ -1549,6 +1561,218 @@ __global__ __launch_bounds__(kBlock) void k_permute(int n, int ncomp, const int 
     }
 }
 
+// ===========================================================================
+// Batched gathers: K scalars on one mesh (no counterpart in the reference).
+// Scalar k's results are those of the plain kernel on scalar k's arrays, bit
+// for bit. Per-scalar arrays are scalar-innermost (phi[c*K + k], vectors
+// [(c*3 + j)*K + k]), so the G lanes that share a cell read one contiguous
+// run per field, and the cell's connectivity and geometry are read once for
+// all K. A group of G lanes (G = 4, 16 or 64; K > 64 takes blockIdx.y chunks
+// of 64) owns one cell and walks its CSR row in order, each lane folding its
+// own scalar exactly as the plain kernels' folds do.
+// ===========================================================================
+struct BatchLane {
+    int c, k;
+    bool live;
+};
+template <class T, int G>
+__device__ __forceinline__ BatchLane batch_lane(const DevMesh<T> &m, int K, int per, int nblk) {
+    BatchLane r;
+    const int lb = xcd_block(blockIdx.x, per);
+    r.c = lb * (kBlock / G) + threadIdx.x / G;
+    r.k = blockIdx.y * G + threadIdx.x % G;
+    r.live = lb < nblk && r.c < m.C && r.k < K;
+    return r;
+}
+
+// Row of cell c as the batched kernels walk it. RL > 0: rows of exactly RL
+// entries (uniform meshes), whose entries, face values and neighbour values
+// are all loaded before any term is computed (no store sits between them, so
+// every load of the row is in flight at once); RL = 0: any row, loaded entry
+// by entry. With G = 64 the cell index is made wave-uniform, so the cell's
+// connectivity and geometry are read with scalar loads.
+template <int G> __device__ __forceinline__ int batch_uniform(int c) {
+    return G == 64 ? __builtin_amdgcn_readfirstlane(c) : c;
+}
+
+template <class T, bool SWEEP0, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, GradArgs<T> a, int K, int per, int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    const int c = batch_uniform<G>(L.c);
+    const T *phi = a.phi + L.k, *phiB = a.phiB + L.k;
+    const T *go = SWEEP0 ? nullptr : a.grado + L.k;
+    auto cellv = [&](int x) -> GradCell<T> {
+        GradCell<T> r;
+        r.phi = phi[(size_t)x * K];
+        if (!SWEEP0) {
+            r.go = ld3s(go, x, K);
+            r.cc = xyz(m.cgeo[x]);
+        }
+        return r;
+    };
+    const GradCell<T> own = cellv(c);
+    T acc[3] = {0, 0, 0};
+    auto face = [&](int2 en, const V4<T> &s, const V4<T> &xf, const GradCell<T> &o) {
+        T t[3];
+        if (en.y >= 0) {
+            const bool in = (en.x & 1) != 0;
+            gradient_interior<T, SWEEP0>(xf, s, in ? o : own, in ? own : o, t);
+        } else {
+            t[0] = o.phi * s.x;  // o.phi holds phiB of the boundary
+            t[1] = o.phi * s.y;
+            t[2] = o.phi * s.z;
+        }
+        acc[0] = acc[0] + t[0];
+        acc[1] = acc[1] + t[1];
+        acc[2] = acc[2] + t[2];
+    };
+    auto other = [&](int2 en) -> GradCell<T> {
+        if (en.y >= 0) return cellv(en.y);
+        GradCell<T> r{};
+        r.phi = phiB[(size_t)(~en.y) * K];
+        return r;
+    };
+    int e0, n;
+    row_range(m, c, e0, n);
+    if (RL) {
+        int2 en[RL > 0 ? RL : 1];
+        V4<T> sv[RL > 0 ? RL : 1], xv[RL > 0 ? RL : 1];
+        GradCell<T> ov[RL > 0 ? RL : 1];
+#pragma unroll
+        for (int i = 0; i < RL; i++) en[i] = m.ent[e0 + i];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            const int f = en[i].x >> 1;
+            sv[i] = m.fgeoS[f];
+            if (!SWEEP0 && en[i].y >= 0) xv[i] = m.fgeoX[f];
+            ov[i] = other(en[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < RL; i++) face(en[i], sv[i], SWEEP0 ? sv[i] : xv[i], ov[i]);
+    } else {
+        for (int e = e0; e < e0 + n; e++) {
+            const int2 en = m.ent[e];
+            const int f = en.x >> 1;
+            const V4<T> s = m.fgeoS[f];
+            face(en, s, (SWEEP0 || en.y < 0) ? s : m.fgeoX[f], other(en));
+        }
+    }
+    const T vol = m.cvol[c];
+    const T fact = 1.0 / vol;
+    for (int j = 0; j < 3; j++) {
+        const T r = acc[j] * fact;
+        const size_t o = ((size_t)c * 3 + j) * K + L.k;
+        a.out1[o] = r;
+        if (a.out2) a.out2[o] = r;
+    }
+}
+
+// The parts of an interior face's scalar terms that do not depend on phi
+// (ScalarKernels.ipp:135-217), evaluated once per face for a whole batch by
+// k_scalar_coef with the expressions scalar_interior uses: the face
+// diffusivity Visac and VisFace, Xpn, the upwind weights min/max(mf, 0) (in
+// double, as the reference's std::min/max of a promoted mass flux) and, on
+// meshes without carried PhiFace, fce = mf * PhiFace.
+template <class T> struct ScalarFaceCoef {
+    T visac, visface, xpx, xpy, xpz, fce;
+    double fmn, fmx;
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scalar_coef(DevMesh<T> m, ScalarArgs<T> a, ScalarFaceCoef<T> *coef) {
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= m.Fi) return;
+    const int2 cn = m.fconn[f];
+    const T facn = m.fgeoS[f].w;
+    const T facp = 1.0 - facn;
+    T Visac = a.visEff[cn.y] * facn + a.visEff[cn.x] * facp;
+    Visac = scalar_visac(a, Visac);
+    const v3<T> Xpn = sub3(xyz(m.cgeo[cn.y]), xyz(m.cgeo[cn.x]));
+    const T VisFace = Visac * m.frl[f];
+    const T mf = a.massFlux[f];
+    ScalarFaceCoef<T> q;
+    q.visac = Visac;
+    q.visface = VisFace;
+    q.xpx = Xpn.x;
+    q.xpy = Xpn.y;
+    q.xpz = Xpn.z;
+    q.fce = m.typed ? (T)0 : mf * m.carry_init;
+    q.fmn = fmin((double)mf, 0.0);
+    q.fmx = fmax((double)mf, 0.0);
+    coef[f] = q;
+    a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
+    a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
+}
+
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarArgs<T> a0,
+                                                           const ScalarFaceCoef<T> *coef, int K, int per, int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    // this lane's view: per-scalar arrays offset to scalar k, stride K
+    ScalarArgs<T> a = a0;
+    a.phi += L.k;
+    a.phiB += L.k;
+    a.au += L.k;
+    a.su += L.k;
+    a.dPhidx += L.k;
+    a.ks = K;
+    a.write_state = (L.k == K - 1);  // K plain calls leave the last scalar's mesh state
+    const int c = batch_uniform<G>(L.c);
+    const ScalarCell<T> own{a.phi[(size_t)c * K], a.visEff[c], ld3s(a.dPhidx, c, K), xyz(m.cgeo[c])};
+    T su = a.su[(size_t)c * K];
+    T au = 0;
+    bool au_loaded = false;
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
+        const int2 en = m.ent[e];
+        const int f = en.x >> 1;
+        const V4<T> s = m.fgeoS[f];
+        T t[4];
+        bool ta = false;
+        if (en.y >= 0) {
+            // scalar_interior's terms with the face's phi-free parts from coef
+            const ScalarFaceCoef<T> q = coef[f];
+            const T ophi = a.phi[(size_t)en.y * K];
+            const v3<T> og = ld3s(a.dPhidx, en.y, K);
+            const bool in = (en.x & 1) != 0;
+            const T phiP = in ? ophi : own.phi, phiN = in ? own.phi : ophi;
+            const v3<T> gP = in ? og : own.g, gN = in ? own.g : og;
+            const v3<T> S{s.x, s.y, s.z};
+            const T facn = s.w;
+            const T facp = 1.0 - facn;
+            const v3<T> g = lerp3(gN, facn, gP, facp);
+            const T fce = m.typed ? a.massFlux[f] * scalar_carry(m, a, f) : q.fce;
+            const T fde1 = q.visac * dot3(g, S);
+            const T fci = q.fmn * (double)phiN + q.fmx * (double)phiP;
+            const T fdi = q.visface * dot3(g, v3<T>{q.xpx, q.xpy, q.xpz});
+            const T blend = a.gammaBlend * (fce - fci);
+            t[0] = -blend;
+            t[1] = fde1;
+            t[2] = -fdi;
+            if (in) {
+                t[0] = -t[0];
+                t[1] = -t[1];
+                t[2] = -t[2];
+            }
+        } else {
+            ta = scalar_boundary(m, a, f, s, c, own, ~en.y, t);
+        }
+        su = ((su + t[0]) + t[1]) + t[2];
+        if (ta) {
+            if (!au_loaded) {
+                au = a.au[(size_t)c * K];
+                au_loaded = true;
+            }
+            au = au + t[3];
+        }
+    }
+    a.su[(size_t)c * K] = su;
+    if (au_loaded) a.au[(size_t)c * K] = au;
+}
+
 }  // namespace cupcfd_hip
 
 // ===========================================================================
@@ -1684,16 +1908,14 @@ int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, co
     return launch_ok();
 }
 
+// The scalar loop's arguments for one call (ks = 1: one scalar; batches set
+// their own stride in the kernel)
 template <class T>
-int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den, const T *MF,
-               const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx, int SolveTurb,
-               int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k, int VarED,
-               T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large, T TMCmu, void *stream) {
-    (void)Large;
-    Mesh<T> *M = get<T>(h);
-    if (!M) return CUPCFD_E_NULL_PTR;
-    const DevMesh<T> &d = M->d;
-    ScalarArgs<T> a{};
+int scalar_args(Mesh<T> &M, ScalarArgs<T> &a, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den,
+                const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx, int SolveTurb,
+                int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k,
+                int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T TMCmu) {
+    a = ScalarArgs<T>{};
     a.phi = Phi;
     a.visEff = VisEff;
     a.den = Den;
@@ -1721,16 +1943,44 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
     a.sigmaE = Sigma_e;
     a.sigmaS = Sigma_s;
     a.schmidt = Schmidt;
+    if (SolveTurb) {
+        a.visD1 = ivar == VarT ? Sigma_T : ivar == VarTE ? Sigma_k : ivar == VarED ? Sigma_e : Sigma_s;
+        a.visD2 = ivar == VarT ? Prandtl : Schmidt;
+        a.visHasD2 = !(ivar == VarTE || ivar == VarED);
+    } else {
+        a.visD1 = 1;
+        a.visD2 = ivar == VarT ? Prandtl : Schmidt;
+        a.visHasD2 = 1;
+    }
     a.gammaBlend = GammaBlend;
     a.small = Small;
     // SLres and Cmu25 (ScalarKernels.ipp:394-395) are per-call constants
     a.slres = 9.24 * (pow((Prandtl / Sigma_T), 0.75) - 1.0) * (1.0 + 0.28 * exp(-0.007 * Prandtl / Sigma_T));
     a.cmu25 = pow(TMCmu, 0.25);
+    a.ks = 1;
+    a.write_state = 1;
     a.src = nullptr;
-    if (d.typed) {
-        a.src = scalar_carry_src<T>(*M, a.inletAssigns != 0, a.wallActive != 0);
+    if (M.d.typed) {
+        a.src = scalar_carry_src<T>(M, a.inletAssigns != 0, a.wallActive != 0);
         if (!a.src) return CUPCFD_E_ERROR;
     }
+    return CUPCFD_E_SUCCESS;
+}
+
+template <class T>
+int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den, const T *MF,
+               const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx, int SolveTurb,
+               int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k, int VarED,
+               T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large, T TMCmu, void *stream) {
+    (void)Large;
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    ScalarArgs<T> a;
+    const int rc = scalar_args(*M, a, Phi, PhiB, VisEff, Au, Su, Den, MF, TE, CpB, VisEffB, RFace, dPhidx, SolveTurb,
+                               SolveEnthalpy, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e,
+                               Sigma_s, Schmidt, GammaBlend, Small, TMCmu);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
     if (d.C) {
         size_t lds;
         const int lpc = gather_schedule(*M, 8, 4, true, lds);
@@ -1743,6 +1993,99 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else hipLaunchKernelGGL(k_scalar<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
     }
+    return launch_ok();
+}
+
+// Lanes per cell of the batched kernels: 4, 16 or 64 (the smallest that holds K)
+inline int batch_group(int K) { return K <= 4 ? 4 : (K <= 16 ? 16 : 64); }
+inline Grid batch_grid(int C, int G) { return xcd_grid(C, kBlock / G); }
+
+template <class T>
+int dev_gradient_batched(cupcfd_hip_mesh *h, int nGradient, int K, const T *phi, const T *phiB, T *g, T *go,
+                         void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    const DevMesh<T> &d = M->d;
+    hipStream_t s = (hipStream_t)stream;
+    if (d.C == 0) return CUPCFD_E_SUCCESS;
+    const size_t vbytes = 3 * (size_t)d.C * K * sizeof(T);
+    if (nGradient <= 0) return hipMemsetAsync(go, 0, vbytes, s) == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+    T *t[2] = {nullptr, nullptr};
+    if (nGradient > 1) {
+        t[0] = (T *)M->ws_get(44, vbytes);
+        t[1] = (T *)M->ws_get(45, vbytes);
+        if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
+    }
+    const int G = batch_group(K);
+    const int rl = d.rowlen;
+    const Grid gr = batch_grid(d.C, G);
+    const dim3 grid(gr.blocks, (K + G - 1) / G);
+    const T *prev = nullptr;
+    for (int it = 0; it < nGradient; it++) {
+        const bool last = (it == nGradient - 1);
+        T *dst = last ? g : t[it & 1];
+        GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
+        const bool s0 = (it == 0);
+#define CUPCFD_GB(GG, RR)                                                                                     \
+    if (s0)                                                                                                   \
+        hipLaunchKernelGGL((k_gradient_batched<T, true, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per,       \
+                           gr.nblk);                                                                          \
+    else                                                                                                      \
+        hipLaunchKernelGGL((k_gradient_batched<T, false, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per, gr.nblk);
+#define CUPCFD_GBR(GG)                                                                                        \
+    if (rl == 6) {                                                                                            \
+        CUPCFD_GB(GG, 6)                                                                                      \
+    } else if (rl == 4) {                                                                                     \
+        CUPCFD_GB(GG, 4)                                                                                      \
+    } else {                                                                                                  \
+        CUPCFD_GB(GG, 0)                                                                                      \
+    }
+        // the preloaded rows pay off when the row's connectivity is wave-uniform
+        // (G = 64); narrower groups walk the row entry by entry
+        if (G == 4) {
+            CUPCFD_GB(4, 0)
+        } else if (G == 16) {
+            CUPCFD_GB(16, 0)
+        } else {
+            CUPCFD_GBR(64)
+        }
+#undef CUPCFD_GBR
+#undef CUPCFD_GB
+        if (hipGetLastError() != hipSuccess) return CUPCFD_E_ERROR;
+        prev = dst;
+    }
+    return CUPCFD_E_SUCCESS;
+}
+
+template <class T>
+int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den,
+                       const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx,
+                       int SolveTurb, int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
+                       T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large,
+                       T TMCmu, void *stream) {
+    (void)Large;
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    const DevMesh<T> &d = M->d;
+    ScalarArgs<T> a;
+    const int rc = scalar_args(*M, a, Phi, PhiB, VisEff, Au, Su, Den, MF, TE, CpB, VisEffB, RFace, dPhidx, SolveTurb,
+                               SolveEnthalpy, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e,
+                               Sigma_s, Schmidt, GammaBlend, Small, TMCmu);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    if (d.C == 0) return CUPCFD_E_SUCCESS;
+    hipStream_t st = (hipStream_t)stream;
+    // the faces' phi-free coefficients, once for all K scalars (and RFace)
+    ScalarFaceCoef<T> *coef = (ScalarFaceCoef<T> *)M->ws_get(46, (size_t)(d.Fi ? d.Fi : 1) * sizeof(ScalarFaceCoef<T>));
+    if (!coef) return CUPCFD_E_ERROR;
+    if (d.Fi) hipLaunchKernelGGL(k_scalar_coef<T>, dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a, coef);
+    const int G = batch_group(K);
+    const Grid gr = batch_grid(d.C, G);
+    const dim3 grid(gr.blocks, (K + G - 1) / G);
+    if (G == 4) hipLaunchKernelGGL((k_scalar_batched<T, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    else if (G == 16) hipLaunchKernelGGL((k_scalar_batched<T, 16>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    else hipLaunchKernelGGL((k_scalar_batched<T, 64>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
     return launch_ok();
 }
 
@@ -1977,6 +2320,28 @@ DEV_API(double, f64)
 DEV_API(float, f32)
 
 #undef DEV_API
+
+#define BATCH_API(T, S)                                                                                            \
+    int cupcfd_dev_gradient_batched_##S(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K, const T *phiCell,      \
+                                        const T *phiBoundary, T *dPhidxCell, T *dPhidxoCell, void *stream) {       \
+        return dev_gradient_batched<T>(mesh, nGradient, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream); \
+    }                                                                                                              \
+    int cupcfd_dev_flux_scalar_batched_##S(                                                                        \
+        cupcfd_hip_mesh *mesh, int32_t K, const T *PhiCell, T *PhiBoundary, const T *VisEff, T *Au, T *Su,         \
+        const T *Den, const T *MassFlux, const T *TE, const T *CpBoundary, const T *visEffBoundary, T *RFace,       \
+        const T *dPhidx, int32_t SolveTurb, int32_t SolveEnthalpy, T vislam, int32_t ivar, int32_t VarT,           \
+        T Sigma_T, T Prandtl, int32_t VarTE, T Sigma_k, int32_t VarED, T Sigma_e, T Sigma_s, T Schmidt,            \
+        T GammaBlend, T Small, T Large, T TMCmu, void *stream) {                                                   \
+        return dev_scalar_batched<T>(mesh, K, PhiCell, PhiBoundary, VisEff, Au, Su, Den, MassFlux, TE, CpBoundary, \
+                                     visEffBoundary, RFace, dPhidx, SolveTurb, SolveEnthalpy, vislam, ivar, VarT,  \
+                                     Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt,           \
+                                     GammaBlend, Small, Large, TMCmu, stream);                                     \
+    }
+
+BATCH_API(double, f64)
+BATCH_API(float, f32)
+
+#undef BATCH_API
 
 const char *cupcfd_hip_version(void) { return "cupcfd_hip 0.1 gfx950"; }
 

@@ -118,12 +118,18 @@ HOST_SYMBOLS = [
 ]
 
 
+# batched resident entry points (cupcfd_dev_<name>_<f64|f32>)
+BATCHED = ["gradient_batched", "flux_scalar_batched"]
+
+
 def all_symbols():
     """Every symbol include/cupcfd_hip.h declares."""
     out = list(HOST_SYMBOLS)
     for s in ("f64", "f32"):
         for k in KERNELS:
             out.append(f"cupcfd_fvm_{k}_{s}")
+            out.append(f"cupcfd_dev_{k}_{s}")
+        for k in BATCHED:
             out.append(f"cupcfd_dev_{k}_{s}")
     return out
 
@@ -154,6 +160,8 @@ def _declare(L):
         for k in KERNELS:
             for mode in ("fvm", "dev"):
                 getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
+        for k in BATCHED:
+            getattr(L, f"cupcfd_dev_{k}_{s}").restype = ct.c_int
 
 
 def check(fn, code):

@@ -231,3 +231,22 @@ class resident:
         cls.call(mesh, "flux_uvw_face_loop1", mesh.handle, mesh.T(gammaBlend), mesh.T(small), t(u), t(v), t(w),
                  t(uB), t(vB), t(wB), t(visEff), t(visEffB), t(mf), t(dudx), t(dvdx), t(dwdx), t(rface), t(su),
                  t(sv), t(sw), t(au), t(av), t(aw), cls._s(stream))
+
+    @classmethod
+    def gradient_batched(cls, mesh, nGradient, K, phi, phiB, g, go, stream=None):
+        """K scalars: phi (C, K), phiB (B, K), g and go (C, 3, K), scalar-innermost."""
+        cls.call(mesh, "gradient_batched", mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB),
+                 cls._t(g), cls._t(go), cls._s(stream))
+
+    @classmethod
+    def scalar_batched(cls, mesh, K, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx, params,
+                       stream=None):
+        """K scalars: phi, au, su (C, K), phiB (B, K), dphidx (C, 3, K); the rest shared."""
+        t = cls._t
+        T = mesh.T
+        p = params
+        cls.call(mesh, "flux_scalar_batched", mesh.handle, _I(K), t(phi), t(phiB), t(visEff), t(au), t(su), t(den),
+                 t(mf), t(te), t(cpB), t(visEffB), t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]),
+                 T(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]), T(p["Sigma_T"]), T(p["Prandtl"]), _I(p["VarTE"]),
+                 T(p["Sigma_k"]), _I(p["VarED"]), T(p["Sigma_e"]), T(p["Sigma_s"]), T(p["Schmidt"]),
+                 T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"]), cls._s(stream))

@@ -308,6 +308,35 @@ int cupcfd_dev_viscosity_cell_loop2_##S(cupcfd_hip_mesh *mesh, T visLam, T *visE
 CUPCFD_FVM_API(double, cupcfd_v3_f64, f64)
 CUPCFD_FVM_API(float, cupcfd_v3_f32, f32)
 
+/* ---------------------------------------------------------------------------
+ * Batched kernels: K scalars on one mesh, resident mode (device arrays,
+ * internal numbering, async on stream). No counterpart in the reference: the
+ * result for scalar k is, bit for bit, that of K separate calls of
+ *   gradient_batched      GradientPhiGaussDolfyn     GradientKernels.h:48-54
+ *   flux_scalar_batched   FluxScalarDolfynFaceLoop   ScalarKernels.h:47-67
+ * on scalar k's arrays. Per-scalar arrays are scalar-innermost:
+ *   cell scalars     phiCell, Au, Su            [c*K + k]
+ *   boundary values  phiBoundary, PhiBoundary   [b*K + k]
+ *   cell vectors     dPhidxCell, dPhidxoCell, dPhidx   [(c*3 + j)*K + k]
+ * Shared by the K scalars (one array, as in the single-scalar entry points):
+ * VisEff, Den, MassFlux, TE, CpBoundary, visEffBoundary, and RFace (written
+ * once: it does not depend on phi). Mesh state written by the scalar loop
+ * (boundary Q/H/T) is left as scalar K-1's call leaves it. K >= 1; returns
+ * E_CONFIG_INVALID_VALUE for K < 1.
+ * ------------------------------------------------------------------------- */
+#define CUPCFD_BATCH_API(T, S)                                                                     \
+int cupcfd_dev_gradient_batched_##S(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K,           \
+    const T *phiCell, const T *phiBoundary, T *dPhidxCell, T *dPhidxoCell, void *stream);          \
+int cupcfd_dev_flux_scalar_batched_##S(cupcfd_hip_mesh *mesh, int32_t K,                           \
+    const T *PhiCell, T *PhiBoundary, const T *VisEff, T *Au, T *Su, const T *Den,                 \
+    const T *MassFlux, const T *TE, const T *CpBoundary, const T *visEffBoundary, T *RFace,        \
+    const T *dPhidx, int32_t SolveTurb, int32_t SolveEnthalpy, T vislam, int32_t ivar,             \
+    int32_t VarT, T Sigma_T, T Prandtl, int32_t VarTE, T Sigma_k, int32_t VarED, T Sigma_e,        \
+    T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large, T TMCmu, void *stream);
+
+CUPCFD_BATCH_API(double, f64)
+CUPCFD_BATCH_API(float, f32)
+
 /* Library version string and the gfx target it was built for. */
 const char *cupcfd_hip_version(void);
 

@@ -1,0 +1,138 @@
+"""Batched K-scalar kernels against K calls of the single-scalar kernels.
+
+cupcfd_dev_gradient_batched_* and cupcfd_dev_flux_scalar_batched_* promise,
+for every scalar k, the bits of the single-scalar resident entry point on
+scalar k's arrays (which test_gpu_parity.py pins to the oracle). Per-scalar
+arrays are scalar-innermost: (C, K), (B, K), (C, 3, K). Cases cover the
+three lane-group widths (K = 1, 3 -> 4 lanes; 16; 64) and a K above 64
+(several chunks of 64), on the reference test mesh, a typed relabelled mesh
+(carried PhiFace, INLET/OUTLET/SYMP/WALL branches) and a 32^3 typed mesh, in
+FP64 and FP32, with nGradient 1 and 3.
+"""
+import numpy as np
+import pytest
+
+import cupcfd_hip as ch
+from cupcfd_hip import workload as W
+from cupcfd_hip.fvm import resident as R
+from kernel_runs import seed_state
+
+torch = pytest.importorskip("torch")
+
+MESHES = {
+    "ref5": dict(nx=5, bounds=(-1, 1, -1, 1, -1, 1)),
+    "typed_perm": dict(nx=7, ny=6, nz=5, typed=True, permute_cells=True, permute_faces=True, face_extras=True,
+                       seed=11),
+    "typed32": dict(nx=32, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=3),
+}
+_cache = {}
+
+
+def _mesh(name, precision):
+    if (name, precision) not in _cache:
+        host = ch.HostMesh(**MESHES[name])
+        _cache[(name, precision)] = (host, ch.DeviceMesh(host, precision=precision, reorder=True))
+    return _cache[(name, precision)]
+
+
+def _dev(a, dt):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+
+
+def _same(a, b):
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("meshname", list(MESHES))
+@pytest.mark.parametrize("K", [1, 3, 16, 64, 70])
+@pytest.mark.parametrize("nG", [1, 3])
+def test_gradient_batched_matches_single(K, meshname, precision, nG):
+    if meshname == "typed32" and K > 16 and nG == 3:
+        pytest.skip("covered by the smaller meshes")
+    host, dev = _mesh(meshname, precision)
+    dt = np.float64 if precision == 64 else np.float32
+    C, B = host.nCells, host.nBoundaries
+    rng = np.random.default_rng(100 + K)
+    phi = rng.uniform(-1, 1, (C, K))
+    phiB = rng.uniform(-1, 1, (B, K))
+    g = torch.zeros((C, 3, K), dtype=torch.float64 if precision == 64 else torch.float32, device="cuda")
+    go = torch.full_like(g, 7.0)
+    R.gradient_batched(dev, nG, K, _dev(phi, dt), _dev(phiB, dt), g, go)
+    for k in range(K):
+        g1 = torch.zeros((C, 3), dtype=g.dtype, device="cuda")
+        go1 = torch.full_like(g1, 7.0)
+        R.gradient(dev, nG, _dev(phi[:, k], dt), _dev(phiB[:, k], dt), g1, go1)
+        torch.cuda.synchronize()
+        assert _same(g[:, :, k].contiguous(), g1), k
+        assert _same(go[:, :, k].contiguous(), go1), k
+
+
+def _scalar_case(host, dev, precision, K, p, seed):
+    dt = np.float64 if precision == 64 else np.float32
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    rng = np.random.default_rng(seed)
+    f = W.Fields(host, seed=seed)
+    per = dict(phi=rng.uniform(-1, 1, (C, K)), phiB=rng.uniform(-1, 1, (B, K)), au=rng.uniform(0.5, 1.5, (C, K)),
+               su=rng.uniform(-1, 1, (C, K)), dphidx=rng.uniform(-1, 1, (C, 3, K)))
+    shared = dict(visEff=f.visEff, den=f.den, mf=f.massFlux, te=f.te, cpB=f.cpB, visEffB=f.visEffB)
+    st0 = seed_state(host)
+
+    def reset():
+        host.set_state(st0, precision)
+        dev.push_state()
+
+    # batched
+    reset()
+    bt = {k: _dev(v, dt) for k, v in per.items()}
+    sh = {k: _dev(v, dt) for k, v in shared.items()}
+    rface_b = torch.zeros(2 * F, dtype=bt["phi"].dtype, device="cuda")
+    R.scalar_batched(dev, K, bt["phi"], bt["phiB"], sh["visEff"], bt["au"], bt["su"], sh["den"], sh["mf"], sh["te"],
+                     sh["cpB"], sh["visEffB"], rface_b, bt["dphidx"], p)
+    torch.cuda.synchronize()
+    dev.pull_state()
+    state_b = host.state(precision)
+    # K single-scalar calls
+    reset()
+    outs = []
+    for k in range(K):
+        one = {n: _dev(v[..., k], dt) for n, v in per.items()}
+        rface = torch.zeros(2 * F, dtype=rface_b.dtype, device="cuda")
+        R.scalar(dev, one["phi"], one["phiB"], sh["visEff"], one["au"], one["su"], sh["den"], sh["mf"], sh["te"],
+                 sh["cpB"], sh["visEffB"], rface, one["dphidx"], p)
+        outs.append((one, rface))
+    torch.cuda.synchronize()
+    dev.pull_state()
+    state_1 = host.state(precision)
+    for k, (one, rface) in enumerate(outs):
+        for n in ("su", "au", "phiB"):
+            assert _same(bt[n][..., k].contiguous(), one[n]), (n, k)
+        assert _same(rface_b, rface), ("RFace", k)
+    for n in state_b:
+        assert np.array_equal(state_b[n], state_1[n]), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("meshname", list(MESHES))
+@pytest.mark.parametrize("K", [1, 3, 16, 64, 70])
+@pytest.mark.parametrize("variant", ["scalar", "scalar_wall"])
+def test_scalar_batched_matches_single(K, meshname, precision, variant):
+    if meshname == "typed32" and K > 16:
+        pytest.skip("covered by the smaller meshes")
+    host, dev = _mesh(meshname, precision)
+    p = dict(W.SCALAR)
+    if variant == "scalar_wall":
+        p["ivar"] = p["VarT"]
+        p["SolveEnthalpy"] = 1
+    _scalar_case(host, dev, precision, K, p, seed=7 + K)
+
+
+@pytest.mark.gpu
+def test_batched_rejects_bad_k():
+    host, dev = _mesh("ref5", 64)
+    z = torch.zeros(8, dtype=torch.float64, device="cuda")
+    with pytest.raises(ch.CupCfdError):
+        R.gradient_batched(dev, 1, 0, z, z, z, z)
