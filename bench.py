@@ -206,6 +206,69 @@ def bench_batched(torch, n, K, steps, warmup):
     }
 
 
+def bench_adjoint(torch, n, steps, warmup):
+    """Forward (gradient nGradient=1 + scalar flux) against forward + adjoint w.r.t. (phi, massFlux) on a
+    relabelled StructGen n^3 mesh in FP64, seeded N(0,1) cotangents on Su and RFace."""
+    import cupcfd_hip as ch
+    from cupcfd_hip import workload as W
+    from cupcfd_hip.fvm import resident as R
+
+    host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
+    dev = ch.DeviceMesh(host, 64, reorder=True)
+    C, F = host.nCells, host.nFaces
+    f = W.Fields(host, seed=5)
+    co, fo = dev.cell_order, dev.face_order
+    cuda = torch.device("cuda:0")
+    dt = torch.float64
+
+    def cells(a):
+        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda)
+
+    phi, visEff, den, te = cells(f.phi), cells(f.visEff), cells(f.den), cells(f.te)
+    mf = torch.from_numpy(np.ascontiguousarray(f.massFlux[fo])).to(cuda)
+    phiB = torch.from_numpy(f.phiB).to(cuda)
+    cpB, visEffB = torch.from_numpy(f.cpB).to(cuda), torch.from_numpy(f.visEffB).to(cuda)
+    g = torch.empty(3 * C, dtype=dt, device=cuda)
+    go = torch.empty_like(g)
+    su = torch.zeros(C, dtype=dt, device=cuda)
+    au = torch.zeros(C, dtype=dt, device=cuda)
+    rface = torch.zeros(2 * F, dtype=dt, device=cuda)
+    pB = phiB.clone()
+    rng = np.random.default_rng(77)
+    sb = torch.from_numpy(rng.normal(size=C)).to(cuda)
+    rb = torch.from_numpy(rng.normal(size=2 * F)).to(cuda)
+    pbar = torch.empty(C, dtype=dt, device=cuda)
+    mbar = torch.empty(F, dtype=dt, device=cuda)
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        R.gradient(dev, 1, phi, phiB, g, go, stream)
+        R.scalar(dev, phi, pB, visEff, au, su, den, mf, te, cpB, visEffB, rface, g, W.SCALAR, stream)
+
+    def adj():
+        R.scalar_adjoint(dev, phi, visEff, mf, sb, rb, W.SCALAR, pbar, mbar, stream)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    tf = timed(fwd)
+    tfa = timed(lambda: (fwd(), adj()))
+    return {
+        "workload": f"gradient (nGradient=1) + scalar flux, forward and reverse w.r.t. (phi, massFlux), StructGen "
+                    f"{n}^3 relabelled, FP64, one DEFAULT region",
+        "cells": C, "forward_ms": tf * 1e3, "forward_adjoint_ms": tfa * 1e3, "ratio": tfa / tf,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +279,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched K-scalar comparison")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched K-scalar comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint comparison")
+    ap.add_argument("--adjoint-n", type=int, default=64)
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-k", type=int, default=64)
     ap.add_argument("--schedule", type=int, default=0,
@@ -357,6 +422,8 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(torch, args.batch_n, args.batch_k, steps=max(2, min(args.steps, 10)),
                                        warmup=1)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(torch, args.adjoint_n, steps=max(5, min(args.steps, 20)), warmup=2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, f, args.cpu_budget)
         out["cpu_baseline"]["cores"] = 1

@@ -1773,6 +1773,170 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarA
     if (au_loaded) a.au[(size_t)c * K] = au;
 }
 
+// ===========================================================================
+// Adjoint of GradientPhiGaussDolfyn (nGradient = 1) followed by
+// FluxScalarDolfynFaceLoop w.r.t. (phi, MassFlux), on all-DEFAULT meshes
+// (no reference counterpart; derived from GradientKernels.ipp:38-136 and
+// ScalarKernels.ipp:135-217). For interior face f (cell1 P, cell2 N):
+//   w_f = Visac*S - VisFace*Xpn,  D_f = Subar[P] - Subar[N]
+//   pass A (per cell c):  h_c = (1/vol_c) sum_f (D_f*fac)*w_f,
+//                         dir_c = sum_f (D_f*GammaBlend) * (c == P ? max(mf,0) : min(mf,0))
+//   pass B (per cell c):  phibar_c = dir_c + sum_f fac * dot(h_P + h_N, S_f)
+//   faces:                mfbar_f = -(D_f*GammaBlend*(PhiFace - dfci)) + rterm
+// with fac = facp for c == P, facn for c == N, and the sums over each cell's
+// faces in row order (ascending mesh face id). Boundary faces add nothing.
+// ===========================================================================
+template <class T> struct AdjArgs {
+    ScalarArgs<T> s;  // phi, visEff, massFlux and the diffusivity parameters
+    const T *subar, *rbar;
+    T *h, *dir, *phibar, *mfbar;
+};
+
+// Pass-A terms of CSR entry en of cell c: t[0..2] = (D*fac)*w, t[3] = the direct phi term
+template <class T>
+__device__ __forceinline__ void adj_a_terms(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en, T t[4]) {
+    if (en.y < 0) {
+        t[0] = t[1] = t[2] = t[3] = nzero<T>();
+        return;
+    }
+    const int f = en.x >> 1;
+    const bool in = (en.x & 1) != 0;
+    const int ip = in ? en.y : c, nn = in ? c : en.y;
+    const V4<T> s = m.fgeoS[f];
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    T Visac = a.s.visEff[nn] * facn + a.s.visEff[ip] * facp;
+    Visac = scalar_visac(a.s, Visac);
+    const T VisFace = Visac * m.frl[f];
+    const v3<T> Xpn = sub3(xyz(m.cgeo[nn]), xyz(m.cgeo[ip]));
+    const T D = a.subar[ip] - a.subar[nn];
+    const T Dg = D * a.s.gammaBlend;
+    const T fac = in ? facn : facp;
+    const T tt = D * fac;
+    t[0] = tt * (Visac * s.x - VisFace * Xpn.x);
+    t[1] = tt * (Visac * s.y - VisFace * Xpn.y);
+    t[2] = tt * (Visac * s.z - VisFace * Xpn.z);
+    const T mf = a.s.massFlux[f];
+    t[3] = Dg * (in ? (T)fmin((double)mf, 0.0) : (T)fmax((double)mf, 0.0));
+}
+
+// Pass-B term of CSR entry en of cell c
+template <class T>
+__device__ __forceinline__ T adj_b_term(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en) {
+    if (en.y < 0) return nzero<T>();
+    const int f = en.x >> 1;
+    const bool in = (en.x & 1) != 0;
+    const int ip = in ? en.y : c, nn = in ? c : en.y;
+    const V4<T> s = m.fgeoS[f];
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    const T fac = in ? facn : facp;
+    const v3<T> hp = ld3(a.h, ip), hn = ld3(a.h, nn);
+    const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
+    const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
+    return fac * pF;
+}
+
+template <class T> __device__ __forceinline__ void adj_a_store(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int k,
+                                                               T acc) {
+    if (k < 3) {
+        const T fact = 1.0 / m.cvol[c];
+        a.h[3 * (size_t)c + k] = acc * fact;
+    } else {
+        a.dir[c] = acc;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_a(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int c = lb * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T acc[4] = {0, 0, 0, 0};
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
+        T t[4];
+        adj_a_terms(m, a, c, m.ent[e], t);
+        for (int k = 0; k < 4; k++) acc[k] = acc[k] + t[k];
+    }
+    for (int k = 0; k < 4; k++) adj_a_store(m, a, c, k, acc[k]);
+}
+
+template <class T, int LPC>
+__global__ __launch_bounds__(LPC * 32) void k_adj_a_vec(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kCellsPerBlock, NT = 4, P = LPC + 1;
+    __shared__ T lds[NT * CPB * P];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
+    const int c = lb * CPB + cl;
+    int e0 = 0, n = 0;
+    if (c < m.C) row_range(m, c, e0, n);
+    if (j < n) {
+        T t[NT];
+        adj_a_terms(m, a, c, m.ent[e0 + j], t);
+        for (int k = 0; k < NT; k++) lds[(k * CPB + cl) * P + j] = t[k];
+    }
+    __syncthreads();
+    if (c >= m.C) return;
+    for (int k = j; k < NT; k += LPC) {
+        T acc = 0;
+        for (int i = 0; i < n; i++) acc = acc + lds[(k * CPB + cl) * P + i];
+        adj_a_store(m, a, c, k, acc);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_b(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int c = lb * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    T acc = a.dir[c];
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) acc = acc + adj_b_term(m, a, c, m.ent[e]);
+    a.phibar[c] = acc;
+}
+
+template <class T, int LPC>
+__global__ __launch_bounds__(LPC * 32) void k_adj_b_vec(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    constexpr int CPB = kCellsPerBlock, P = LPC + 1;
+    __shared__ T lds[CPB * P];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
+    const int c = lb * CPB + cl;
+    int e0 = 0, n = 0;
+    if (c < m.C) row_range(m, c, e0, n);
+    if (j < n) lds[cl * P + j] = adj_b_term(m, a, c, m.ent[e0 + j]);
+    __syncthreads();
+    if (c >= m.C || j != 0) return;
+    T acc = a.dir[c];
+    for (int i = 0; i < n; i++) acc = acc + lds[cl * P + i];
+    a.phibar[c] = acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_faces(DevMesh<T> m, AdjArgs<T> a) {
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= m.F) return;
+    const int2 cn = m.fconn[f];
+    if (cn.y < 0) {
+        a.mfbar[f] = 0.0;
+        return;
+    }
+    const T mf = a.s.massFlux[f];
+    const T D = a.subar[cn.x] - a.subar[cn.y];
+    const T Dg = D * a.s.gammaBlend;
+    const T dfci = mf < 0 ? a.s.phi[cn.y] : (mf > 0 ? a.s.phi[cn.x] : (T)0.0);
+    const T rterm = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
+    const T PhiFace = m.carry_init;
+    a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
+}
+
 }  // namespace cupcfd_hip
 
 // ===========================================================================
@@ -2113,6 +2277,62 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
     return launch_ok();
 }
 
+// Lanes per cell of the adjoint gathers (the lane-per-face schedule; a
+// tile schedule of the mesh maps to the row length)
+template <class T> int adj_lanes(const Mesh<T> &M) {
+    int l = M.schedule_lpc;
+    if (l == 0 || l == kTile) {
+        if (M.d.rowlen == 4 || M.d.rowlen == 6) l = M.d.rowlen;
+        else l = M.d.maxrow <= 4 ? 4 : (M.d.maxrow <= 8 ? 8 : 1);
+    }
+    if (l != 1 && M.d.maxrow > l) l = 1;
+    return l == 4 || l == 6 || l == 8 ? l : 0;
+}
+
+template <class T>
+int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const T *MF, const T *SuBar,
+                       const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
+                       T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T *PhiBar, T *MFBar,
+                       void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    if (d.typed) return CUPCFD_E_NOT_IMPLEMENTED;
+    AdjArgs<T> a{};
+    const int rc = scalar_args<T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, SolveTurb, 0, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE,
+                               Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, (T)0, (T)1);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    a.subar = SuBar;
+    a.rbar = RFaceBar;
+    a.phibar = PhiBar;
+    a.mfbar = MFBar;
+    hipStream_t st = (hipStream_t)stream;
+    if (d.C) {
+        a.h = (T *)M->ws_get(47, 3 * (size_t)d.C * sizeof(T));
+        a.dir = (T *)M->ws_get(48, (size_t)d.C * sizeof(T));
+        if (!a.h || !a.dir) return CUPCFD_E_ERROR;
+        const int lpc = adj_lanes(*M);
+        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+        const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
+        if (lpc == 8) {
+            hipLaunchKernelGGL((k_adj_a_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        } else if (lpc == 6) {
+            hipLaunchKernelGGL((k_adj_a_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        } else if (lpc == 4) {
+            hipLaunchKernelGGL((k_adj_a_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+        } else {
+            hipLaunchKernelGGL(k_adj_a<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+            hipLaunchKernelGGL(k_adj_b<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+        }
+    }
+    if (d.F) hipLaunchKernelGGL(k_adj_faces<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a);
+    return launch_ok();
+}
+
 int permute(cupcfd_hip_mesh *h, bool cells, const void *src, void *dst, int ncomp, int eb, void *stream, int dir) {
     if (!h || !src || !dst) return CUPCFD_E_NULL_PTR;
     if (ncomp < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
@@ -2342,6 +2562,17 @@ BATCH_API(double, f64)
 BATCH_API(float, f32)
 
 #undef BATCH_API
+
+int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
+                                  const double *MassFlux, const double *SuBar, const double *RFaceBar,
+                                  int32_t SolveTurb, double vislam, int32_t ivar, int32_t VarT, double Sigma_T,
+                                  double Prandtl, int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e,
+                                  double Sigma_s, double Schmidt, double GammaBlend, double *PhiBar,
+                                  double *MassFluxBar, void *stream) {
+    return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
+                                      Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
+                                      PhiBar, MassFluxBar, stream);
+}
 
 const char *cupcfd_hip_version(void) { return "cupcfd_hip 0.1 gfx950"; }
 

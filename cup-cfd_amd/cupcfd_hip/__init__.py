@@ -120,6 +120,8 @@ HOST_SYMBOLS = [
 
 # batched resident entry points (cupcfd_dev_<name>_<f64|f32>)
 BATCHED = ["gradient_batched", "flux_scalar_batched"]
+# adjoint entry points (FP64 only)
+ADJOINT = ["cupcfd_dev_scalar_adjoint_f64"]
 
 
 def all_symbols():
@@ -131,6 +133,7 @@ def all_symbols():
             out.append(f"cupcfd_dev_{k}_{s}")
         for k in BATCHED:
             out.append(f"cupcfd_dev_{k}_{s}")
+    out += ADJOINT
     return out
 
 
@@ -162,6 +165,8 @@ def _declare(L):
                 getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
         for k in BATCHED:
             getattr(L, f"cupcfd_dev_{k}_{s}").restype = ct.c_int
+    for k in ADJOINT:
+        getattr(L, k).restype = ct.c_int
 
 
 def check(fn, code):

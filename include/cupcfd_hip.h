@@ -337,6 +337,24 @@ int cupcfd_dev_flux_scalar_batched_##S(cupcfd_hip_mesh *mesh, int32_t K,        
 CUPCFD_BATCH_API(double, f64)
 CUPCFD_BATCH_API(float, f32)
 
+/* ---------------------------------------------------------------------------
+ * Adjoint (resident mode). No counterpart in the reference: the reverse mode
+ * of GradientPhiGaussDolfyn (GradientKernels.h:48-54, nGradient = 1) followed
+ * by FluxScalarDolfynFaceLoop (ScalarKernels.h:47-67), with respect to
+ * (PhiCell, MassFlux). Given the cotangents SuBar[C] on Su and RFaceBar[2F]
+ * on RFace, writes PhiBar[C] and MassFluxBar[F] (overwritten, internal
+ * numbering). The scalar loop's parameters have the same meaning as there;
+ * the derivative of min/max(mf, 0) at mf = 0 is taken as 0. Meshes whose
+ * regions are all DEFAULT (the reference StructGen's) only: other meshes
+ * return E_NOT_IMPLEMENTED. Sums are taken per cell over its faces in
+ * ascending face id, so results are deterministic.
+ * ------------------------------------------------------------------------- */
+int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
+    const double *MassFlux, const double *SuBar, const double *RFaceBar, int32_t SolveTurb,
+    double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
+    double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
+    double *PhiBar, double *MassFluxBar, void *stream);
+
 /* Library version string and the gfx target it was built for. */
 const char *cupcfd_hip_version(void);
 

@@ -6,6 +6,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 #include "oracle_fvm.h"
 
@@ -17,6 +18,7 @@ void or_set_carry_init(double v) { or_carry_init = v; }
 #define S f64
 #define SQRTW sqrt
 #include "oracle_kernels.inc"
+#include "oracle_adjoint.inc"
 #undef T
 #undef S
 #undef SQRTW

@@ -21,6 +21,7 @@ extern "C" {
 
 #define OR_E_SUCCESS 95
 #define OR_E_INVALID_INDEX 68
+#define OR_E_NOT_IMPLEMENTED 109
 
 #define OR_RTYPE_DEFAULT 0
 #define OR_RTYPE_INLET 1
@@ -110,6 +111,23 @@ OR_DECL(double, f64)
 OR_DECL(float, f32)
 
 #undef OR_DECL
+
+/* Parameters of the scalar face loop that its tangent and adjoint read. */
+typedef struct {
+    int SolveTurb, ivar, VarT, VarTE, VarED;
+    double vislam, Sigma_T, Prandtl, Sigma_k, Sigma_e, Sigma_s, Schmidt, GammaBlend;
+} or_scalar_params;
+
+/* Tangent (sudot, rfacedot) and adjoint (phibar, mfbar) of GradientPhiGaussDolfyn
+ * (nGradient = 1) followed by FluxScalarDolfynFaceLoop, w.r.t. (phi, MassFlux),
+ * on all-DEFAULT meshes (oracle_adjoint.inc). Outputs are overwritten.
+ * Returns OR_E_NOT_IMPLEMENTED on a mesh with a non-DEFAULT region. */
+int or_scalar_tangent_f64(or_mesh_f64 *m, const double *phi, const double *VisEff, const double *MassFlux,
+                          const double *phidot, const double *mfdot, const or_scalar_params *p, double *sudot,
+                          double *rfacedot);
+int or_scalar_adjoint_f64(or_mesh_f64 *m, const double *phi, const double *VisEff, const double *MassFlux,
+                          const double *subar, const double *rbar, const or_scalar_params *p, double *phibar,
+                          double *mfbar);
 
 #ifdef __cplusplus
 }

@@ -225,3 +225,37 @@ def calculateViscosityDolfynBoundaryLoop(m, tmcmu, large, small, kappa, visLam, 
 
 def calculateViscosityDolfynCellLoop2(m, visLam, visEffCell, visEffBoundary):
     return _fn(m, "viscosity_cell_loop2")(m.ref, _T(m, visLam), *_A(visEffCell), *_A(visEffBoundary))
+
+
+class ScalarParams(ct.Structure):
+    """or_scalar_params: the scalar face loop's parameters its tangent and adjoint read."""
+    _fields_ = [("SolveTurb", ct.c_int), ("ivar", ct.c_int), ("VarT", ct.c_int), ("VarTE", ct.c_int),
+                ("VarED", ct.c_int), ("vislam", ct.c_double), ("Sigma_T", ct.c_double), ("Prandtl", ct.c_double),
+                ("Sigma_k", ct.c_double), ("Sigma_e", ct.c_double), ("Sigma_s", ct.c_double),
+                ("Schmidt", ct.c_double), ("GammaBlend", ct.c_double)]
+
+    @classmethod
+    def of(cls, p):
+        return cls(*[p[n] for n, _ in cls._fields_])
+
+
+def scalar_tangent(m, phi, visEff, massFlux, phidot, mfdot, params):
+    """(sudot, rfacedot): tangent of gradient (nGradient = 1) + scalar face loop w.r.t. (phi, massFlux)."""
+    assert m.precision == 64
+    sd = np.zeros(m.nCells)
+    rd = np.zeros(2 * m.nFaces)
+    P = ScalarParams.of(params)
+    rc = _fn(m, "scalar_tangent")(m.ref, _P(phi), _P(visEff), _P(massFlux), _P(phidot), _P(mfdot), ct.byref(P),
+                                  _P(sd), _P(rd))
+    return rc, sd, rd
+
+
+def scalar_adjoint(m, phi, visEff, massFlux, subar, rbar, params):
+    """(phibar, mfbar): adjoint of gradient (nGradient = 1) + scalar face loop w.r.t. (phi, massFlux)."""
+    assert m.precision == 64
+    pb = np.zeros(m.nCells)
+    mb = np.zeros(m.nFaces)
+    P = ScalarParams.of(params)
+    rc = _fn(m, "scalar_adjoint")(m.ref, _P(phi), _P(visEff), _P(massFlux), _P(subar), _P(rbar), ct.byref(P),
+                                  _P(pb), _P(mb))
+    return rc, pb, mb

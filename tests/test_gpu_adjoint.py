@@ -1,0 +1,191 @@
+"""GPU adjoint of gradient (nGradient = 1) + scalar face loop w.r.t. (phi, MassFlux).
+
+* cupcfd_dev_scalar_adjoint_f64 equals the oracle's adjoint bit for bit
+  (both sum each cell's faces in ascending face id), on the reference test
+  mesh and on relabelled meshes, across gather schedules;
+* central differences of the GPU forward chain along 256 seeded unit
+  directions at 64^3 relabelled (faces with |mf| < 1e-5 left out of the
+  MassFlux directions) agree with the adjoint's directional derivatives;
+* the dot-product identity holds between the GPU adjoint and the FP64 CPU
+  tangent;
+* torch.autograd through ScalarFlux gives the adjoint;
+* 50 explicit pseudo-time steps reversed with checkpoints every 10 give the
+  same bits as the reverse sweep that keeps every state, and match
+  central differences of the marched state;
+* typed meshes are refused with E_NOT_IMPLEMENTED.
+"""
+import numpy as np
+import pytest
+
+import cupcfd_hip as ch
+import pyoracle as po
+from cupcfd_hip import adjoint as A
+from cupcfd_hip import workload as W
+from cupcfd_hip.fvm import resident as R
+
+torch = pytest.importorskip("torch")
+P = dict(W.SCALAR)
+
+
+class Case:
+    def __init__(self, **mesh):
+        self.host = ch.HostMesh(**mesh)
+        self.dev = ch.DeviceMesh(self.host, precision=64, reorder=True)
+        self.co, self.fo = self.dev.cell_order, self.dev.face_order
+        self.f = W.Fields(self.host, seed=31)
+        self.C, self.F = self.host.nCells, self.host.nFaces
+
+    def cells(self, a):
+        return torch.from_numpy(np.ascontiguousarray(a[self.co])).cuda()
+
+    def faces(self, a, k=1):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(self.F, k)[self.fo].ravel())).cuda()
+
+    def to_mesh_cells(self, t):
+        out = np.empty(self.C)
+        out[self.co] = t.cpu().numpy()
+        return out
+
+    def to_mesh_faces(self, t):
+        out = np.empty(self.F)
+        out[self.fo] = t.cpu().numpy()
+        return out
+
+    def chain(self):
+        f = self.f
+        return A.Chain(self.dev, torch.from_numpy(f.phiB).cuda(), self.cells(f.visEff), P, den=self.cells(f.den),
+                       te=self.cells(f.te), cpB=torch.from_numpy(f.cpB).cuda(),
+                       visEffB=torch.from_numpy(f.visEffB).cuda())
+
+
+_cases = {}
+
+
+def case(name):
+    spec = {"ref5": dict(nx=5, bounds=(-1, 1, -1, 1, -1, 1)),
+            "perm12": dict(nx=12, ny=11, nz=10, seed=8, permute_cells=True, permute_faces=True),
+            "perm64": dict(nx=64, seed=12345, permute_cells=True, permute_faces=True)}
+    if name not in _cases:
+        _cases[name] = Case(**spec[name])
+    return _cases[name]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", [0, 1, 8])
+@pytest.mark.parametrize("name", ["ref5", "perm12"])
+def test_adjoint_matches_oracle_bitwise(name, schedule):
+    c = case(name)
+    c.dev.set_schedule(schedule)
+    rng = np.random.default_rng(4)
+    sb = rng.normal(size=c.C)
+    rb = rng.normal(size=2 * c.F)
+    m = po.OracleMesh(c.host, 64)
+    rc, pb_ref, mb_ref = po.scalar_adjoint(m, c.f.phi, c.f.visEff, c.f.massFlux, sb, rb, P)
+    assert rc == 95
+    pb = torch.empty(c.C, dtype=torch.float64, device="cuda")
+    mb = torch.empty(c.F, dtype=torch.float64, device="cuda")
+    R.scalar_adjoint(c.dev, c.cells(c.f.phi), c.cells(c.f.visEff), c.faces(c.f.massFlux), c.cells(sb),
+                     c.faces(rb, 2), P, pb, mb)
+    torch.cuda.synchronize()
+    c.dev.set_schedule(0)
+    assert np.array_equal(c.to_mesh_cells(pb).view(np.uint64), pb_ref.view(np.uint64))
+    assert np.array_equal(c.to_mesh_faces(mb).view(np.uint64), mb_ref.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_typed_mesh_refused():
+    host = ch.HostMesh(4, typed=True)
+    dev = ch.DeviceMesh(host, precision=64)
+    z = lambda n: torch.zeros(n, dtype=torch.float64, device="cuda")  # noqa: E731
+    with pytest.raises(ch.CupCfdError) as e:
+        R.scalar_adjoint(dev, z(host.nCells), z(host.nCells), z(host.nFaces), z(host.nCells), z(2 * host.nFaces), P,
+                         z(host.nCells), z(host.nFaces))
+    assert "109" in str(e.value)
+
+
+def _unit_direction(rng, c):
+    pd = rng.normal(size=c.C)
+    md = rng.normal(size=c.F)
+    md[np.abs(c.f.massFlux) < 1e-5] = 0.0
+    md[c.host.a["faceBnd"] > -1] = 0.0
+    n = np.sqrt(pd @ pd + md @ md)
+    return pd / n, md / n
+
+
+@pytest.mark.gpu
+def test_central_differences_256_directions_64cubed():
+    c = case("perm64")
+    ch_ = c.chain()
+    rng = np.random.default_rng(2026)
+    phi, mf = c.cells(c.f.phi), c.faces(c.f.massFlux)
+    sb = torch.from_numpy(rng.normal(size=c.C)).cuda()
+    rb = torch.from_numpy(rng.normal(size=2 * c.F)).cuda()
+    pb, mb = ch_.adjoint(phi, mf, sb, rb)
+    eps = 1e-6
+    worst = 0.0
+    for _ in range(256):
+        pd, md = _unit_direction(rng, c)
+        pd_t, md_t = c.cells(pd), c.faces(md)
+        sp, rp = ch_.forward(phi + eps * pd_t, mf + eps * md_t)
+        sm, rm = ch_.forward(phi - eps * pd_t, mf - eps * md_t)
+        fd = (torch.dot(sb, sp - sm) + torch.dot(rb, rp - rm)).item() / (2 * eps)
+        ad = (torch.dot(pb, pd_t) + torch.dot(mb, md_t)).item()
+        worst = max(worst, abs(fd - ad) / max(1.0, abs(ad)))
+    assert worst < 1e-6, worst
+
+
+@pytest.mark.gpu
+def test_dot_product_against_cpu_tangent():
+    c = case("perm12")
+    rng = np.random.default_rng(7)
+    m = po.OracleMesh(c.host, 64)
+    ch_ = c.chain()
+    phi, mf = c.cells(c.f.phi), c.faces(c.f.massFlux)
+    for _ in range(4):
+        pd, md = _unit_direction(rng, c)
+        sb = rng.normal(size=c.C)
+        rb = rng.normal(size=2 * c.F)
+        _, sd, rd = po.scalar_tangent(m, c.f.phi, c.f.visEff, c.f.massFlux, pd, md, P)
+        pb, mb = ch_.adjoint(phi, mf, c.cells(sb), c.faces(rb, 2))
+        lhs = sb @ sd + rb @ rd
+        rhs = c.to_mesh_cells(pb) @ pd + c.to_mesh_faces(mb) @ md
+        assert abs(lhs - rhs) <= 1e-12 * (np.abs(sb) @ np.abs(sd) + np.abs(rb) @ np.abs(rd))
+
+
+@pytest.mark.gpu
+def test_torch_autograd():
+    c = case("perm12")
+    ch_ = c.chain()
+    rng = np.random.default_rng(9)
+    phi = c.cells(c.f.phi).requires_grad_(True)
+    mf = c.faces(c.f.massFlux).requires_grad_(True)
+    sb = torch.from_numpy(rng.normal(size=c.C)).cuda()
+    rb = torch.from_numpy(rng.normal(size=2 * c.F)).cuda()
+    su, rface = A.scalar_flux(phi, mf, ch_)
+    loss = (su * sb).sum() + (rface * rb).sum()
+    loss.backward()
+    pb, mb = ch_.adjoint(phi.detach(), mf.detach(), sb, rb)
+    assert torch.equal(phi.grad, pb) and torch.equal(mf.grad, mb)
+
+
+@pytest.mark.gpu
+def test_march_checkpointed_reverse():
+    c = case("perm12")
+    ch_ = c.chain()
+    rng = np.random.default_rng(11)
+    vol = c.cells(c.host.a["cellVolume"])
+    phi0, mf = c.cells(c.f.phi), c.faces(c.f.massFlux)
+    dtau = 2e-6
+    wT = torch.from_numpy(rng.normal(size=c.C)).cuda()
+    pb10, mb10 = A.march_adjoint(ch_, phi0, mf, 50, dtau, vol, wT, every=10)
+    pball, mball = A.march_adjoint(ch_, phi0, mf, 50, dtau, vol, wT, every=50)
+    assert torch.equal(pb10, pball) and torch.equal(mb10, mball)
+    eps = 1e-6
+    for _ in range(4):
+        pd, md = _unit_direction(rng, c)
+        pd_t, md_t = c.cells(pd), c.faces(md)
+        jp = torch.dot(wT, A.march(ch_, phi0 + eps * pd_t, mf + eps * md_t, 50, dtau, vol)).item()
+        jm = torch.dot(wT, A.march(ch_, phi0 - eps * pd_t, mf - eps * md_t, 50, dtau, vol)).item()
+        fd = (jp - jm) / (2 * eps)
+        ad = (torch.dot(pb10, pd_t) + torch.dot(mb10, md_t)).item()
+        assert abs(fd - ad) <= 1e-6 * max(1.0, abs(ad)), (fd, ad)
