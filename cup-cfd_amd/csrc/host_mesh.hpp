@@ -51,6 +51,16 @@ struct SplitMix64 {
     uint64_t below(uint64_t n) { return next() % n; }
 };
 
+// typed regions, one per side of the box: x-low INLET, x-high OUTLET, y-low
+// standard WALL, y-high moving non-standard WALL, z-low / z-high SYMP
+enum { REG_INLET = 0, REG_OUTLET, REG_WALL_LO, REG_WALL_HI, REG_SYMP_LO, REG_SYMP_HI };
+// one DEFAULT region, or the six typed regions
+void add_regions(cupcfd_host_mesh &m, bool typed);
+// size every per-cell, per-face and per-boundary array (zero / -1 filled)
+void resize_mesh(cupcfd_host_mesh &m, int C, int F, int B);
+// seeded non-zero xpac / xnac and rlencos = area / |Cc2 - Cc1|
+void fill_face_extras(cupcfd_host_mesh &m, uint64_t seed, double h);
+
 int build_structgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m);
 int build_tetgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m);
 void relabel(cupcfd_host_mesh &m, uint64_t seed, bool cells, bool faces);

@@ -27,7 +27,7 @@ struct Grid {
     double hi(double mn, double d, int k) const { return mn + (k + 1) * d; }
 };
 
-enum { REG_INLET = 0, REG_OUTLET, REG_WALL_LO, REG_WALL_HI, REG_SYMP_LO, REG_SYMP_HI };
+}  // namespace
 
 void add_regions(cupcfd_host_mesh &m, bool typed) {
     auto push = [&](int type, bool std_, bool flux, bool adiab, double ylog, double elog, double density,
@@ -112,8 +112,6 @@ void fill_face_extras(cupcfd_host_mesh &m, uint64_t seed, double h) {
         m.faceRLencos[f] = m.faceArea[f] / std::sqrt(d2);
     }
 }
-
-}  // namespace
 
 int build_structgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m) {
     if (p.nx < 1 || p.ny < 1 || p.nz < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;

@@ -177,11 +177,15 @@ def check(fn, code):
 # ---------------------------------------------------------------------------
 # Host mesh
 # ---------------------------------------------------------------------------
-def _np(ptr, n, dtype, shape=None):
+def _np(ptr, n, dtype, shape=None, owner=None):
+    """numpy view of n elements at ptr; the view keeps ``owner`` (the object whose memory it is) alive."""
     if n == 0:
         return np.zeros(shape if shape else (0,), dtype=dtype)
     ctype = np.ctypeslib.as_ctypes_type(np.dtype(dtype))
-    a = np.ctypeslib.as_array(ct.cast(ptr, ct.POINTER(ctype)), shape=(n,))
+    addr = ptr if isinstance(ptr, int) else ct.cast(ptr, ct.c_void_p).value
+    buf = (ctype * n).from_address(addr)
+    buf._owner = owner
+    a = np.ndarray((n,), dtype=dtype, buffer=buf)
     return a.reshape(shape) if shape else a
 
 
@@ -209,27 +213,27 @@ class HostMesh:
         C, F, B, R = self.nCells, self.nFaces, self.nBoundaries, self.nRegions
         i32, f64, u8 = np.int32, np.float64, np.uint8
         self.a = {
-            "faceCell1": _np(v.faceCell1ID, F, i32), "faceCell2": _np(v.faceCell2ID, F, i32),
-            "faceBnd": _np(v.faceBoundaryID, F, i32), "faceLambda": _np(v.faceLambda, F, f64),
-            "faceRLencos": _np(v.faceRLencos, F, f64), "faceArea": _np(v.faceArea, F, f64),
-            "faceNorm": _np(v.faceNorm.base, 3 * F, f64, (F, 3)),
-            "faceCenter": _np(v.faceCenter.base, 3 * F, f64, (F, 3)),
-            "faceXpac": _np(v.faceXpac.base, 3 * F, f64, (F, 3)),
-            "faceXnac": _np(v.faceXnac.base, 3 * F, f64, (F, 3)),
-            "cellCenter": _np(v.cellCenter.base, 3 * C, f64, (C, 3)), "cellVolume": _np(v.cellVolume, C, f64),
-            "bndFace": _np(v.boundaryFaceID, B, i32), "bndRegion": _np(v.boundaryRegionID, B, i32),
-            "bndDistance": _np(v.boundaryDistance, B, f64), "bndYPlus": _np(v.boundaryYPlus, B, f64),
-            "bndUPlus": _np(v.boundaryUPlus, B, f64), "bndQ": _np(v.boundaryQ, B, f64),
-            "bndH": _np(v.boundaryH, B, f64), "bndT": _np(v.boundaryT, B, f64),
-            "bndShear": _np(v.boundaryShear.base, 3 * B, f64, (B, 3)),
-            "regType": _np(v.regionType, R, i32), "regStd": _np(v.regionStd, R, u8),
-            "regFlux": _np(v.regionFlux, R, u8), "regAdiab": _np(v.regionAdiab, R, u8),
-            "regYLog": _np(v.regionYLog, R, f64), "regELog": _np(v.regionELog, R, f64),
-            "regDensity": _np(v.regionDensity, R, f64), "regTurbKE": _np(v.regionTurbKE, R, f64),
-            "regTurbDiss": _np(v.regionTurbDiss, R, f64), "regSplvl": _np(v.regionSplvl, R, f64),
-            "regDen": _np(v.regionDen, R, f64), "regR": _np(v.regionR, R, f64), "regT": _np(v.regionT, R, f64),
-            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f64, (R, 3)),
-            "regUVW": _np(v.regionUVW.base, 3 * R, f64, (R, 3)),
+            "faceCell1": _np(v.faceCell1ID, F, i32, owner=self), "faceCell2": _np(v.faceCell2ID, F, i32, owner=self),
+            "faceBnd": _np(v.faceBoundaryID, F, i32, owner=self), "faceLambda": _np(v.faceLambda, F, f64, owner=self),
+            "faceRLencos": _np(v.faceRLencos, F, f64, owner=self), "faceArea": _np(v.faceArea, F, f64, owner=self),
+            "faceNorm": _np(v.faceNorm.base, 3 * F, f64, (F, 3), owner=self),
+            "faceCenter": _np(v.faceCenter.base, 3 * F, f64, (F, 3), owner=self),
+            "faceXpac": _np(v.faceXpac.base, 3 * F, f64, (F, 3), owner=self),
+            "faceXnac": _np(v.faceXnac.base, 3 * F, f64, (F, 3), owner=self),
+            "cellCenter": _np(v.cellCenter.base, 3 * C, f64, (C, 3), owner=self), "cellVolume": _np(v.cellVolume, C, f64, owner=self),
+            "bndFace": _np(v.boundaryFaceID, B, i32, owner=self), "bndRegion": _np(v.boundaryRegionID, B, i32, owner=self),
+            "bndDistance": _np(v.boundaryDistance, B, f64, owner=self), "bndYPlus": _np(v.boundaryYPlus, B, f64, owner=self),
+            "bndUPlus": _np(v.boundaryUPlus, B, f64, owner=self), "bndQ": _np(v.boundaryQ, B, f64, owner=self),
+            "bndH": _np(v.boundaryH, B, f64, owner=self), "bndT": _np(v.boundaryT, B, f64, owner=self),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f64, (B, 3), owner=self),
+            "regType": _np(v.regionType, R, i32, owner=self), "regStd": _np(v.regionStd, R, u8, owner=self),
+            "regFlux": _np(v.regionFlux, R, u8, owner=self), "regAdiab": _np(v.regionAdiab, R, u8, owner=self),
+            "regYLog": _np(v.regionYLog, R, f64, owner=self), "regELog": _np(v.regionELog, R, f64, owner=self),
+            "regDensity": _np(v.regionDensity, R, f64, owner=self), "regTurbKE": _np(v.regionTurbKE, R, f64, owner=self),
+            "regTurbDiss": _np(v.regionTurbDiss, R, f64, owner=self), "regSplvl": _np(v.regionSplvl, R, f64, owner=self),
+            "regDen": _np(v.regionDen, R, f64, owner=self), "regR": _np(v.regionR, R, f64, owner=self), "regT": _np(v.regionT, R, f64, owner=self),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f64, (R, 3), owner=self),
+            "regUVW": _np(v.regionUVW.base, 3 * R, f64, (R, 3), owner=self),
         }
 
     STATE = ("bndYPlus", "bndUPlus", "bndQ", "bndH", "bndT", "bndShear", "regYLog", "regForceTangent")
@@ -248,12 +252,12 @@ class HostMesh:
         B, R = self.nBoundaries, self.nRegions
         f32 = np.float32
         return {
-            "bndYPlus": _np(v.boundaryYPlus, B, f32).copy(), "bndUPlus": _np(v.boundaryUPlus, B, f32).copy(),
-            "bndQ": _np(v.boundaryQ, B, f32).copy(), "bndH": _np(v.boundaryH, B, f32).copy(),
-            "bndT": _np(v.boundaryT, B, f32).copy(),
-            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3)).copy(),
-            "regYLog": _np(v.regionYLog, R, f32).copy(),
-            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3)).copy(),
+            "bndYPlus": _np(v.boundaryYPlus, B, f32, owner=self).copy(), "bndUPlus": _np(v.boundaryUPlus, B, f32, owner=self).copy(),
+            "bndQ": _np(v.boundaryQ, B, f32, owner=self).copy(), "bndH": _np(v.boundaryH, B, f32, owner=self).copy(),
+            "bndT": _np(v.boundaryT, B, f32, owner=self).copy(),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3), owner=self).copy(),
+            "regYLog": _np(v.regionYLog, R, f32, owner=self).copy(),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3), owner=self).copy(),
         }
 
     def set_state(self, st, precision=64):
@@ -265,10 +269,10 @@ class HostMesh:
         B, R = self.nBoundaries, self.nRegions
         f32 = np.float32
         tgt = {
-            "bndYPlus": _np(v.boundaryYPlus, B, f32), "bndUPlus": _np(v.boundaryUPlus, B, f32),
-            "bndQ": _np(v.boundaryQ, B, f32), "bndH": _np(v.boundaryH, B, f32), "bndT": _np(v.boundaryT, B, f32),
-            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3)), "regYLog": _np(v.regionYLog, R, f32),
-            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3)),
+            "bndYPlus": _np(v.boundaryYPlus, B, f32, owner=self), "bndUPlus": _np(v.boundaryUPlus, B, f32, owner=self),
+            "bndQ": _np(v.boundaryQ, B, f32, owner=self), "bndH": _np(v.boundaryH, B, f32, owner=self), "bndT": _np(v.boundaryT, B, f32, owner=self),
+            "bndShear": _np(v.boundaryShear.base, 3 * B, f32, (B, 3), owner=self), "regYLog": _np(v.regionYLog, R, f32, owner=self),
+            "regForceTangent": _np(v.regionForceTangent.base, 3 * R, f32, (R, 3), owner=self),
         }
         for k, val in st.items():
             tgt[k][...] = val

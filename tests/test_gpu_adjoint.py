@@ -64,7 +64,8 @@ _cases = {}
 def case(name):
     spec = {"ref5": dict(nx=5, bounds=(-1, 1, -1, 1, -1, 1)),
             "perm12": dict(nx=12, ny=11, nz=10, seed=8, permute_cells=True, permute_faces=True),
-            "perm64": dict(nx=64, seed=12345, permute_cells=True, permute_faces=True)}
+            "perm64": dict(nx=64, seed=12345, permute_cells=True, permute_faces=True),
+            "tets": dict(nx=5, ny=4, nz=4, tets=True, jitter=0.2, seed=6, permute_cells=True, permute_faces=True)}
     if name not in _cases:
         _cases[name] = Case(**spec[name])
     return _cases[name]
@@ -72,7 +73,7 @@ def case(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("schedule", [0, 1, 8])
-@pytest.mark.parametrize("name", ["ref5", "perm12"])
+@pytest.mark.parametrize("name", ["ref5", "perm12", "tets"])
 def test_adjoint_matches_oracle_bitwise(name, schedule):
     c = case(name)
     c.dev.set_schedule(schedule)

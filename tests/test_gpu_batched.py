@@ -24,6 +24,8 @@ MESHES = {
     "typed_perm": dict(nx=7, ny=6, nz=5, typed=True, permute_cells=True, permute_faces=True, face_extras=True,
                        seed=11),
     "typed32": dict(nx=32, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=3),
+    "tets_typed": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, typed=True, permute_cells=True,
+                       permute_faces=True, face_extras=True, seed=5),
 }
 _cache = {}
 

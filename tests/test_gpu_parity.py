@@ -32,6 +32,9 @@ MESHES = {
                        seed=11),
     # 32^3 over [0,1]^3 (the CPU benchmark mesh), typed, relabelled
     "typed32": dict(nx=32, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=3),
+    # 4x3x3 hexes split into 6 jittered tetrahedra each (216 cells, 4-face rows), typed, relabelled
+    "tets_typed": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, typed=True, permute_cells=True,
+                       permute_faces=True, face_extras=True, seed=5),
 }
 # kernels whose FP64 results must be bit-identical to the oracle
 BITEXACT = {"gradient", "gradient3", "mass", "mass_bl1", "mass_bl2", "mass_bl3", "mass_bl4", "mass_bl5",
@@ -148,7 +151,7 @@ def test_size_errors_are_reported():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32"])
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32", "tets_typed"])
 def test_schedules_bit_identical(meshname):
     """One thread per cell, 6 or 8 lanes per cell and 64-cell tiles give the same bits."""
     host, _ = _setup(meshname, 64)
