@@ -16,7 +16,17 @@ cpu_baseline: the sequential C restatement of the same kernels
 (oracle/liboracle_fvm.so, -O2, no FMA, one core) timed on this host over a
 bounded sample of the same workload.
 
+Further fields of the same line (each skippable):
+  batched  K = 64 scalars through gradient + scalar flux on a relabelled 96^3
+           mesh: one batched call of each against K single-scalar calls;
+  adjoint  gradient + scalar flux forward against forward + adjoint w.r.t.
+           (phi, massFlux) on a relabelled 64^3 mesh;
+  tets     the tetrahedral stress mesh (100^3 hexes x 6 jittered tets,
+           relabelled): the plain sweep, batched K = 16 and forward + adjoint.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+       [--no-cpu] [--no-batched] [--no-adjoint] [--no-tets]
+       [--batched-only] [--tets-only] [--schedule S]
 """
 import argparse
 import json
@@ -121,16 +131,105 @@ def batched_fields(host, K, seed):
     return phi, phiB
 
 
-def bench_batched(torch, n, K, steps, warmup):
-    """K scalars through gradient + scalar flux on a relabelled StructGen n^3 mesh (FP64): one batched call of
-    each against K single-scalar calls on K separate arrays. The scalars share the mass flux of one
-    FluxMassDolfynFaceLoop call. Returns per-scalar cell throughputs and their ratio."""
-    import cupcfd_hip as ch
+def make_sweep(torch, host, dev, f):
+    """The four face-loop kernels of one sweep on resident copies of ``f`` (FP64, internal numbering), as
+    (name, callable) pairs launching on the current stream."""
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
-    host = ch.HostMesh(n, seed=4242, permute_cells=True, permute_faces=True)
-    dev = ch.DeviceMesh(host, 64, reorder=True)
+    C, F = host.nCells, host.nFaces
+    co, fo = dev.cell_order, dev.face_order
+    cuda = torch.device("cuda:0")
+
+    def cells(a):
+        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda)
+
+    def faces(a, k=1):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(F, k)[fo].ravel())).to(cuda)
+
+    def bnd(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+
+    d = dict(phi=cells(f.phi), phiB=bnd(f.phiB), u=cells(f.u), v=cells(f.v), w=cells(f.w), den=cells(f.den),
+             p=cells(f.p), ar=cells(f.ar), visEff=cells(f.visEff), visEffB=bnd(f.visEffB),
+             dudx=cells(f.dudx), dvdx=cells(f.dvdx), dwdx=cells(f.dwdx), dpdx=cells(f.dpdx), dphidx=cells(f.dphidx),
+             mf=faces(f.massFlux), te=cells(f.te), ed=cells(f.ed), t=cells(f.t), teB=bnd(f.teB), edB=bnd(f.edB),
+             tB=bnd(f.tB), denB=bnd(f.denB), cpB=bnd(f.cpB), uB=bnd(f.uB), vB=bnd(f.vB), wB=bnd(f.wB),
+             su=cells(f.su), sv=cells(f.sv), sw=cells(f.sw), au=cells(f.au), av=cells(f.av), aw=cells(f.aw))
+    d["g"] = torch.empty(C * 3, dtype=torch.float64, device=cuda)
+    d["go"] = torch.empty_like(d["g"])
+    d["rface"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
+    d["rface2"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
+    d["rface3"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
+    d["mf_out"] = d["mf"].clone()
+    d["Su"] = d["su"].clone()
+    d["Au"] = d["au"].clone()
+    d["phiB2"] = d["phiB"].clone()
+    stream = torch.cuda.current_stream()
+    mass_flags = (W.MASS["solveTurbEnergy"], W.MASS["solveTurbDiss"], W.MASS["solveVisc"], W.MASS["solveEnthalpy"])
+
+    def k_gradient():
+        R.gradient(dev, 1, d["phi"], d["phiB"], d["g"], d["go"], stream)
+
+    def k_mass():
+        R.mass(dev, d["dudx"], d["dvdx"], d["dwdx"], d["dpdx"], d["den"], d["denB"], d["u"], d["v"], d["w"],
+               d["mf_out"], d["p"], d["ar"], d["su"], d["rface"], flags=mass_flags, teC=d["te"], teB=d["teB"],
+               edC=d["ed"], edB=d["edB"], visC=d["visEff"], visB=d["visEffB"], tC=d["t"], tB=d["tB"], stream=stream)
+
+    def k_scalar():
+        R.scalar(dev, d["phi"], d["phiB2"], d["visEff"], d["Au"], d["Su"], d["den"], d["mf"], d["te"], d["cpB"],
+                 d["visEffB"], d["rface2"], d["dphidx"], W.SCALAR, stream)
+
+    def k_uvw():
+        R.uvw(dev, W.UVW["gammaBlend"], W.UVW["small"], d["u"], d["v"], d["w"], d["uB"], d["vB"], d["wB"],
+              d["visEff"], d["visEffB"], d["mf"], d["dudx"], d["dvdx"], d["dwdx"], d["rface3"], d["su"], d["sv"],
+              d["sw"], d["au"], d["av"], d["aw"], stream)
+
+    kernels = [("gradient", k_gradient), ("mass", k_mass), ("scalar", k_scalar), ("uvw", k_uvw)]
+    return kernels
+
+
+def time_sweep(torch, kernels, steps, warmup):
+    """Mean seconds per kernel and per sweep over ``steps`` timed sweeps (HIP events on the launch stream)."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        for _, k in kernels:
+            k()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(kernels) + 1)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    for s in range(steps):
+        ev[s][0].record(stream)
+        for j, (_, k) in enumerate(kernels):
+            k()
+            ev[s][j + 1].record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    per = {name: float(np.mean([ev[s][j].elapsed_time(ev[s][j + 1]) for s in range(steps)])) / 1e3
+           for j, (name, _) in enumerate(kernels)}
+    step_s = float(np.mean([ev[s][0].elapsed_time(ev[s][-1]) for s in range(steps)])) / 1e3
+    return per, step_s, t_wall
+
+
+def kernel_report(C, F, Fi, B, per):
+    """Per-kernel time, Gcell-sweeps/s and algorithmic-byte bandwidth against the HBM peak."""
+    ab = algorithmic_bytes(C, F, Fi, B, 8)
+    out = {}
+    for name, t in per.items():
+        gbs = ab[name] / t / 1e9
+        out[name] = {"ms": t * 1e3, "gcell_sweeps_per_s": C / t / 1e9, "algorithmic_bytes": ab[name],
+                     "algorithmic_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
+def bench_batched(torch, host, dev, K, steps, warmup, label):
+    """K scalars through gradient + scalar flux on ``host`` (FP64): one batched call of each against K
+    single-scalar calls on K separate arrays. The scalars share the mass flux of one FluxMassDolfynFaceLoop
+    call. Returns per-scalar cell throughputs and their ratio."""
+    from cupcfd_hip import workload as W
+    from cupcfd_hip.fvm import resident as R
+
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
     f = W.Fields(host, seed=99)
     co, fo = dev.cell_order, dev.face_order
@@ -197,7 +296,7 @@ def bench_batched(torch, n, K, steps, warmup):
     tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
                                          sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
     return {
-        "workload": f"gradient + scalar flux of K={K} scalars, StructGen {n}^3 relabelled, FP64, shared mass flux",
+        "workload": f"gradient + scalar flux of K={K} scalars, {label}, FP64, shared mass flux",
         "cells": C, "K": K,
         "batched_ms": tb * 1e3, "plain_ms": tp * 1e3,
         "batched_gcell_scalar_per_s": C * K / tb / 1e9, "plain_gcell_scalar_per_s": C * K / tp / 1e9,
@@ -206,15 +305,12 @@ def bench_batched(torch, n, K, steps, warmup):
     }
 
 
-def bench_adjoint(torch, n, steps, warmup):
-    """Forward (gradient nGradient=1 + scalar flux) against forward + adjoint w.r.t. (phi, massFlux) on a
-    relabelled StructGen n^3 mesh in FP64, seeded N(0,1) cotangents on Su and RFace."""
-    import cupcfd_hip as ch
+def bench_adjoint(torch, host, dev, steps, warmup, label):
+    """Forward (gradient nGradient=1 + scalar flux) against forward + adjoint w.r.t. (phi, massFlux) on
+    ``host`` (all regions DEFAULT) in FP64, seeded N(0,1) cotangents on Su and RFace."""
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
-    host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
-    dev = ch.DeviceMesh(host, 64, reorder=True)
     C, F = host.nCells, host.nFaces
     f = W.Fields(host, seed=5)
     co, fo = dev.cell_order, dev.face_order
@@ -263,10 +359,44 @@ def bench_adjoint(torch, n, steps, warmup):
     tf = timed(fwd)
     tfa = timed(lambda: (fwd(), adj()))
     return {
-        "workload": f"gradient (nGradient=1) + scalar flux, forward and reverse w.r.t. (phi, massFlux), StructGen "
-                    f"{n}^3 relabelled, FP64, one DEFAULT region",
+        "workload": f"gradient (nGradient=1) + scalar flux, forward and reverse w.r.t. (phi, massFlux), {label}, "
+                    f"FP64, one DEFAULT region",
         "cells": C, "forward_ms": tf * 1e3, "forward_adjoint_ms": tfa * 1e3, "ratio": tfa / tf,
     }
+
+
+def structgen(n, seed):
+    import cupcfd_hip as ch
+    host = ch.HostMesh(n, seed=seed, permute_cells=True, permute_faces=True)
+    return host, ch.DeviceMesh(host, 64, reorder=True), f"StructGen {n}^3 relabelled"
+
+
+def bench_tets(torch, n, K, steps, warmup):
+    """The tetrahedral stress mesh (n^3 hexes x 6 jittered tets, ids relabelled, one DEFAULT region), FP64:
+    the plain sweep, the batched K-scalar comparison and forward + adjoint."""
+    import cupcfd_hip as ch
+    from cupcfd_hip import workload as W
+
+    t0 = time.time()
+    host = ch.HostMesh(n, seed=606, tets=True, jitter=0.2, permute_cells=True, permute_faces=True)
+    dev = ch.DeviceMesh(host, 64, reorder=True)
+    t_mesh = time.time() - t0
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    f = W.Fields(host, seed=808)
+    kernels = make_sweep(torch, host, dev, f)
+    per, step_s, _ = time_sweep(torch, kernels, steps, warmup)
+    del kernels
+    torch.cuda.empty_cache()
+    label = f"{n}^3 hexes x 6 jittered tets relabelled"
+    out = {"workload": f"plain FP64 gradient+mass+scalar+UVW sweep, {label} ({C} cells, {F} faces), one DEFAULT "
+                       f"region", "cells": C, "faces": F, "boundaries": B, "mesh_setup_s": t_mesh,
+           "gcell_sweeps_per_s": C / step_s / 1e9, "ms_per_step": step_s * 1e3,
+           "kernels": kernel_report(C, F, F - B, B, per)}
+    out["batched"] = bench_batched(torch, host, dev, K, steps=max(2, min(steps, 5)), warmup=1, label=label)
+    torch.cuda.empty_cache()
+    out["adjoint"] = bench_adjoint(torch, host, dev, steps=max(3, min(steps, 10)), warmup=1, label=label)
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -280,6 +410,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched K-scalar comparison")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched K-scalar comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint comparison")
+    ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral stress mesh")
+    ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
+    ap.add_argument("--tets-n", type=int, default=100)
+    ap.add_argument("--tets-k", type=int, default=16)
     ap.add_argument("--adjoint-n", type=int, default=64)
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-k", type=int, default=64)
@@ -298,8 +432,12 @@ def main():
     from cupcfd_hip.fvm import resident as R
 
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(torch, args.batch_n, args.batch_k, steps=args.steps,
-                                                   warmup=args.warmup)}))
+        hb, db, lb = structgen(args.batch_n, 4242)
+        print(json.dumps({"batched": bench_batched(torch, hb, db, args.batch_k, steps=args.steps,
+                                                   warmup=args.warmup, label=lb)}))
+        return
+    if args.tets_only:
+        print(json.dumps({"tets": bench_tets(torch, args.tets_n, args.tets_k, args.steps, args.warmup)}))
         return
     n = args.n
     t0 = time.time()
@@ -311,80 +449,10 @@ def main():
     Fi = int((host.a["faceBnd"] < 0).sum())
     f = W.Fields(host, seed=777)
 
-    co, fo = dev.cell_order, dev.face_order
-    cuda = torch.device("cuda:0")
-
-    def cells(a):
-        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda)
-
-    def faces(a, k=1):
-        return torch.from_numpy(np.ascontiguousarray(a.reshape(F, k)[fo].ravel())).to(cuda)
-
-    def bnd(a):
-        return torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
-
-    d = dict(phi=cells(f.phi), phiB=bnd(f.phiB), u=cells(f.u), v=cells(f.v), w=cells(f.w), den=cells(f.den),
-             p=cells(f.p), ar=cells(f.ar), visEff=cells(f.visEff), visEffB=bnd(f.visEffB),
-             dudx=cells(f.dudx), dvdx=cells(f.dvdx), dwdx=cells(f.dwdx), dpdx=cells(f.dpdx), dphidx=cells(f.dphidx),
-             mf=faces(f.massFlux), te=cells(f.te), ed=cells(f.ed), t=cells(f.t), teB=bnd(f.teB), edB=bnd(f.edB),
-             tB=bnd(f.tB), denB=bnd(f.denB), cpB=bnd(f.cpB), uB=bnd(f.uB), vB=bnd(f.vB), wB=bnd(f.wB),
-             su=cells(f.su), sv=cells(f.sv), sw=cells(f.sw), au=cells(f.au), av=cells(f.av), aw=cells(f.aw))
-    d["g"] = torch.empty(C * 3, dtype=torch.float64, device=cuda)
-    d["go"] = torch.empty_like(d["g"])
-    d["rface"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
-    d["rface2"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
-    d["rface3"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
-    d["mf_out"] = d["mf"].clone()
-    d["Su"] = d["su"].clone()
-    d["Au"] = d["au"].clone()
-    d["phiB2"] = d["phiB"].clone()
-    stream = torch.cuda.current_stream()
-    mass_flags = (W.MASS["solveTurbEnergy"], W.MASS["solveTurbDiss"], W.MASS["solveVisc"], W.MASS["solveEnthalpy"])
-
-    def k_gradient():
-        R.gradient(dev, 1, d["phi"], d["phiB"], d["g"], d["go"], stream)
-
-    def k_mass():
-        R.mass(dev, d["dudx"], d["dvdx"], d["dwdx"], d["dpdx"], d["den"], d["denB"], d["u"], d["v"], d["w"],
-               d["mf_out"], d["p"], d["ar"], d["su"], d["rface"], flags=mass_flags, teC=d["te"], teB=d["teB"],
-               edC=d["ed"], edB=d["edB"], visC=d["visEff"], visB=d["visEffB"], tC=d["t"], tB=d["tB"], stream=stream)
-
-    def k_scalar():
-        R.scalar(dev, d["phi"], d["phiB2"], d["visEff"], d["Au"], d["Su"], d["den"], d["mf"], d["te"], d["cpB"],
-                 d["visEffB"], d["rface2"], d["dphidx"], W.SCALAR, stream)
-
-    def k_uvw():
-        R.uvw(dev, W.UVW["gammaBlend"], W.UVW["small"], d["u"], d["v"], d["w"], d["uB"], d["vB"], d["wB"],
-              d["visEff"], d["visEffB"], d["mf"], d["dudx"], d["dvdx"], d["dwdx"], d["rface3"], d["su"], d["sv"],
-              d["sw"], d["au"], d["av"], d["aw"], stream)
-
-    kernels = [("gradient", k_gradient), ("mass", k_mass), ("scalar", k_scalar), ("uvw", k_uvw)]
-    for _ in range(args.warmup):
-        for _, k in kernels:
-            k()
-    torch.cuda.synchronize()
-
+    kernels = make_sweep(torch, host, dev, f)
+    per, step_s, t_wall = time_sweep(torch, kernels, args.steps, args.warmup)
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(kernels) + 1)] for _ in range(K)]
-    torch.cuda.synchronize()
-    t_wall0 = time.perf_counter()
-    for s in range(K):
-        ev[s][0].record(stream)
-        for j, (_, k) in enumerate(kernels):
-            k()
-            ev[s][j + 1].record(stream)
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall0
-
-    per = {name: float(np.mean([ev[s][j].elapsed_time(ev[s][j + 1]) for s in range(K)])) / 1e3
-           for j, (name, _) in enumerate(kernels)}
-    step_s = float(np.mean([ev[s][0].elapsed_time(ev[s][-1]) for s in range(K)])) / 1e3
-    ab = algorithmic_bytes(C, F, Fi, B, 8)
-    kern = {}
-    for name, t in per.items():
-        gbs = ab[name] / t / 1e9
-        kern[name] = {"ms": t * 1e3, "gcell_sweeps_per_s": C / t / 1e9, "algorithmic_bytes": ab[name],
-                      "algorithmic_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    kern = kernel_report(C, F, Fi, B, per)
     dom = max(per, key=per.get)
     copy_gbs = stream_copy_gbs(torch)
 
@@ -420,10 +488,17 @@ def main():
         "mesh_setup_s": t_mesh,
     }
     if not args.no_batched:
-        out["batched"] = bench_batched(torch, args.batch_n, args.batch_k, steps=max(2, min(args.steps, 10)),
-                                       warmup=1)
+        hb, db, lb = structgen(args.batch_n, 4242)
+        out["batched"] = bench_batched(torch, hb, db, args.batch_k, steps=max(2, min(args.steps, 10)), warmup=1,
+                                       label=lb)
+        del hb, db
     if not args.no_adjoint:
-        out["adjoint"] = bench_adjoint(torch, args.adjoint_n, steps=max(5, min(args.steps, 20)), warmup=2)
+        ha, da, la = structgen(args.adjoint_n, 12345)
+        out["adjoint"] = bench_adjoint(torch, ha, da, steps=max(5, min(args.steps, 20)), warmup=2, label=la)
+        del ha, da
+    if not args.no_tets:
+        torch.cuda.empty_cache()
+        out["tets"] = bench_tets(torch, args.tets_n, args.tets_k, steps=max(3, min(args.steps, 10)), warmup=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, f, args.cpu_budget)
         out["cpu_baseline"]["cores"] = 1
