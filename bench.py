@@ -293,6 +293,14 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
     same = all(torch.equal(g_b[:, :, k], one[k]["g"]) and torch.equal(su_b[:, k], one[k]["su"])
                for k in (0, K // 2, K - 1))
     tbg = timed(lambda: R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream))
+    # FP32 storage, FP64 arithmetic: time, and max-norm error against the FP64 gradient of the FP64 inputs
+    phi32, phiB32 = phi_b.float(), phiB_b.float()
+    g32 = torch.zeros((C, 3, K), dtype=torch.float32, device=cuda)
+    go32 = torch.zeros_like(g32)
+    tbg32 = timed(lambda: R.gradient_batched_f32s(dev, 1, K, phi32, phiB32, g32, go32, stream))
+    R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream)
+    torch.cuda.synchronize()
+    err32 = ((g32.double() - g_b).abs().max() / g_b.abs().max()).item()
     tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
                                          sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
     return {
@@ -302,6 +310,7 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
         "batched_gcell_scalar_per_s": C * K / tb / 1e9, "plain_gcell_scalar_per_s": C * K / tp / 1e9,
         "speedup": tp / tb, "bit_identical_sample": bool(same),
         "batched_gradient_ms": tbg * 1e3, "batched_scalar_ms": tbs * 1e3,
+        "batched_gradient_f32_storage_ms": tbg32 * 1e3, "f32_storage_gradient_rel_err": err32,
     }
 
 

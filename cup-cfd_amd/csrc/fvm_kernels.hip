@@ -1595,16 +1595,26 @@ template <int G> __device__ __forceinline__ int batch_uniform(int c) {
     return G == 64 ? __builtin_amdgcn_readfirstlane(c) : c;
 }
 
-template <class T, bool SWEEP0, int G, int RL>
-__global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, GradArgs<T> a, int K, int per, int nblk) {
+// Arguments of the batched gradient: per-scalar inputs stored as TI, outputs
+// as TO (float storage with double geometry and accumulation is T = double,
+// TI = TO = float); the previous sweep's gradient is always T.
+template <class T, class TI, class TO> struct BatchGradArgs {
+    const TI *phi, *phiB;
+    const T *grado;
+    TO *out1, *out2;
+};
+
+template <class T, class TI, class TO, bool SWEEP0, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, BatchGradArgs<T, TI, TO> a, int K, int per,
+                                                             int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     const int c = batch_uniform<G>(L.c);
-    const T *phi = a.phi + L.k, *phiB = a.phiB + L.k;
+    const TI *phi = a.phi + L.k, *phiB = a.phiB + L.k;
     const T *go = SWEEP0 ? nullptr : a.grado + L.k;
     auto cellv = [&](int x) -> GradCell<T> {
         GradCell<T> r;
-        r.phi = phi[(size_t)x * K];
+        r.phi = (T)phi[(size_t)x * K];
         if (!SWEEP0) {
             r.go = ld3s(go, x, K);
             r.cc = xyz(m.cgeo[x]);
@@ -1630,7 +1640,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, GradA
     auto other = [&](int2 en) -> GradCell<T> {
         if (en.y >= 0) return cellv(en.y);
         GradCell<T> r{};
-        r.phi = phiB[(size_t)(~en.y) * K];
+        r.phi = (T)phiB[(size_t)(~en.y) * K];
         return r;
     };
     int e0, n;
@@ -1663,8 +1673,8 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, GradA
     for (int j = 0; j < 3; j++) {
         const T r = acc[j] * fact;
         const size_t o = ((size_t)c * 3 + j) * K + L.k;
-        a.out1[o] = r;
-        if (a.out2) a.out2[o] = r;
+        a.out1[o] = (TO)r;
+        if (a.out2) a.out2[o] = (TO)r;
     }
 }
 
@@ -2164,8 +2174,38 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
 inline int batch_group(int K) { return K <= 4 ? 4 : (K <= 16 ? 16 : 64); }
 inline Grid batch_grid(int C, int G) { return xcd_grid(C, kBlock / G); }
 
-template <class T>
-int dev_gradient_batched(cupcfd_hip_mesh *h, int nGradient, int K, const T *phi, const T *phiB, T *g, T *go,
+template <class T, class TI, class TO>
+void launch_gradient_batched(const DevMesh<T> &d, const BatchGradArgs<T, TI, TO> &ga, bool s0, int K, int G,
+                             const Grid &gr, hipStream_t s) {
+    const dim3 grid(gr.blocks, (K + G - 1) / G);
+    const int rl = d.rowlen;
+#define CUPCFD_GB(GG, RR)                                                                                        \
+    if (s0)                                                                                                      \
+        hipLaunchKernelGGL((k_gradient_batched<T, TI, TO, true, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per,  \
+                           gr.nblk);                                                                             \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_gradient_batched<T, TI, TO, false, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per, \
+                           gr.nblk);
+    // the preloaded rows pay off when the row's connectivity is wave-uniform
+    // (G = 64); narrower groups walk the row entry by entry
+    if (G == 4) {
+        CUPCFD_GB(4, 0)
+    } else if (G == 16) {
+        CUPCFD_GB(16, 0)
+    } else if (rl == 6) {
+        CUPCFD_GB(64, 6)
+    } else if (rl == 4) {
+        CUPCFD_GB(64, 4)
+    } else {
+        CUPCFD_GB(64, 0)
+    }
+#undef CUPCFD_GB
+}
+
+// Batched gradient sweeps; the sweeps before the last keep their gradient in
+// T scratch, the last writes dPhidx and dPhidxo as TO
+template <class T, class TI, class TO>
+int dev_gradient_batched(cupcfd_hip_mesh *h, int nGradient, int K, const TI *phi, const TI *phiB, TO *g, TO *go,
                          void *stream) {
     Mesh<T> *M = get<T>(h);
     if (!M) return CUPCFD_E_NULL_PTR;
@@ -2173,54 +2213,31 @@ int dev_gradient_batched(cupcfd_hip_mesh *h, int nGradient, int K, const T *phi,
     const DevMesh<T> &d = M->d;
     hipStream_t s = (hipStream_t)stream;
     if (d.C == 0) return CUPCFD_E_SUCCESS;
-    const size_t vbytes = 3 * (size_t)d.C * K * sizeof(T);
-    if (nGradient <= 0) return hipMemsetAsync(go, 0, vbytes, s) == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+    const size_t nv = 3 * (size_t)d.C * K;
+    if (nGradient <= 0)
+        return hipMemsetAsync(go, 0, nv * sizeof(TO), s) == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
     T *t[2] = {nullptr, nullptr};
     if (nGradient > 1) {
-        t[0] = (T *)M->ws_get(44, vbytes);
-        t[1] = (T *)M->ws_get(45, vbytes);
+        t[0] = (T *)M->ws_get(44, nv * sizeof(T));
+        t[1] = (T *)M->ws_get(45, nv * sizeof(T));
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
     const int G = batch_group(K);
-    const int rl = d.rowlen;
     const Grid gr = batch_grid(d.C, G);
-    const dim3 grid(gr.blocks, (K + G - 1) / G);
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
-        const bool last = (it == nGradient - 1);
-        T *dst = last ? g : t[it & 1];
-        GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
         const bool s0 = (it == 0);
-#define CUPCFD_GB(GG, RR)                                                                                     \
-    if (s0)                                                                                                   \
-        hipLaunchKernelGGL((k_gradient_batched<T, true, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per,       \
-                           gr.nblk);                                                                          \
-    else                                                                                                      \
-        hipLaunchKernelGGL((k_gradient_batched<T, false, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per, gr.nblk);
-#define CUPCFD_GBR(GG)                                                                                        \
-    if (rl == 6) {                                                                                            \
-        CUPCFD_GB(GG, 6)                                                                                      \
-    } else if (rl == 4) {                                                                                     \
-        CUPCFD_GB(GG, 4)                                                                                      \
-    } else {                                                                                                  \
-        CUPCFD_GB(GG, 0)                                                                                      \
-    }
-        // the preloaded rows pay off when the row's connectivity is wave-uniform
-        // (G = 64); narrower groups walk the row entry by entry
-        if (G == 4) {
-            CUPCFD_GB(4, 0)
-        } else if (G == 16) {
-            CUPCFD_GB(16, 0)
+        if (it == nGradient - 1) {
+            launch_gradient_batched(d, BatchGradArgs<T, TI, TO>{phi, phiB, prev, g, go}, s0, K, G, gr, s);
         } else {
-            CUPCFD_GBR(64)
+            launch_gradient_batched(d, BatchGradArgs<T, TI, T>{phi, phiB, prev, t[it & 1], nullptr}, s0, K, G, gr, s);
+            prev = t[it & 1];
         }
-#undef CUPCFD_GBR
-#undef CUPCFD_GB
         if (hipGetLastError() != hipSuccess) return CUPCFD_E_ERROR;
-        prev = dst;
     }
     return CUPCFD_E_SUCCESS;
 }
+
 
 template <class T>
 int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den,
@@ -2544,7 +2561,8 @@ DEV_API(float, f32)
 #define BATCH_API(T, S)                                                                                            \
     int cupcfd_dev_gradient_batched_##S(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K, const T *phiCell,      \
                                         const T *phiBoundary, T *dPhidxCell, T *dPhidxoCell, void *stream) {       \
-        return dev_gradient_batched<T>(mesh, nGradient, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream); \
+        return dev_gradient_batched<T, T, T>(mesh, nGradient, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell,    \
+                                             stream);                                                          \
     }                                                                                                              \
     int cupcfd_dev_flux_scalar_batched_##S(                                                                        \
         cupcfd_hip_mesh *mesh, int32_t K, const T *PhiCell, T *PhiBoundary, const T *VisEff, T *Au, T *Su,         \
@@ -2562,6 +2580,12 @@ BATCH_API(double, f64)
 BATCH_API(float, f32)
 
 #undef BATCH_API
+
+int cupcfd_dev_gradient_batched_f32s(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K, const float *phiCell,
+                                     const float *phiBoundary, float *dPhidxCell, float *dPhidxoCell, void *stream) {
+    return dev_gradient_batched<double, float, float>(mesh, nGradient, K, phiCell, phiBoundary, dPhidxCell,
+                                                      dPhidxoCell, stream);
+}
 
 int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
                                   const double *MassFlux, const double *SuBar, const double *RFaceBar,

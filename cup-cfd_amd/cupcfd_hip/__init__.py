@@ -122,6 +122,8 @@ HOST_SYMBOLS = [
 BATCHED = ["gradient_batched", "flux_scalar_batched"]
 # adjoint entry points (FP64 only)
 ADJOINT = ["cupcfd_dev_scalar_adjoint_f64"]
+# FP32-storage entry points on 64-bit meshes
+F32S = ["cupcfd_dev_gradient_batched_f32s"]
 
 
 def all_symbols():
@@ -133,7 +135,7 @@ def all_symbols():
             out.append(f"cupcfd_dev_{k}_{s}")
         for k in BATCHED:
             out.append(f"cupcfd_dev_{k}_{s}")
-    out += ADJOINT
+    out += ADJOINT + F32S
     return out
 
 
@@ -165,7 +167,7 @@ def _declare(L):
                 getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
         for k in BATCHED:
             getattr(L, f"cupcfd_dev_{k}_{s}").restype = ct.c_int
-    for k in ADJOINT:
+    for k in ADJOINT + F32S:
         getattr(L, k).restype = ct.c_int
 
 

@@ -264,3 +264,12 @@ class resident:
             _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
             _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
             t(mfbar), cls._s(stream)))
+
+    @classmethod
+    def gradient_batched_f32s(cls, mesh, nGradient, K, phi, phiB, g, go, stream=None):
+        """Batched gradient with float32 storage (phi, phiB, g, go float32 tensors) on a 64-bit mesh: double
+        geometry and arithmetic, the final gradient rounded to float."""
+        from . import lib, check
+        assert mesh.precision == 64
+        check("cupcfd_dev_gradient_batched_f32s", lib().cupcfd_dev_gradient_batched_f32s(
+            mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB), cls._t(g), cls._t(go), cls._s(stream)))

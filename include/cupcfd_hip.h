@@ -337,6 +337,16 @@ int cupcfd_dev_flux_scalar_batched_##S(cupcfd_hip_mesh *mesh, int32_t K,        
 CUPCFD_BATCH_API(double, f64)
 CUPCFD_BATCH_API(float, f32)
 
+/* Batched gradient with FP32 storage and FP64 arithmetic, on a 64-bit mesh
+ * (no counterpart in the reference). phiCell, phiBoundary, dPhidxCell and
+ * dPhidxoCell are float, laid out as cupcfd_dev_gradient_batched_f64's; the
+ * geometry, every product and sum, and the intermediate gradients of
+ * nGradient > 1 are double, and only the final gradient is rounded to float.
+ * The result is therefore cupcfd_dev_gradient_batched_f64's on the inputs
+ * widened to double, rounded to float (round to nearest even). */
+int cupcfd_dev_gradient_batched_f32s(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K,
+    const float *phiCell, const float *phiBoundary, float *dPhidxCell, float *dPhidxoCell, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint (resident mode). No counterpart in the reference: the reverse mode
  * of GradientPhiGaussDolfyn (GradientKernels.h:48-54, nGradient = 1) followed

@@ -138,3 +138,25 @@ def test_batched_rejects_bad_k():
     z = torch.zeros(8, dtype=torch.float64, device="cuda")
     with pytest.raises(ch.CupCfdError):
         R.gradient_batched(dev, 1, 0, z, z, z, z)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "tets_typed"])
+@pytest.mark.parametrize("K", [3, 16, 64])
+@pytest.mark.parametrize("nG", [1, 3])
+def test_gradient_batched_f32_storage_is_rounded_f64(K, meshname, nG):
+    """FP32 storage, FP64 arithmetic: the float result is the FP64 result on the widened inputs, rounded."""
+    host, dev = _mesh(meshname, 64)
+    C, B = host.nCells, host.nBoundaries
+    rng = np.random.default_rng(300 + K)
+    phi = torch.from_numpy(rng.uniform(-1, 1, (C, K)).astype(np.float32)).cuda()
+    phiB = torch.from_numpy(rng.uniform(-1, 1, (B, K)).astype(np.float32)).cuda()
+    g32 = torch.zeros((C, 3, K), dtype=torch.float32, device="cuda")
+    go32 = torch.zeros_like(g32)
+    R.gradient_batched_f32s(dev, nG, K, phi, phiB, g32, go32)
+    g64 = torch.zeros((C, 3, K), dtype=torch.float64, device="cuda")
+    go64 = torch.zeros_like(g64)
+    R.gradient_batched(dev, nG, K, phi.double(), phiB.double(), g64, go64)
+    torch.cuda.synchronize()
+    assert _same(g32, g64.to(torch.float32))
+    assert _same(go32, go64.to(torch.float32))
