@@ -11,9 +11,10 @@
 // vptr as EuclideanVector does. Other benchmark types are reported and
 // skipped. Optional keys under "BenchmarkKernels" that the reference ignores:
 // "Seed" (input stream), "Permute" (relabel cell/face ids), "Typed" (six
-// typed boundary regions). Each kernel call is timed on the host around the
-// (synchronous) call, as TreeTimer times the reference's; the report is JSON
-// on stdout.
+// typed boundary regions), "Graph" (also run the 16 kernels on device-resident
+// inputs, eagerly and as one captured hipGraph, see GraphBench). Each kernel
+// call is timed on the host around the (synchronous) call, as TreeTimer times
+// the reference's; the report is JSON on stdout.
 //
 // Usage: cupcfd_bench [path/to/cupcfd.json]   (default ./cupcfd.json)
 #include <chrono>
@@ -28,6 +29,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime.h>
 
 #include "cupcfd_hip.h"
 #include "cupcfd_hip_fvm.hpp"
@@ -366,6 +369,188 @@ template <class T> struct Bench {
     }
 };
 
+
+// ----------------------------------------------------------------------------
+// GraphBench: the same 16 kernels, with BenchmarkKernels' parameter values, on
+// device-resident inputs through the resident C ABI (cupcfd_dev_*), first
+// launched eagerly on a stream and then captured once into a hipGraph and
+// replayed. One eager repetition before the capture makes every lazily
+// allocated workspace exist, so the captured calls allocate nothing. Before
+// timing, one eager repetition and one graph launch from the same device
+// state are compared byte for byte.
+// ----------------------------------------------------------------------------
+template <class T> struct DevFns;
+#define CUPCFD_DEVFNS(T, S)                                                       \
+    template <> struct DevFns<T> {                                                \
+        static constexpr auto gradient = cupcfd_dev_gradient_phi_gauss_dolfyn_##S; \
+        static constexpr auto mass = cupcfd_dev_flux_mass_face_loop_##S;          \
+        static constexpr auto bl1 = cupcfd_dev_flux_mass_boundary_loop1_##S;      \
+        static constexpr auto bl2 = cupcfd_dev_flux_mass_boundary_loop2_##S;      \
+        static constexpr auto bl3 = cupcfd_dev_flux_mass_boundary_loop3_##S;      \
+        static constexpr auto bl4 = cupcfd_dev_flux_mass_boundary_loop4_##S;      \
+        static constexpr auto bl5 = cupcfd_dev_flux_mass_boundary_loop5_##S;      \
+        static constexpr auto mreg = cupcfd_dev_flux_mass_region_loop_##S;        \
+        static constexpr auto scalar = cupcfd_dev_flux_scalar_face_loop_##S;      \
+        static constexpr auto uvw = cupcfd_dev_flux_uvw_face_loop1_##S;           \
+        static constexpr auto ureg = cupcfd_dev_flux_uvw_region_loop1_##S;        \
+        static constexpr auto ubnd = cupcfd_dev_flux_uvw_bnds_loop1_##S;          \
+        static constexpr auto vc1 = cupcfd_dev_viscosity_cell_loop1_##S;          \
+        static constexpr auto vreg = cupcfd_dev_viscosity_region_loop_##S;        \
+        static constexpr auto vbnd = cupcfd_dev_viscosity_boundary_loop_##S;      \
+        static constexpr auto vc2 = cupcfd_dev_viscosity_cell_loop2_##S;          \
+    };
+CUPCFD_DEVFNS(double, f64)
+CUPCFD_DEVFNS(float, f32)
+#undef CUPCFD_DEVFNS
+
+template <class T> struct GraphBench {
+    cupcfd_hip_mesh *dev;
+    int C, F, B, R;
+    hipStream_t s = nullptr;
+    std::mt19937_64 rng;
+    struct Buf {
+        T *d;
+        size_t n;
+    };
+    std::vector<Buf> bufs;
+    int icinl = 0, icout = 0, icsym = 0, icwal = 0;
+    int failures = 0;
+    GraphBench(cupcfd_hip_mesh *d, int C_, int F_, int B_, int R_, uint64_t seed)
+        : dev(d), C(C_), F(F_), B(B_), R(R_), rng(seed) {}
+    ~GraphBench() {
+        for (auto &b : bufs) (void)hipFree(b.d);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    // a device array of n values drawn from U(1e-6, 1e-2), as the reference's wrappers fill theirs
+    T *U(size_t n) {
+        std::uniform_real_distribution<double> dist(1e-6, 1e-2);
+        std::vector<T> h(n ? n : 1);
+        for (auto &x : h) x = (T)dist(rng);
+        T *d = nullptr;
+        if (hipMalloc(&d, h.size() * sizeof(T)) != hipSuccess) return nullptr;
+        if (hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        bufs.push_back({d, h.size()});
+        return d;
+    }
+    void ok(int rc, const char *what) {
+        if (rc != CUPCFD_E_SUCCESS) {
+            failures++;
+            std::fprintf(stderr, "graph mode: %s returned %d\n", what, rc);
+        }
+    }
+
+    T *phi, *phiB, *g, *go, *massFlux, *dudx, *dvdx, *dwdx, *dpdx, *den, *denB, *u, *v, *w, *p, *ar, *su, *rface,
+        *te, *teB, *ed, *edB, *vis, *visB, *t, *tB, *flows, *flowRegion, *flowFact, *uB, *vB, *wB, *denB2, *su5,
+        *sphi, *sphiB, *svis, *sau, *ssu, *sden, *ste, *scpB, *svisB, *srf, *sg, *uu, *uv, *uw, *uuB, *uvB, *uwB,
+        *uvis, *uvisB, *urf, *usu, *usv, *usw, *uau, *uav, *uaw, *udu, *udv, *udw, *vte, *ved, *vden, *vvis, *vvisB;
+
+    bool alloc() {
+        T **all[] = {&phi, &phiB, &g, &go, &massFlux, &dudx, &dvdx, &dwdx, &dpdx, &den, &denB, &u, &v, &w, &p,
+                     &ar, &su, &rface, &te, &teB, &ed, &edB, &vis, &visB, &t, &tB, &flows, &flowRegion, &flowFact,
+                     &uB, &vB, &wB, &denB2, &su5, &sphi, &sphiB, &svis, &sau, &ssu, &sden, &ste, &scpB, &svisB,
+                     &srf, &sg, &uu, &uv, &uw, &uuB, &uvB, &uwB, &uvis, &uvisB, &urf, &usu, &usv, &usw, &uau, &uav,
+                     &uaw, &udu, &udv, &udw, &vte, &ved, &vden, &vvis, &vvisB};
+        const size_t c = C, f = F, b = B, r = R;
+        const size_t n[] = {c, b, 3 * c, 3 * c, f, 3 * c, 3 * c, 3 * c, 3 * c, c, b, c, c, c, c,
+                            c, c, 2 * f, c, b, c, b, c, b, c, b, 4, r, r,
+                            b, b, b, b, c, c, b, c, c, c, c, c, b, b,
+                            2 * f, 3 * c, c, c, c, b, b, b, c, b, 2 * f, c, c, c, c, c,
+                            c, 3 * c, 3 * c, 3 * c, c, c, c, c, b};
+        static_assert(sizeof(all) / sizeof(all[0]) == sizeof(n) / sizeof(n[0]), "one size per buffer");
+        for (size_t i = 0; i < sizeof(n) / sizeof(n[0]); i++)
+            if (!(*all[i] = U(n[i]))) return false;
+        return hipStreamCreate(&s) == hipSuccess;
+    }
+
+    // BenchmarkKernels::runBenchmark's 16 calls (BenchmarkKernels.cpp:55-115), resident
+    void repetition() {
+        using D = DevFns<T>;
+        ok(D::gradient(dev, 1, phi, phiB, g, go, s), "gradient");
+        ok(D::mass(dev, dudx, dvdx, dwdx, dpdx, den, denB, u, v, w, massFlux, p, ar, su, rface, (T)1e-18, &icinl,
+                   &icout, &icsym, &icwal, 0, 0, 1, 0, te, teB, ed, edB, vis, visB, t, tB, s),
+           "mass");
+        ok(D::bl1(dev, massFlux, flows + 0, s), "mass_bl1");
+        ok(D::bl2(dev, massFlux, flowRegion, flows + 1, s), "mass_bl2");
+        ok(D::bl3(dev, (T)1.1, flows + 2, s), "mass_bl3");
+        ok(D::bl4(dev, massFlux, uB, vB, wB, denB2, (T)1.0, flows + 1, s), "mass_bl4");
+        ok(D::bl5(dev, massFlux, su5, uB, vB, wB, (T)1.1, 1, 1, 1, flowFact, flows + 3, s), "mass_bl5");
+        ok(D::mreg(dev, flowFact, flowRegion, (T)0.0, s), "mass_region");
+        ok(D::scalar(dev, sphi, sphiB, svis, sau, ssu, sden, massFlux, ste, scpB, svisB, srf, sg, 1, 0, (T)0.001, 1,
+                     2, (T)1.4, (T)1.2e-3, 3, (T)1.5, 4, (T)1.6, (T)1.7, (T)1.4e-3, (T)1.1, (T)1e-18, (T)1e18,
+                     (T)1.8, s),
+           "scalar");
+        ok(D::uvw(dev, (T)0.9, (T)1e-18, uu, uv, uw, uuB, uvB, uwB, uvis, uvisB, massFlux, udu, udv, udw, urf, usu,
+                  usv, usw, uau, uav, uaw, s),
+           "uvw");
+        ok(D::ureg(dev, s), "uvw_region1");
+        ok(D::ubnd(dev, s), "uvw_bnds1");
+        ok(D::vc1(dev, (T)1e-18, (T)1e-4, (T)0.232, (T)0.00547, vte, ved, vden, vvis, s), "visc_cell1");
+        ok(D::vreg((T)1.1, dev, s), "visc_region");
+        ok(D::vbnd(dev, (T)1.0, (T)1e18, (T)1e-18, (T)1.1, (T)1.2, vte, vden, vvis, vvisB, s), "visc_boundary");
+        ok(D::vc2(dev, (T)1.1, vvis, vvisB, s), "visc_cell2");
+    }
+
+    std::vector<std::vector<T>> snapshot() {
+        std::vector<std::vector<T>> out;
+        for (auto &b : bufs) {
+            out.emplace_back(b.n);
+            (void)hipMemcpy(out.back().data(), b.d, b.n * sizeof(T), hipMemcpyDeviceToHost);
+        }
+        return out;
+    }
+    void restore(const std::vector<std::vector<T>> &st) {
+        for (size_t i = 0; i < bufs.size(); i++)
+            (void)hipMemcpy(bufs[i].d, st[i].data(), bufs[i].n * sizeof(T), hipMemcpyHostToDevice);
+        (void)cupcfd_hip_mesh_push_state(dev);
+    }
+
+    // eager and graph repetitions per second; returns false on any failure
+    bool run(int reps, double &eager_ms, double &graph_ms, bool &same) {
+        if (!alloc()) return false;
+        (void)cupcfd_hip_mesh_push_state(dev);
+        repetition();  // workspaces come into existence here
+        if (hipStreamSynchronize(s) != hipSuccess || failures) return false;
+        (void)cupcfd_hip_mesh_push_state(dev);  // the eager and the graph repetition start from the same state
+        const auto st0 = snapshot();
+        repetition();
+        if (hipStreamSynchronize(s) != hipSuccess || failures) return false;
+        const auto eager = snapshot();
+        restore(st0);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        repetition();
+        if (hipStreamEndCapture(s, &graph) != hipSuccess || failures) return false;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) return false;
+        if (hipGraphLaunch(exec, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return false;
+        const auto replay = snapshot();
+        same = true;
+        for (size_t i = 0; i < eager.size(); i++)
+            if (std::memcmp(eager[i].data(), replay[i].data(), eager[i].size() * sizeof(T)) != 0) same = false;
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        float ms = 0;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; r++) repetition();
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        eager_ms = ms / reps;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; r++) (void)hipGraphLaunch(exec, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        graph_ms = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return failures == 0;
+    }
+};
+
 static double num(const Json *o, const char *k, double dflt) {
     const Json *v = o ? o->get(k) : nullptr;
     return v && v->kind == Json::NUM ? v->num : (v && v->kind == Json::BOOL ? (v->b ? 1.0 : 0.0) : dflt);
@@ -417,7 +602,24 @@ template <class T> int run_all(const Json &cfg, const cupcfd_structgen_params &b
                         kv.second.size(), s / kv.second.size());
             fk = false;
         }
-        std::printf("}}");
+        std::printf("}");
+        if (num(bk, "Graph", 0)) {
+            int grc = CUPCFD_E_SUCCESS;
+            cupcfd_hip_mesh *dev = cupcfd_hip::shim::device<T>(mesh, &grc);
+            double em = 0, gm = 0;
+            bool same = false;
+            bool good = dev != nullptr;
+            if (good) {
+                GraphBench<T> gb(dev, mesh.properties.lTCells, mesh.properties.lFaces, mesh.properties.lBoundaries,
+                                 mesh.properties.lRegions, p.seed ^ 0x6a5eedull);
+                good = gb.run(R, em, gm, same);
+            }
+            std::printf(", \"graph\": {\"ok\": %s, \"replay_matches_eager\": %s, \"eager_ms_per_repetition\": %.6f, "
+                        "\"graph_ms_per_repetition\": %.6f}",
+                        good ? "true" : "false", same ? "true" : "false", em, gm);
+            if (!good || !same) status = CUPCFD_E_ERROR;
+        }
+        std::printf("}");
         first = false;
         cupcfd_hip::shim::forget(&mesh);
         if (bench.failures) status = CUPCFD_E_ERROR;

@@ -39,3 +39,26 @@ def test_driver_runs_reference_schema(tmp_path, precision):
         assert sorted(b["kernels"]) == sorted(KERNELS)
         for k in KERNELS:
             assert b["kernels"][k]["calls"] == b["Repetitions"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_driver_graph_mode(tmp_path, precision):
+    """"Graph": the 16 kernels on resident inputs, captured once into a hipGraph; one replay equals one eager
+    repetition from the same device state, byte for byte, on a DEFAULT and on a typed relabelled mesh."""
+    cfg = json.load(open(CFG))
+    cfg["DataTypes"]["FloatDataType"] = precision
+    for b in cfg["Benchmarks"]:
+        if "BenchmarkKernels" in b:
+            b["BenchmarkKernels"]["Graph"] = 1
+            b["BenchmarkKernels"]["Repetitions"] = 3
+    p = tmp_path / "cupcfd.json"
+    p.write_text(json.dumps(cfg))
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout)
+    assert len(rep["benchmarks"]) == 2
+    for b in rep["benchmarks"]:
+        g = b["graph"]
+        assert g["ok"] and g["replay_matches_eager"], b
+        assert g["graph_ms_per_repetition"] > 0 and g["eager_ms_per_repetition"] > 0
