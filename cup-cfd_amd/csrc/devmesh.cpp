@@ -345,7 +345,7 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
 
     // ---- cell tiles ---------------------------------------------------------
     {
-        const int TC = 64;
+        const int TC = kTileCells;
         const int NT = (C + TC - 1) / TC;
         std::vector<int> istart(C + 1, 0), bstart(C + 1, 0);  // first owned interior / boundary face per cell
         for (int k = 0; k < Fi; k++) istart[fconn[k].x + 1]++;

@@ -31,6 +31,14 @@
 struct cupcfd_hip_mesh;
 
 namespace cupcfd_hip {
+// cell tiles of the LDS-staged gathers: kTileCells consecutive cells per tile,
+// one workgroup of kTileBlock = 4 * kTileCells threads per tile (phase A
+// stages one element of a cell field per thread)
+constexpr int kTileCells = 64;
+constexpr int kTileBlock = 4 * kTileCells;
+}  // namespace cupcfd_hip
+
+namespace cupcfd_hip {
 
 template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
 template <class T> struct alignas(2 * sizeof(T)) V2 { T x, y; };

@@ -278,8 +278,7 @@ __global__ __launch_bounds__(LPC * 32) void k_gradient_vec(DevMesh<T> m, GradArg
 //     terms of slot eslot[e] (negated for the cell2 side where the reference's
 //     cell2 update is the cell1 update with every term's sign flipped).
 // ---------------------------------------------------------------------------
-constexpr int kTileCells = 64;
-static_assert(kBlock >= 4 * kTileCells, "phase A gives each thread at most one element of a cell field");
+static_assert(kTileBlock >= 4 * kTileCells, "phase A gives each thread at most one element of a cell field");
 
 struct TileSlots {
     int4 ti;  // first owned interior face, count, first owned boundary face, count
@@ -342,12 +341,12 @@ template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> 
         e0 = m.row[c0];
         e1 = m.row[c0 + nc];
     }
-    for (int i = threadIdx.x; i < e1 - e0; i += kBlock) le[i] = m.eslot[e0 + i];
+    for (int i = threadIdx.x; i < e1 - e0; i += kTileBlock) le[i] = m.eslot[e0 + i];
     return e0;
 }
 
 template <class T, bool SWEEP0>
-__global__ __launch_bounds__(kBlock) void k_gradient_tile(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
+__global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, GradArgs<T> a, int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int TC = kTileCells, NF = SWEEP0 ? 1 : 7;  // phi (+ dPhidxo xyz, centre xyz)
     const int S = m.tslots;
@@ -403,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient_tile(DevMesh<T> m, GradArgs
         }
         return r;
     };
-    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
         if (sl != tid) load(sl);
         T tt[3];
         if (cn.y >= 0) gradient_interior<T, SWEEP0>(xf, s, cell(cn.x), cell(cn.y), tt);
@@ -920,7 +919,7 @@ __global__ __launch_bounds__(LPC * 32) void k_scalar_vec(DevMesh<T> m, ScalarArg
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
+__global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int TC = kTileCells, NF = 8;  // phi, VisEff, dPhidx xyz, centre xyz
     const int S = m.tslots;
@@ -972,7 +971,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs
         return ScalarCell<T>{lc[l], lc[TC + l], v3<T>{lc[2 * TC + l], lc[3 * TC + l], lc[4 * TC + l]},
                              v3<T>{lc[5 * TC + l], lc[6 * TC + l], lc[7 * TC + l]}};
     };
-    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
         if (sl != tid) load(sl);
         T tt[4];
         bool fl = false;
@@ -1310,7 +1309,7 @@ __global__ __launch_bounds__(LPC * 32) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+__global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int TC = kTileCells, NF = 16;  // u v w VisEff, dudx dvdx dwdx centre (xyz each)
     const int S = m.tslots;
@@ -1375,7 +1374,7 @@ __global__ __launch_bounds__(kBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T> a,
                           v3<T>{q[10 * TC], q[11 * TC], q[12 * TC]},
                           v3<T>{q[13 * TC], q[14 * TC], q[15 * TC]}};
     };
-    for (int sl = tid; sl < ts.nslot; sl += kBlock) {
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
         if (sl != tid) load(sl);
         T tt[10];
         bool fl = false;
@@ -1985,7 +1984,7 @@ template <class T> size_t tile_lds(const DevMesh<T> &d, int nf, int nt, bool fla
 // schedule if set (1 = one thread per cell), else the row length of uniform 4-
 // or 6-face meshes (every lane busy), else 8 if every row fits, else one
 // thread per cell. Returns kTile, 4, 6, 8, or 0 for one thread per cell.
-constexpr int kTile = kTileCells;
+constexpr int kTile = 64;  // the tile schedule's code in cupcfd_hip_mesh_set_schedule
 template <class T> int gather_schedule(const Mesh<T> &M, int nf, int nt, bool flags, size_t &lds) {
     int l = M.schedule_lpc;
     lds = 0;
@@ -2030,7 +2029,7 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
     size_t lds;
     const int lpc = gather_schedule(*M, 7, 3, false, lds);
     const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-    const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
+    const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
         const bool last = (it == nGradient - 1);
@@ -2159,7 +2158,7 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         size_t lds;
         const int lpc = gather_schedule(*M, 8, 4, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-        const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
+        const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
         if (lpc == kTile) hipLaunchKernelGGL(k_scalar_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
         else if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
@@ -2283,7 +2282,7 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
         size_t lds;
         const int lpc = gather_schedule(*M, 16, 10, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-        const int nthr = lpc && lpc != kTile ? lpc * kCellsPerBlock : kBlock;
+        const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
         if (lpc == kTile) hipLaunchKernelGGL(k_uvw_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
         else if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
