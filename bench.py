@@ -426,8 +426,9 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=64)
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-k", type=int, default=64)
-    ap.add_argument("--schedule", type=int, default=0,
-                    help="gather schedule: 0 automatic, 1 thread per cell, 4/6/8 lanes per cell, 64 cell tiles")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="gather schedule: -1 autotuned per kernel (default), 0 automatic, 1 thread per cell, "
+                         "4/6/8 lanes per cell, 64 cell tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -452,8 +453,12 @@ def main():
     t0 = time.time()
     host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
     dev = ch.DeviceMesh(host, 64, reorder=True)
-    dev.set_schedule(args.schedule)
     t_mesh = time.time() - t0
+    if args.schedule >= 0:
+        dev.set_schedule(args.schedule)
+        schedules = [args.schedule] * 3
+    else:
+        schedules = dev.autotune()
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
     Fi = int((host.a["faceBnd"] < 0).sum())
     f = W.Fields(host, seed=777)
@@ -488,7 +493,8 @@ def main():
         "data": "synthetic (seeded fields on a seeded relabelled StructGen mesh)",
         "config": {"workload": f"plain FP64 gradient+mass+scalar+UVW sweep, StructGen {n}^3 hex, cell and face ids "
                                f"permuted (seed 12345), one DEFAULT region", "cells": C, "faces": F,
-                   "boundaries": B, "parallelism": "single GPU"},
+                   "boundaries": B, "parallelism": "single GPU",
+                   "gather_schedules": dict(zip(("gradient", "scalar", "uvw"), schedules))},
         "kernels": kern,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["algorithmic_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kern[dom]["hbm_frac"], "traffic": traffic},

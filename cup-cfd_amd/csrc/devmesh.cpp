@@ -614,7 +614,11 @@ extern "C" int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes
     if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 6 &&
         lanes_per_cell != 8 && lanes_per_cell != 64)
         return CUPCFD_E_CONFIG_INVALID_VALUE;
-    if (mesh->precision == 64) mesh->m64->schedule_lpc = lanes_per_cell;
-    else mesh->m32->schedule_lpc = lanes_per_cell;
+    auto set = [&](auto *M) {
+        M->schedule_lpc = lanes_per_cell;
+        for (int &k : M->schedule_k) k = -1;  // an explicit schedule replaces an autotuned one
+    };
+    if (mesh->precision == 64) set(mesh->m64);
+    else set(mesh->m32);
     return CUPCFD_E_SUCCESS;
 }

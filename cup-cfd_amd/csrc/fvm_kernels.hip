@@ -1946,6 +1946,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(DevMesh<T> m, AdjArgs<T> a
     a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
 }
 
+// values in [0.5, 1.5) from a hash of the index: finite, non-zero inputs for timing runs
+template <class T> __global__ __launch_bounds__(kBlock) void k_fill_timing(T *x, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 16;
+    x[i] = (T)(0.5 + (double)(h & 0xffffu) / 65536.0);
+}
+
 }  // namespace cupcfd_hip
 
 // ===========================================================================
@@ -1978,15 +1987,17 @@ template <class T> size_t tile_lds(const DevMesh<T> &d, int nf, int nt, bool fla
     return b <= 65536 ? b : 0;
 }
 
-// Gather schedule of gradient, scalar and UVW. Automatic (mesh schedule 0)
+// Gather schedule of gradient, scalar and UVW: the kernel's autotuned schedule
+// if any, else the mesh's. Automatic (schedule 0)
 // and 64 take the cell-tile kernels when the mesh's tiles fit their slot
 // index and `lds` bytes of LDS (set here); otherwise lanes per cell: the mesh's
 // schedule if set (1 = one thread per cell), else the row length of uniform 4-
 // or 6-face meshes (every lane busy), else 8 if every row fits, else one
 // thread per cell. Returns kTile, 4, 6, 8, or 0 for one thread per cell.
 constexpr int kTile = 64;  // the tile schedule's code in cupcfd_hip_mesh_set_schedule
-template <class T> int gather_schedule(const Mesh<T> &M, int nf, int nt, bool flags, size_t &lds) {
-    int l = M.schedule_lpc;
+enum { kGatherGradient = 0, kGatherScalar = 1, kGatherUvw = 2 };
+template <class T> int gather_schedule(const Mesh<T> &M, int kind, int nf, int nt, bool flags, size_t &lds) {
+    int l = M.schedule_k[kind] >= 0 ? M.schedule_k[kind] : M.schedule_lpc;
     lds = 0;
     if (l == kTile || l == 0) {
         if (M.d.ntiles > 0) lds = tile_lds(M.d, nf, nt, flags);
@@ -2027,7 +2038,7 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
     size_t lds;
-    const int lpc = gather_schedule(*M, 7, 3, false, lds);
+    const int lpc = gather_schedule(*M, kGatherGradient, 7, 3, false, lds);
     const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
     const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
     const T *prev = nullptr;
@@ -2156,7 +2167,7 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
     if (rc != CUPCFD_E_SUCCESS) return rc;
     if (d.C) {
         size_t lds;
-        const int lpc = gather_schedule(*M, 8, 4, true, lds);
+        const int lpc = gather_schedule(*M, kGatherScalar, 8, 4, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
@@ -2280,7 +2291,7 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
                  gammaBlend, small};
     if (d.C) {
         size_t lds;
-        const int lpc = gather_schedule(*M, 16, 10, true, lds);
+        const int lpc = gather_schedule(*M, kGatherUvw, 16, 10, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
@@ -2347,6 +2358,114 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
     }
     if (d.F) hipLaunchKernelGGL(k_adj_faces<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a);
     return launch_ok();
+}
+
+// Per-kernel schedule choice by timing: each candidate schedule (cell tiles,
+// the lane-per-face width the mesh allows, one thread per cell) of the
+// gradient, scalar and UVW gathers runs on scratch inputs on a private stream
+// (one warm-up, then the best of three timed calls) and the fastest is kept.
+// The mesh's mutable boundary and region state is saved and restored around
+// the runs. Every schedule gives the same bits, so only speed is chosen.
+template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    const size_t C = d.C, F = d.F, B = d.B, R = d.R;
+    int cand[3];
+    int nc = 0;
+    cand[nc++] = kTile;
+    if (d.rowlen == 4 || d.rowlen == 6) cand[nc++] = d.rowlen;
+    else if (d.maxrow <= 8) cand[nc++] = d.maxrow <= 4 ? 4 : 8;
+    cand[nc++] = 1;
+    if (C == 0) {
+        for (int k = 0; k < 3; k++) {
+            M->schedule_k[k] = -1;
+            if (chosen) chosen[k] = 0;
+        }
+        return CUPCFD_E_SUCCESS;
+    }
+    // scratch: 24 cell arrays, 8 boundary arrays, 3 face arrays, and the saved state
+    const size_t nstate = 5 * B + 3 * B + R + 3 * R;
+    const size_t n = 24 * C + 8 * B + 3 * F + nstate;
+    T *pool = nullptr;
+    if (hipMalloc(&pool, (n ? n : 1) * sizeof(T)) != hipSuccess) return CUPCFD_E_ERROR;
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess) {
+        (void)hipFree(pool);
+        return CUPCFD_E_ERROR;
+    }
+    T *state = pool + 24 * C + 8 * B + 3 * F;
+    hipLaunchKernelGGL(k_fill_timing<T>, dim3((unsigned)((n - nstate + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                       pool, n - nstate);
+    T *sv[8] = {d.byplus, d.buplus, d.bq, d.bh, d.bt, d.bshear, d.rylog, d.rforcet};
+    const size_t sn[8] = {B, B, B, B, B, 3 * B, R, 3 * R};
+    {
+        T *q = state;
+        for (int k = 0; k < 8; k++) {
+            if (sn[k]) (void)hipMemcpyAsync(q, sv[k], sn[k] * sizeof(T), hipMemcpyDeviceToDevice, st);
+            q += sn[k];
+        }
+    }
+    T *c = pool, *b = pool + 24 * C, *f = pool + 24 * C + 8 * B;
+    auto cell = [&](int i) { return c + (size_t)i * C; };  // i < 24 (a 3-vector takes 3)
+    auto bnd = [&](int i) { return b + (size_t)i * B; };
+    T *mf = f, *rface = f + F;
+    // scalar loop parameters of BenchmarkKernels (BenchmarkKernels.cpp:677-695)
+    const T sp[] = {(T)0.001, (T)1.4, (T)1.2e-3, (T)1.5, (T)1.6, (T)1.7, (T)1.4e-3, (T)1.1, (T)1e-18, (T)1e18, (T)1.8};
+    auto run = [&](int kind) -> int {
+        if (kind == kGatherGradient)
+            return dev_gradient<T>(h, 1, cell(0), bnd(0), cell(1), cell(4), st);
+        if (kind == kGatherScalar)
+            return dev_scalar<T>(h, cell(0), bnd(1), cell(7), cell(8), cell(9), cell(10), mf, cell(11), bnd(2),
+                                 bnd(3), rface, cell(12), 1, 0, sp[0], 1, 2, sp[1], sp[2], 3, sp[3], 4, sp[4], sp[5],
+                                 sp[6], sp[7], sp[8], sp[9], sp[10], st);
+        return dev_uvw<T>(h, (T)0.9, (T)1e-18, cell(0), cell(7), cell(10), bnd(4), bnd(5), bnd(6), cell(11), bnd(7),
+                          mf, cell(1), cell(4), cell(12), rface, cell(15), cell(16), cell(17), cell(18), cell(19),
+                          cell(20), st);
+    };
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int rc = CUPCFD_E_SUCCESS;
+    for (int kind = 0; kind < 3 && rc == CUPCFD_E_SUCCESS; kind++) {
+        float best = 0;
+        int pick = cand[0];
+        for (int j = 0; j < nc && rc == CUPCFD_E_SUCCESS; j++) {
+            M->schedule_k[kind] = cand[j];
+            rc = run(kind);  // warm-up (and lazily built tables)
+            float tmin = 0;
+            for (int r = 0; r < 3 && rc == CUPCFD_E_SUCCESS; r++) {
+                (void)hipEventRecord(e0, st);
+                rc = run(kind);
+                (void)hipEventRecord(e1, st);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (r == 0 || ms < tmin) tmin = ms;
+            }
+            if (j == 0 || tmin < best) {
+                best = tmin;
+                pick = cand[j];
+            }
+        }
+        M->schedule_k[kind] = pick;
+        if (chosen) chosen[kind] = pick;
+    }
+    {
+        T *q = state;
+        for (int k = 0; k < 8; k++) {
+            if (sn[k]) (void)hipMemcpyAsync(sv[k], q, sn[k] * sizeof(T), hipMemcpyDeviceToDevice, st);
+            q += sn[k];
+        }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) rc = CUPCFD_E_ERROR;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(pool);
+    if (rc != CUPCFD_E_SUCCESS)
+        for (int &k : M->schedule_k) k = -1;
+    return rc;
 }
 
 int permute(cupcfd_hip_mesh *h, bool cells, const void *src, void *dst, int ncomp, int eb, void *stream, int dir) {
@@ -2595,6 +2714,11 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
                                       Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
                                       PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    return mesh->precision == 64 ? autotune<double>(mesh, schedules) : autotune<float>(mesh, schedules);
 }
 
 const char *cupcfd_hip_version(void) { return "cupcfd_hip 0.1 gfx950"; }

@@ -112,6 +112,7 @@ HOST_SYMBOLS = [
     "cupcfd_host_mesh_destroy", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
+    "cupcfd_hip_mesh_autotune",
     "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
@@ -153,6 +154,7 @@ def _declare(L):
     L.cupcfd_hip_mesh_face_order.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_set_carry_init.argtypes = [ct.c_void_p, ct.c_double]
     L.cupcfd_hip_mesh_set_schedule.argtypes = [ct.c_void_p, ct.c_int32]
+    L.cupcfd_hip_mesh_autotune.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_push_state.argtypes = [ct.c_void_p]
     L.cupcfd_hip_mesh_pull_state.argtypes = [ct.c_void_p]
     for n in ("cells_to_internal", "cells_from_internal", "faces_to_internal", "faces_from_internal"):
@@ -339,6 +341,13 @@ class DeviceMesh:
 
     def set_carry_init(self, v):
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
+
+    def autotune(self):
+        """Time the gather schedules of gradient, scalar and UVW on this mesh and keep the fastest of each;
+        returns the three choices (64 tiles, 4/6/8 lanes per cell, 1 thread per cell)."""
+        out = np.zeros(3, np.int32)
+        check("autotune", lib().cupcfd_hip_mesh_autotune(self._h, _ptr(out)))
+        return [int(x) for x in out]
 
     def set_schedule(self, lanes_per_cell):
         """0 = automatic, 1 = one thread per cell, 4, 6 or 8 lanes per cell, 64 = 64-cell tiles."""

@@ -164,6 +164,13 @@ int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
  * all-tet / all-hex meshes, else 8. Every schedule gives bit-identical
  * results. */
 int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
+/* Choose the gather schedule of the gradient, scalar and UVW loops separately
+ * by timing each candidate (cell tiles, the lane-per-face width the mesh
+ * allows, one thread per cell) on scratch inputs on a private stream; the
+ * mesh's boundary and region state is left as it was. schedules (may be NULL)
+ * receives the three choices. A later cupcfd_hip_mesh_set_schedule replaces
+ * them. Synchronous; no counterpart in the reference. */
+int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules);
 /* copy mutable mesh state view → device (push) or device → view (pull) */
 int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh);
 int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh);

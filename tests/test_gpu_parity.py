@@ -167,3 +167,31 @@ def test_schedules_bit_identical(meshname):
         for k in outs[(1, kernel)]:
             for lpc in (6, 8, 64):
                 assert compare(outs[(1, kernel)][k], outs[(lpc, kernel)][k])[0], (kernel, k, lpc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["typed_perm", "typed32", "tets_typed"])
+def test_autotune_keeps_bits_and_state(meshname):
+    """cupcfd_hip_mesh_autotune picks one valid schedule per gather kernel, leaves the mesh state as it was,
+    and the kernels keep giving the bits of one thread per cell."""
+    host, _ = _setup(meshname, 64)
+    dev = ch.DeviceMesh(host, 64)
+    host.set_state(seed_state(host))
+    dev.push_state()
+    before = host.state(64)
+    chosen = dev.autotune()
+    assert len(chosen) == 3 and all(c in (64, 4, 6, 8, 1) for c in chosen), chosen
+    dev.pull_state()
+    after = host.state(64)
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    ref = ch.DeviceMesh(host, 64)
+    ref.set_schedule(1)
+    for kernel in ("gradient", "scalar", "scalar_wall", "uvw"):
+        outs = []
+        for d in (dev, ref):
+            f = W.Fields(host, seed=5)
+            host.set_state(seed_state(host))
+            outs.append(run(kernel, fvm, d, f, lambda: host.state(64))[1])
+        for k in outs[1]:
+            assert compare(outs[0][k], outs[1][k])[0], (kernel, k)
