@@ -374,6 +374,90 @@ def bench_adjoint(torch, host, dev, steps, warmup, label):
     }
 
 
+def partition_sizes(nparts, seed):
+    """Grid sizes of ``nparts`` StructGen partitions with cell counts N log-uniform in [8^3, 64^3]:
+    n_x = n_y = round(N^(1/3)), n_z = round(N / n_x^2), each at least 8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nparts):
+        N = float(np.exp(rng.uniform(np.log(8**3), np.log(64**3))))
+        nx = max(8, int(round(N ** (1.0 / 3.0))))
+        nz = max(8, int(round(N / (nx * nx))))
+        out.append((nx, nx, nz))
+    return out
+
+
+def bench_partitions(torch, nparts, K, steps, warmup):
+    """The batched gradient (K scalars, nGradient = 1, FP64) over many uneven relabelled StructGen
+    partitions: one launch per partition, those launches captured in a CUDA/HIP graph, one persistent
+    launch over the whole mesh set, and that launch in a graph."""
+    import cupcfd_hip as ch
+    from cupcfd_hip.fvm import resident as R
+
+    sizes = partition_sizes(nparts, seed=48)
+    hosts = [ch.HostMesh(*sz, seed=100 + i, permute_cells=True, permute_faces=True) for i, sz in enumerate(sizes)]
+    devs = [ch.DeviceMesh(h, 64, reorder=True) for h in hosts]
+    mset = ch.DeviceMeshSet(devs)
+    cuda = torch.device("cuda:0")
+    rng = np.random.default_rng(7)
+    phi = [torch.from_numpy(rng.uniform(-1, 1, (h.nCells, K))).to(cuda) for h in hosts]
+    phiB = [torch.from_numpy(rng.uniform(-1, 1, (h.nBoundaries, K))).to(cuda) for h in hosts]
+    g = [torch.zeros((h.nCells, 3, K), dtype=torch.float64, device=cuda) for h in hosts]
+    go = [torch.zeros_like(x) for x in g]
+    ptr = lambda ts: torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device=cuda)  # noqa: E731
+    pp, pb, pg, pgo = ptr(phi), ptr(phiB), ptr(g), ptr(go)
+    C = sum(h.nCells for h in hosts)
+
+    def per_mesh():
+        for i, d in enumerate(devs):
+            R.gradient_batched(d, 1, K, phi[i], phiB[i], g[i], go[i])
+
+    def persistent():
+        R.gradient_batched_set(mset, K, pp, pb, pg, pgo)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps, (time.perf_counter() - t0) / steps
+
+    def graphed(fn):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr):
+            fn()
+        return gr.replay
+
+    res = {}
+    for name, fn in (("per_partition_launches", per_mesh), ("persistent_launch", persistent)):
+        res[name + "_ms"], res[name + "_host_ms"] = (x * 1e3 for x in timed(fn))
+        res[name + "_graph_ms"], res[name + "_graph_host_ms"] = (x * 1e3 for x in timed(graphed(fn)))
+    ref = [x.clone() for x in g]
+    persistent()
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(ref, g))
+    best = min(res["per_partition_launches_ms"], res["persistent_launch_ms"], res["persistent_launch_graph_ms"],
+               res["per_partition_launches_graph_ms"])
+    res.update({"workload": f"batched gradient, K={K} scalars, {nparts} relabelled StructGen partitions with cell "
+                            f"counts log-uniform in [8^3, 64^3], FP64",
+                "partitions": nparts, "cells": C, "K": K, "persistent_matches_per_partition": bool(same),
+                "best_gcell_scalar_per_s": C * K / (best / 1e3) / 1e9})
+    del mset
+    return res
+
+
 def structgen(n, seed):
     import cupcfd_hip as ch
     host = ch.HostMesh(n, seed=seed, permute_cells=True, permute_faces=True)
@@ -420,6 +504,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched K-scalar comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint comparison")
     ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral stress mesh")
+    ap.add_argument("--no-partitions", action="store_true", help="skip the many-partition batched launch")
+    ap.add_argument("--partitions-only", action="store_true", help="run only the many-partition batched launch")
     ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
     ap.add_argument("--tets-n", type=int, default=100)
     ap.add_argument("--tets-k", type=int, default=16)
@@ -445,6 +531,9 @@ def main():
         hb, db, lb = structgen(args.batch_n, 4242)
         print(json.dumps({"batched": bench_batched(torch, hb, db, args.batch_k, steps=args.steps,
                                                    warmup=args.warmup, label=lb)}))
+        return
+    if args.partitions_only:
+        print(json.dumps({"partitions": bench_partitions(torch, 48, 64, args.steps, args.warmup)}))
         return
     if args.tets_only:
         print(json.dumps({"tets": bench_tets(torch, args.tets_n, args.tets_k, args.steps, args.warmup)}))
@@ -511,6 +600,8 @@ def main():
         ha, da, la = structgen(args.adjoint_n, 12345)
         out["adjoint"] = bench_adjoint(torch, ha, da, steps=max(5, min(args.steps, 20)), warmup=2, label=la)
         del ha, da
+    if not args.no_partitions:
+        out["partitions"] = bench_partitions(torch, 48, 64, steps=max(3, min(args.steps, 10)), warmup=2)
     if not args.no_tets:
         torch.cuda.empty_cache()
         out["tets"] = bench_tets(torch, args.tets_n, args.tets_k, steps=max(3, min(args.steps, 10)), warmup=1)

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <new>
+#include <vector>
 
 #include "devmesh.hpp"
 
@@ -1603,14 +1605,53 @@ template <class T, class TI, class TO> struct BatchGradArgs {
     TO *out1, *out2;
 };
 
-template <class T, class TI, class TO, bool SWEEP0, int G, int RL>
-__global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, BatchGradArgs<T, TI, TO> a, int K, int per,
-                                                             int nblk) {
-    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
-    if (!L.live) return;
-    const int c = batch_uniform<G>(L.c);
-    const TI *phi = a.phi + L.k, *phiB = a.phiB + L.k;
-    const T *go = SWEEP0 ? nullptr : a.grado + L.k;
+// The fields of a device mesh the batched gradient reads (held in registers
+// by the mesh-set kernel, which switches meshes between work units)
+template <class T> struct GradGeom {
+    int C, rowlen;
+    const int *row;
+    const int2 *ent;
+    const V4<T> *fgeoS, *fgeoX, *cgeo;
+    const T *cvol;
+    __device__ __forceinline__ explicit GradGeom(const DevMesh<T> &m)
+        : C(m.C), rowlen(m.rowlen), row(m.row), ent(m.ent), fgeoS(m.fgeoS), fgeoX(m.fgeoX), cgeo(m.cgeo),
+          cvol(m.cvol) {}
+};
+
+// a value every lane of the wave holds, moved to scalar registers
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class P> __device__ __forceinline__ P *uni(P *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (P *)((hi << 32) | lo);
+}
+template <class T> __device__ __forceinline__ GradGeom<T> uni(GradGeom<T> g) {
+    g.C = uni(g.C);
+    g.rowlen = uni(g.rowlen);
+    g.row = uni(g.row);
+    g.ent = uni(g.ent);
+    g.fgeoS = uni(g.fgeoS);
+    g.fgeoX = uni(g.fgeoX);
+    g.cgeo = uni(g.cgeo);
+    g.cvol = uni(g.cvol);
+    return g;
+}
+template <class T, class TI, class TO>
+__device__ __forceinline__ BatchGradArgs<T, TI, TO> uni(BatchGradArgs<T, TI, TO> a) {
+    a.phi = uni(a.phi);
+    a.phiB = uni(a.phiB);
+    a.out1 = uni(a.out1);
+    a.out2 = uni(a.out2);
+    return a;
+}
+
+// the gradient of cell c for scalar k of a batch of K (M: DevMesh or GradGeom)
+template <class T, class TI, class TO, bool SWEEP0, int RL, class M>
+__device__ __forceinline__ void gradient_batched_cell(const M &m, const BatchGradArgs<T, TI, TO> &a, int K, int c,
+                                                      int kk) {
+    const TI *phi = a.phi + kk, *phiB = a.phiB + kk;
+    const T *go = SWEEP0 ? nullptr : a.grado + kk;
     auto cellv = [&](int x) -> GradCell<T> {
         GradCell<T> r;
         r.phi = (T)phi[(size_t)x * K];
@@ -1643,7 +1684,13 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, Batch
         return r;
     };
     int e0, n;
-    row_range(m, c, e0, n);
+    if (m.rowlen) {
+        e0 = c * m.rowlen;
+        n = m.rowlen;
+    } else {
+        e0 = m.row[c];
+        n = m.row[c + 1] - e0;
+    }
     if (RL) {
         int2 en[RL > 0 ? RL : 1];
         V4<T> sv[RL > 0 ? RL : 1], xv[RL > 0 ? RL : 1];
@@ -1671,9 +1718,64 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, Batch
     const T fact = 1.0 / vol;
     for (int j = 0; j < 3; j++) {
         const T r = acc[j] * fact;
-        const size_t o = ((size_t)c * 3 + j) * K + L.k;
+        const size_t o = ((size_t)c * 3 + j) * K + kk;
         a.out1[o] = (TO)r;
         if (a.out2) a.out2[o] = (TO)r;
+    }
+}
+
+template <class T, class TI, class TO, bool SWEEP0, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, BatchGradArgs<T, TI, TO> a, int K, int per,
+                                                             int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    gradient_batched_cell<T, TI, TO, SWEEP0, RL>(m, a, K, batch_uniform<G>(L.c), L.k);
+}
+
+// A set of meshes for one persistent launch: their device descriptors and,
+// per lane-group width, the prefix of work units (blocks of kBlock / G cells)
+template <class T> struct MeshSetArgs {
+    const DevMesh<T> *meshes;
+    const int *unit0;  // n + 1 entries
+    int n, nunits;
+};
+
+// Batched gradient (nGradient = 1) over every mesh of a set in one persistent
+// launch: logical block lb (XCD-aware, as xcd_block) takes the contiguous
+// unit range [lb*U/P, (lb+1)*U/P) of the U units of all meshes and leaves
+// after its last unit. phi[i] .. out2[i] are mesh i's arrays.
+template <class T, class TI, class TO, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_gradient_batched_set(MeshSetArgs<T> s, const TI *const *phi,
+                                                                 const TI *const *phiB, TO *const *out1,
+                                                                 TO *const *out2, int K) {
+    const int P = gridDim.x;
+    const int lb = xcd_block(blockIdx.x, P / kXcd);
+    const int u0 = (int)((int64_t)lb * s.nunits / P), u1 = (int)((int64_t)(lb + 1) * s.nunits / P);
+    if (u0 >= u1) return;
+    int i = 0;  // the mesh of unit u0, by bisection; later units advance it
+    {
+        int hi = s.n - 1;
+        while (i < hi) {
+            const int mid = (i + hi + 1) >> 1;
+            if (s.unit0[mid] <= u0) i = mid;
+            else hi = mid - 1;
+        }
+    }
+    GradGeom<T> m = uni(GradGeom<T>(s.meshes[i]));
+    BatchGradArgs<T, TI, TO> a = uni(BatchGradArgs<T, TI, TO>{phi[i], phiB[i], nullptr, out1[i], out2 ? out2[i] : nullptr});
+    int ub = s.unit0[i], ue = s.unit0[i + 1];
+    const int kk = blockIdx.y * G + threadIdx.x % G;
+    for (int u = u0; u < u1; u++) {
+        if (u >= ue) {  // next mesh with units
+            do i++;
+            while (s.unit0[i + 1] <= u);
+            m = uni(GradGeom<T>(s.meshes[i]));
+            a = uni(BatchGradArgs<T, TI, TO>{phi[i], phiB[i], nullptr, out1[i], out2 ? out2[i] : nullptr});
+            ub = s.unit0[i];
+            ue = s.unit0[i + 1];
+        }
+        const int c = batch_uniform<G>((u - ub) * (kBlock / G) + (int)threadIdx.x / G);
+        if (c < m.C && kk < K) gradient_batched_cell<T, TI, TO, true, RL>(m, a, K, c, kk);
     }
 }
 
@@ -2468,6 +2570,97 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
     return rc;
 }
 
+}  // namespace
+
+// A set of device meshes of one precision for persistent multi-mesh launches
+struct cupcfd_hip_meshset {
+    int precision;
+    int n;
+    void *meshes;      // DevMesh<T>[n] on the device
+    int *unit0[3];     // per lane-group width G = 4, 16, 64: prefix of work units, n + 1 entries, device
+    int nunits[3];
+    int rowlen;        // common row length of every member, else 0
+};
+
+namespace {
+
+inline int group_index(int G) { return G == 4 ? 0 : (G == 16 ? 1 : 2); }
+
+template <class T> int meshset_build(cupcfd_hip_mesh *const *meshes, int n, cupcfd_hip_meshset *set) {
+    std::vector<DevMesh<T>> d(n);
+    int rl = -1;
+    for (int i = 0; i < n; i++) {
+        Mesh<T> *M = get<T>(meshes[i]);
+        if (!M) return CUPCFD_E_NULL_PTR;
+        d[i] = M->d;
+        rl = (rl == -1 || rl == M->d.rowlen) ? M->d.rowlen : 0;
+    }
+    set->rowlen = rl < 0 ? 0 : rl;
+    if (hipMalloc(&set->meshes, (size_t)n * sizeof(DevMesh<T>)) != hipSuccess) return CUPCFD_E_ERROR;
+    if (hipMemcpy(set->meshes, d.data(), (size_t)n * sizeof(DevMesh<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return CUPCFD_E_ERROR;
+    const int Gs[3] = {4, 16, 64};
+    for (int g = 0; g < 3; g++) {
+        std::vector<int> u(n + 1, 0);
+        const int cpb = kBlock / Gs[g];
+        for (int i = 0; i < n; i++) {
+            const int64_t v = (int64_t)u[i] + (d[i].C + cpb - 1) / cpb;
+            if (v >= INT32_MAX) return CUPCFD_E_CONFIG_INVALID_VALUE;
+            u[i + 1] = (int)v;
+        }
+        set->nunits[g] = u[n];
+        if (hipMalloc(&set->unit0[g], (size_t)(n + 1) * sizeof(int)) != hipSuccess) return CUPCFD_E_ERROR;
+        if (hipMemcpy(set->unit0[g], u.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            return CUPCFD_E_ERROR;
+    }
+    return CUPCFD_E_SUCCESS;
+}
+
+template <class T>
+int dev_gradient_batched_set(cupcfd_hip_meshset *set, int K, const T *const *phi, const T *const *phiB,
+                             T *const *g, T *const *go, void *stream) {
+    if (!set || !phi || !phiB || !g) return CUPCFD_E_NULL_PTR;
+    if (set->precision != (int)(8 * sizeof(T))) return CUPCFD_E_NULL_PTR;
+    if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    const int G = batch_group(K), gi = group_index(G);
+    const int U = set->nunits[gi];
+    if (U == 0) return CUPCFD_E_SUCCESS;
+    MeshSetArgs<T> a{(const DevMesh<T> *)set->meshes, set->unit0[gi], set->n, U};
+    hipStream_t s = (hipStream_t)stream;
+    const int rl = set->rowlen;
+    // persistent grid: as many workgroups as can be resident at once (every
+    // range starts at once; a larger grid would leave ranges waiting for a
+    // slot), a multiple of the 8 XCDs, and no more than the units
+    auto grid_for = [&](const void *kernel) {
+        int dev = 0, cus = 256, per_cu = 1;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0);
+        const int cap = std::max(1, cus * std::max(1, per_cu)) / kXcd * kXcd;
+        const int P = std::max(kXcd, std::min(cap, ((U + kXcd - 1) / kXcd) * kXcd));
+        return dim3(P, (K + G - 1) / G);
+    };
+#define CUPCFD_GS(GG, RR)                                                                                      \
+    {                                                                                                          \
+        const void *kfn = (const void *)k_gradient_batched_set<T, T, T, GG, RR>;                               \
+        hipLaunchKernelGGL((k_gradient_batched_set<T, T, T, GG, RR>), grid_for(kfn), kBlock, 0, s, a, phi, phiB, \
+                           g, go, K);                                                                          \
+    }
+    if (G == 4) {
+        CUPCFD_GS(4, 0)
+    } else if (G == 16) {
+        CUPCFD_GS(16, 0)
+    } else if (rl == 6) {
+        CUPCFD_GS(64, 6)
+    } else if (rl == 4) {
+        CUPCFD_GS(64, 4)
+    } else {
+        CUPCFD_GS(64, 0)
+    }
+#undef CUPCFD_GS
+    return launch_ok();
+}
+
 int permute(cupcfd_hip_mesh *h, bool cells, const void *src, void *dst, int ncomp, int eb, void *stream, int dir) {
     if (!h || !src || !dst) return CUPCFD_E_NULL_PTR;
     if (ncomp < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
@@ -2714,6 +2907,45 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
                                       Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
                                       PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_hip_meshset_create(cupcfd_hip_mesh *const *meshes, int32_t n, cupcfd_hip_meshset **out) {
+    if (!meshes || !out) return CUPCFD_E_NULL_PTR;
+    *out = nullptr;
+    if (n < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    for (int i = 0; i < n; i++)
+        if (!meshes[i] || meshes[i]->precision != meshes[0]->precision) return CUPCFD_E_NULL_PTR;
+    cupcfd_hip_meshset *set = new (std::nothrow) cupcfd_hip_meshset();
+    if (!set) return CUPCFD_E_ERROR;
+    set->precision = meshes[0]->precision;
+    set->n = n;
+    const int rc = set->precision == 64 ? meshset_build<double>(meshes, n, set) : meshset_build<float>(meshes, n, set);
+    if (rc != CUPCFD_E_SUCCESS) {
+        cupcfd_hip_meshset_destroy(set);
+        return rc;
+    }
+    *out = set;
+    return CUPCFD_E_SUCCESS;
+}
+
+int cupcfd_hip_meshset_destroy(cupcfd_hip_meshset *set) {
+    if (!set) return CUPCFD_E_NULL_PTR;
+    if (set->meshes) (void)hipFree(set->meshes);
+    for (int *u : set->unit0)
+        if (u) (void)hipFree(u);
+    delete set;
+    return CUPCFD_E_SUCCESS;
+}
+
+int cupcfd_dev_gradient_batched_set_f64(cupcfd_hip_meshset *set, int32_t K, const double *const *phiCell,
+                                        const double *const *phiBoundary, double *const *dPhidxCell,
+                                        double *const *dPhidxoCell, void *stream) {
+    return dev_gradient_batched_set<double>(set, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream);
+}
+int cupcfd_dev_gradient_batched_set_f32(cupcfd_hip_meshset *set, int32_t K, const float *const *phiCell,
+                                        const float *const *phiBoundary, float *const *dPhidxCell,
+                                        float *const *dPhidxoCell, void *stream) {
+    return dev_gradient_batched_set<float>(set, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream);
 }
 
 int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules) {

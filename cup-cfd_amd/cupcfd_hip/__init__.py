@@ -125,6 +125,9 @@ BATCHED = ["gradient_batched", "flux_scalar_batched"]
 ADJOINT = ["cupcfd_dev_scalar_adjoint_f64"]
 # FP32-storage entry points on 64-bit meshes
 F32S = ["cupcfd_dev_gradient_batched_f32s"]
+# mesh sets: persistent launches over several meshes
+MESHSET = ["cupcfd_hip_meshset_create", "cupcfd_hip_meshset_destroy", "cupcfd_dev_gradient_batched_set_f64",
+           "cupcfd_dev_gradient_batched_set_f32"]
 
 
 def all_symbols():
@@ -136,7 +139,7 @@ def all_symbols():
             out.append(f"cupcfd_dev_{k}_{s}")
         for k in BATCHED:
             out.append(f"cupcfd_dev_{k}_{s}")
-    out += ADJOINT + F32S
+    out += ADJOINT + F32S + MESHSET
     return out
 
 
@@ -169,8 +172,10 @@ def _declare(L):
                 getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
         for k in BATCHED:
             getattr(L, f"cupcfd_dev_{k}_{s}").restype = ct.c_int
-    for k in ADJOINT + F32S:
+    for k in ADJOINT + F32S + MESHSET:
         getattr(L, k).restype = ct.c_int
+    L.cupcfd_hip_meshset_create.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_meshset_destroy.argtypes = [ct.c_void_p]
 
 
 def check(fn, code):
@@ -386,3 +391,26 @@ def version():
 
 
 from .fvm import *  # noqa: E402,F401,F403  (kernel wrappers)
+
+
+class DeviceMeshSet:
+    """A set of DeviceMesh objects of one precision for persistent multi-mesh launches
+    (cupcfd_hip_meshset_create). Keeps its members alive; destroyed before them."""
+
+    def __init__(self, meshes):
+        self.meshes = list(meshes)
+        assert self.meshes and len({m.precision for m in self.meshes}) == 1
+        self.precision = self.meshes[0].precision
+        self.sfx = self.meshes[0].sfx
+        arr = (ct.c_void_p * len(self.meshes))(*[m._h.value for m in self.meshes])
+        self._h = ct.c_void_p()
+        check("cupcfd_hip_meshset_create", lib().cupcfd_hip_meshset_create(arr, len(self.meshes), ct.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.cupcfd_hip_meshset_destroy(self._h)
+            self._h = None

@@ -273,3 +273,12 @@ class resident:
         assert mesh.precision == 64
         check("cupcfd_dev_gradient_batched_f32s", lib().cupcfd_dev_gradient_batched_f32s(
             mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB), cls._t(g), cls._t(go), cls._s(stream)))
+
+    @classmethod
+    def gradient_batched_set(cls, meshset, K, phi_ptrs, phiB_ptrs, g_ptrs, go_ptrs=None, stream=None):
+        """Batched gradient (nGradient = 1) on every mesh of a DeviceMeshSet in one persistent launch.
+        *_ptrs are int64 device tensors holding each member's device array address (data_ptr())."""
+        from . import lib, check
+        fn = f"cupcfd_dev_gradient_batched_set_{meshset.sfx}"
+        check(fn, getattr(lib(), fn)(meshset.handle, _I(K), cls._t(phi_ptrs), cls._t(phiB_ptrs), cls._t(g_ptrs),
+                                     cls._t(go_ptrs), cls._s(stream)))

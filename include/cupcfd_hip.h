@@ -355,6 +355,28 @@ int cupcfd_dev_gradient_batched_f32s(cupcfd_hip_mesh *mesh, int32_t nGradient, i
     const float *phiCell, const float *phiBoundary, float *dPhidxCell, float *dPhidxoCell, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Mesh sets: one persistent launch over many meshes of one precision (no
+ * counterpart in the reference; a rank owning several partitions would make
+ * one call per partition). The set holds the members' device descriptors; it
+ * must be destroyed before any member, and a member's schedule settings do
+ * not apply to set launches.
+ * cupcfd_dev_gradient_batched_set_*: GradientPhiGaussDolfyn with nGradient = 1
+ * for K scalars on every member, bit for bit what
+ * cupcfd_dev_gradient_batched_* gives per member. phiCell .. dPhidxoCell are
+ * DEVICE arrays of n device pointers (member i's arrays, laid out as for the
+ * batched entry point); dPhidxoCell may be NULL. The work (blocks of cells of
+ * every member) is split into contiguous ranges over a fixed grid of
+ * workgroups that each leave after their last range.
+ * ------------------------------------------------------------------------- */
+typedef struct cupcfd_hip_meshset cupcfd_hip_meshset;
+int cupcfd_hip_meshset_create(cupcfd_hip_mesh *const *meshes, int32_t n, cupcfd_hip_meshset **out);
+int cupcfd_hip_meshset_destroy(cupcfd_hip_meshset *set);
+int cupcfd_dev_gradient_batched_set_f64(cupcfd_hip_meshset *set, int32_t K, const double *const *phiCell,
+    const double *const *phiBoundary, double *const *dPhidxCell, double *const *dPhidxoCell, void *stream);
+int cupcfd_dev_gradient_batched_set_f32(cupcfd_hip_meshset *set, int32_t K, const float *const *phiCell,
+    const float *const *phiBoundary, float *const *dPhidxCell, float *const *dPhidxoCell, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Adjoint (resident mode). No counterpart in the reference: the reverse mode
  * of GradientPhiGaussDolfyn (GradientKernels.h:48-54, nGradient = 1) followed
  * by FluxScalarDolfynFaceLoop (ScalarKernels.h:47-67), with respect to

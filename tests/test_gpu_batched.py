@@ -160,3 +160,34 @@ def test_gradient_batched_f32_storage_is_rounded_f64(K, meshname, nG):
     torch.cuda.synchronize()
     assert _same(g32, g64.to(torch.float32))
     assert _same(go32, go64.to(torch.float32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("K", [3, 16, 64, 70])
+def test_gradient_batched_meshset_matches_per_mesh(K, precision):
+    """One persistent launch over a set of uneven meshes (hex and tet rows mixed) gives, per member, the bits of
+    the single-mesh batched gradient."""
+    specs = [dict(nx=9, ny=8, nz=8, seed=1, permute_cells=True, permute_faces=True),
+             dict(nx=3, ny=3, nz=3),
+             dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, seed=2, permute_cells=True, permute_faces=True),
+             dict(nx=17, ny=9, nz=8, seed=3, permute_cells=True),
+             dict(nx=8, ny=8, nz=8, typed=True, seed=4)]
+    hosts = [ch.HostMesh(**sp) for sp in specs]
+    devs = [ch.DeviceMesh(h, precision=precision, reorder=True) for h in hosts]
+    dt = torch.float64 if precision == 64 else torch.float32
+    rng = np.random.default_rng(K)
+    phi = [torch.from_numpy(rng.uniform(-1, 1, (h.nCells, K))).to(dt).cuda() for h in hosts]
+    phiB = [torch.from_numpy(rng.uniform(-1, 1, (h.nBoundaries, K))).to(dt).cuda() for h in hosts]
+    g = [torch.zeros((h.nCells, 3, K), dtype=dt, device="cuda") for h in hosts]
+    go = [torch.zeros_like(x) for x in g]
+    ptr = lambda ts: torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device="cuda")  # noqa: E731
+    mset = ch.DeviceMeshSet(devs)
+    R.gradient_batched_set(mset, K, ptr(phi), ptr(phiB), ptr(g), ptr(go))
+    for i, d in enumerate(devs):
+        g1 = torch.zeros_like(g[i])
+        go1 = torch.zeros_like(g[i])
+        R.gradient_batched(d, 1, K, phi[i], phiB[i], g1, go1)
+        torch.cuda.synchronize()
+        assert _same(g[i], g1) and _same(go[i], go1), i
+    del mset
