@@ -301,6 +301,14 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
     R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream)
     torch.cuda.synchronize()
     err32 = ((g32.double() - g_b).abs().max() / g_b.abs().max()).item()
+    # batched adjoint w.r.t. (phi_k, massFlux), seeded N(0,1) cotangents
+    rng2 = np.random.default_rng(99)
+    sbar = torch.from_numpy(rng2.normal(size=(C, K))).to(cuda)
+    rbar = torch.from_numpy(rng2.normal(size=2 * F)).to(cuda)
+    pbar = torch.empty((C, K), dtype=dt, device=cuda)
+    mbar = torch.empty(F, dtype=dt, device=cuda)
+    tadj = timed(lambda: R.scalar_adjoint_batched(dev, K, phi_b, sh["visEff"], mf, sbar, rbar, W.SCALAR, pbar, mbar,
+                                                  stream))
     tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
                                          sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
     return {
@@ -311,6 +319,7 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
         "speedup": tp / tb, "bit_identical_sample": bool(same),
         "batched_gradient_ms": tbg * 1e3, "batched_scalar_ms": tbs * 1e3,
         "batched_gradient_f32_storage_ms": tbg32 * 1e3, "f32_storage_gradient_rel_err": err32,
+        "batched_adjoint_ms": tadj * 1e3, "batched_forward_adjoint_ratio": (tb + tadj) / tb,
     }
 
 

@@ -2048,6 +2048,127 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(DevMesh<T> m, AdjArgs<T> a
     a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
 }
 
+// ---------------------------------------------------------------------------
+// Batched adjoint: K scalars sharing MassFlux, VisEff and RFace (no reference
+// counterpart). PhiBar_k is, bit for bit, the single-scalar adjoint's for
+// scalar k. MassFluxBar_f = rterm_f, then + (-(D_k*GammaBlend*(PhiFace -
+// dfci_k))) for k = 0..K-1 in order, so that K = 1 gives the single-scalar
+// result. Per-scalar arrays are scalar-innermost as for the batched forward
+// kernels (SuBar and PhiBar [c*K + k]).
+// ---------------------------------------------------------------------------
+// The phi-free parts of an interior face's adjoint terms, once per face for a
+// whole batch, with the expressions of the single-scalar pass A
+template <class T> struct AdjFaceCoef {
+    T wx, wy, wz, fmn, fmx;
+};
+
+template <class T> struct AdjBatchArgs {
+    ScalarArgs<T> s;  // phi ([c*K + k]), visEff, massFlux and the diffusivity parameters
+    const T *subar, *rbar;
+    T *h, *dir, *phibar, *mfbar;
+    AdjFaceCoef<T> *coef;
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_coef(DevMesh<T> m, AdjBatchArgs<T> a) {
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= m.Fi) return;
+    const int2 cn = m.fconn[f];
+    const int ip = cn.x, nn = cn.y;
+    const V4<T> s = m.fgeoS[f];
+    const T facn = s.w;
+    const T facp = 1.0 - facn;
+    T Visac = a.s.visEff[nn] * facn + a.s.visEff[ip] * facp;
+    Visac = scalar_visac(a.s, Visac);
+    const T VisFace = Visac * m.frl[f];
+    const v3<T> Xpn = sub3(xyz(m.cgeo[nn]), xyz(m.cgeo[ip]));
+    const T mf = a.s.massFlux[f];
+    a.coef[f] = AdjFaceCoef<T>{Visac * s.x - VisFace * Xpn.x, Visac * s.y - VisFace * Xpn.y,
+                               Visac * s.z - VisFace * Xpn.z, (T)fmin((double)mf, 0.0), (T)fmax((double)mf, 0.0)};
+}
+
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    const int c = batch_uniform<G>(L.c), kk = L.k;
+    T acc[4] = {0, 0, 0, 0};
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
+        const int2 en = m.ent[e];
+        if (en.y < 0) continue;  // boundary faces add nothing (the single-scalar fold adds -0.0)
+        const int f = en.x >> 1;
+        const bool in = (en.x & 1) != 0;
+        const int ip = in ? en.y : c, nn = in ? c : en.y;
+        const AdjFaceCoef<T> q = a.coef[f];
+        const T facn = m.fgeoS[f].w;
+        const T facp = 1.0 - facn;
+        const T D = a.subar[(size_t)ip * K + kk] - a.subar[(size_t)nn * K + kk];
+        const T Dg = D * a.s.gammaBlend;
+        const T fac = in ? facn : facp;
+        const T tt = D * fac;
+        acc[0] = acc[0] + tt * q.wx;
+        acc[1] = acc[1] + tt * q.wy;
+        acc[2] = acc[2] + tt * q.wz;
+        acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+    }
+    const T fact = 1.0 / m.cvol[c];
+    for (int j = 0; j < 3; j++) a.h[((size_t)c * 3 + j) * K + kk] = acc[j] * fact;
+    a.dir[(size_t)c * K + kk] = acc[3];
+}
+
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    const int c = batch_uniform<G>(L.c), kk = L.k;
+    T acc = a.dir[(size_t)c * K + kk];
+    int e0, n;
+    row_range(m, c, e0, n);
+    for (int e = e0; e < e0 + n; e++) {
+        const int2 en = m.ent[e];
+        if (en.y < 0) continue;
+        const int f = en.x >> 1;
+        const bool in = (en.x & 1) != 0;
+        const int ip = in ? en.y : c, nn = in ? c : en.y;
+        const V4<T> s = m.fgeoS[f];
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const T fac = in ? facn : facp;
+        const v3<T> hp = ld3s(a.h + kk, ip, K), hn = ld3s(a.h + kk, nn, K);
+        const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
+        const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
+        acc = acc + fac * pF;
+    }
+    a.phibar[(size_t)c * K + kk] = acc;
+}
+
+// one thread per face, adding its K scalars' terms in order after the RFace term
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_faces_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K) {
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= m.F) return;
+    const int2 cn = m.fconn[f];
+    if (cn.y < 0) {
+        a.mfbar[f] = 0.0;
+        return;
+    }
+    const T mf = a.s.massFlux[f];
+    const T PhiFace = m.carry_init;
+    T acc = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
+    const T *sp = a.subar + (size_t)cn.x * K, *sn = a.subar + (size_t)cn.y * K;
+    const T *pu = mf < 0 ? a.s.phi + (size_t)cn.y * K : a.s.phi + (size_t)cn.x * K;  // the upwind cell's phi
+    const bool up = mf < 0 || mf > 0;
+    for (int kk = 0; kk < K; kk++) {
+        const T D = sp[kk] - sn[kk];
+        const T Dg = D * a.s.gammaBlend;
+        const T dfci = up ? pu[kk] : (T)0.0;
+        acc = acc + (-(Dg * (PhiFace - dfci)));
+    }
+    a.mfbar[f] = acc;
+}
+
 // values in [0.5, 1.5) from a hash of the index: finite, non-zero inputs for timing runs
 template <class T> __global__ __launch_bounds__(kBlock) void k_fill_timing(T *x, size_t n) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -2459,6 +2580,51 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
         }
     }
     if (d.F) hipLaunchKernelGGL(k_adj_faces<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a);
+    return launch_ok();
+}
+
+template <class T>
+int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T *VisEff, const T *MF, const T *SuBar,
+                               const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl,
+                               int VarTE, T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend,
+                               T *PhiBar, T *MFBar, void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    const DevMesh<T> &d = M->d;
+    if (d.typed) return CUPCFD_E_NOT_IMPLEMENTED;
+    AdjBatchArgs<T> a{};
+    const int rc = scalar_args<T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, SolveTurb, 0, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE,
+                                  Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, (T)0, (T)1);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    a.subar = SuBar;
+    a.rbar = RFaceBar;
+    a.phibar = PhiBar;
+    a.mfbar = MFBar;
+    hipStream_t st = (hipStream_t)stream;
+    if (d.C) {
+        a.h = (T *)M->ws_get(49, 3 * (size_t)d.C * K * sizeof(T));
+        a.dir = (T *)M->ws_get(50, (size_t)d.C * K * sizeof(T));
+        a.coef = (AdjFaceCoef<T> *)M->ws_get(51, (size_t)(d.Fi ? d.Fi : 1) * sizeof(AdjFaceCoef<T>));
+        if (!a.h || !a.dir || !a.coef) return CUPCFD_E_ERROR;
+        if (d.Fi) hipLaunchKernelGGL(k_adj_coef<T>, dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a);
+        const int G = batch_group(K);
+        const Grid gr = batch_grid(d.C, G);
+        const dim3 grid(gr.blocks, (K + G - 1) / G);
+        if (G == 4) {
+            hipLaunchKernelGGL((k_adj_a_batched<T, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched<T, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 16) {
+            hipLaunchKernelGGL((k_adj_a_batched<T, 16>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched<T, 16>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else {
+            hipLaunchKernelGGL((k_adj_a_batched<T, 64>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched<T, 64>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        }
+    }
+    if (d.F)
+        hipLaunchKernelGGL(k_adj_faces_batched<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, K);
     return launch_ok();
 }
 
@@ -2907,6 +3073,17 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
                                       Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
                                       PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_dev_scalar_adjoint_batched_f64(cupcfd_hip_mesh *mesh, int32_t K, const double *PhiCell,
+                                          const double *VisEff, const double *MassFlux, const double *SuBar,
+                                          const double *RFaceBar, int32_t SolveTurb, double vislam, int32_t ivar,
+                                          int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE, double Sigma_k,
+                                          int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
+                                          double GammaBlend, double *PhiBar, double *MassFluxBar, void *stream) {
+    return dev_scalar_adjoint_batched<double>(mesh, K, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam,
+                                              ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s,
+                                              Schmidt, GammaBlend, PhiBar, MassFluxBar, stream);
 }
 
 int cupcfd_hip_meshset_create(cupcfd_hip_mesh *const *meshes, int32_t n, cupcfd_hip_meshset **out) {

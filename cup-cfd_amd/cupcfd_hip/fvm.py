@@ -282,3 +282,17 @@ class resident:
         fn = f"cupcfd_dev_gradient_batched_set_{meshset.sfx}"
         check(fn, getattr(lib(), fn)(meshset.handle, _I(K), cls._t(phi_ptrs), cls._t(phiB_ptrs), cls._t(g_ptrs),
                                      cls._t(go_ptrs), cls._s(stream)))
+
+    @classmethod
+    def scalar_adjoint_batched(cls, mesh, K, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None):
+        """Batched adjoint: phi, subar, phibar (C, K) scalar-innermost; visEff, mf, rbar shared; mfbar (F) sums
+        the K scalars' terms after the RFace term."""
+        from . import lib, check
+        t = cls._t
+        p = params
+        D = ct.c_double
+        check("cupcfd_dev_scalar_adjoint_batched_f64", lib().cupcfd_dev_scalar_adjoint_batched_f64(
+            mesh.handle, _I(K), t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
+            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
+            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
+            t(mfbar), cls._s(stream)))

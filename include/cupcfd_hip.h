@@ -388,6 +388,16 @@ int cupcfd_dev_gradient_batched_set_f32(cupcfd_hip_meshset *set, int32_t K, cons
  * return E_NOT_IMPLEMENTED. Sums are taken per cell over its faces in
  * ascending face id, so results are deterministic.
  * ------------------------------------------------------------------------- */
+/* Batched adjoint: K scalars with the batched layout (PhiCell, SuBar, PhiBar
+ * [c*K + k]) sharing VisEff, MassFlux and RFace. PhiBar for scalar k is, bit
+ * for bit, cupcfd_dev_scalar_adjoint_f64's for that scalar; MassFluxBar is the
+ * RFace cotangent term, then the K scalars' terms added in order k = 0..K-1
+ * (K = 1 gives cupcfd_dev_scalar_adjoint_f64's result). */
+int cupcfd_dev_scalar_adjoint_batched_f64(cupcfd_hip_mesh *mesh, int32_t K, const double *PhiCell,
+    const double *VisEff, const double *MassFlux, const double *SuBar, const double *RFaceBar,
+    int32_t SolveTurb, double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl,
+    int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
+    double GammaBlend, double *PhiBar, double *MassFluxBar, void *stream);
 int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
     const double *MassFlux, const double *SuBar, const double *RFaceBar, int32_t SolveTurb,
     double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,

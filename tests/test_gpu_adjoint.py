@@ -190,3 +190,32 @@ def test_march_checkpointed_reverse():
         fd = (jp - jm) / (2 * eps)
         ad = (torch.dot(pb10, pd_t) + torch.dot(mb10, md_t)).item()
         assert abs(fd - ad) <= 1e-6 * max(1.0, abs(ad)), (fd, ad)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("K", [1, 3, 16, 70])
+@pytest.mark.parametrize("name", ["perm12", "tets"])
+def test_batched_adjoint_matches_single(name, K):
+    """PhiBar_k equals the single-scalar adjoint's bit for bit; MassFluxBar is the RFace term followed by the
+    K scalars' terms added in order (each term taken from a single-scalar call with a zero RFace cotangent)."""
+    c = case(name)
+    rng = np.random.default_rng(40 + K)
+    phi = rng.uniform(-1, 1, (c.C, K))
+    sb = rng.normal(size=(c.C, K))
+    rb = rng.normal(size=2 * c.F)
+    mf, vis = c.faces(c.f.massFlux), c.cells(c.f.visEff)
+    pb = torch.empty((c.C, K), dtype=torch.float64, device="cuda")
+    mb = torch.empty(c.F, dtype=torch.float64, device="cuda")
+    R.scalar_adjoint_batched(c.dev, K, c.cells(phi), vis, mf, c.cells(sb), c.faces(rb, 2), P, pb, mb)
+    zero_c = torch.zeros(c.C, dtype=torch.float64, device="cuda")
+    zero_r = torch.zeros(2 * c.F, dtype=torch.float64, device="cuda")
+    p1 = torch.empty(c.C, dtype=torch.float64, device="cuda")
+    m1 = torch.empty(c.F, dtype=torch.float64, device="cuda")
+    R.scalar_adjoint(c.dev, c.cells(phi[:, 0]), vis, mf, zero_c, c.faces(rb, 2), P, p1, m1)
+    want = m1.cpu().numpy().copy()  # the RFace term alone
+    for k in range(K):
+        R.scalar_adjoint(c.dev, c.cells(phi[:, k]), vis, mf, c.cells(sb[:, k]), zero_r, P, p1, m1)
+        torch.cuda.synchronize()
+        assert torch.equal(pb[:, k].contiguous(), p1), k
+        want = want + m1.cpu().numpy()
+    assert np.array_equal(mb.cpu().numpy(), want)
