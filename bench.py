@@ -501,6 +501,26 @@ def bench_tets(torch, n, K, steps, warmup):
     return out
 
 
+def bench_mesh_build(torch, host, dev_host, t_host_repack, label, reps=3):
+    """Host repack (cupcfd_hip_mesh_create_f64) against the GPU repack of the same view
+    (cupcfd_hip_mesh_create_device_f64): wall time of each create call, uploads and
+    host-side mirrors included, and whether the two device meshes are identical."""
+    import cupcfd_hip as ch
+
+    times = []
+    dev = None
+    for _ in range(reps):
+        dev = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev = ch.DeviceMesh(host, 64, reorder=True, on_device=True)
+        times.append(time.perf_counter() - t0)
+    same = dev.digest() == dev_host.digest()
+    return {"mesh": label, "cells": host.nCells, "faces": host.nFaces, "host_repack_s": t_host_repack,
+            "device_repack_s": min(times), "device_repack_first_s": times[0],
+            "speedup": t_host_repack / min(times), "identical": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -514,6 +534,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint comparison")
     ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral stress mesh")
     ap.add_argument("--no-partitions", action="store_true", help="skip the many-partition batched launch")
+    ap.add_argument("--no-mesh-build", action="store_true", help="skip the host vs GPU mesh repack timing")
     ap.add_argument("--partitions-only", action="store_true", help="run only the many-partition batched launch")
     ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
     ap.add_argument("--tets-n", type=int, default=100)
@@ -550,8 +571,10 @@ def main():
     n = args.n
     t0 = time.time()
     host = ch.HostMesh(n, seed=12345, permute_cells=True, permute_faces=True)
+    t_gen = time.time() - t0
     dev = ch.DeviceMesh(host, 64, reorder=True)
     t_mesh = time.time() - t0
+    t_host_repack = t_mesh - t_gen
     if args.schedule >= 0:
         dev.set_schedule(args.schedule)
         schedules = [args.schedule] * 3
@@ -600,6 +623,8 @@ def main():
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
+    if not args.no_mesh_build:
+        out["mesh_build"] = bench_mesh_build(torch, host, dev, t_host_repack, f"StructGen {n}^3 relabelled")
     if not args.no_batched:
         hb, db, lb = structgen(args.batch_n, 4242)
         out["batched"] = bench_batched(torch, hb, db, args.batch_k, steps=max(2, min(args.steps, 10)), warmup=1,

@@ -73,14 +73,7 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
 
 template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     const int C = v.nCells, F = v.nFaces, B = v.nBoundaries, R = v.nRegions;
-    if (C < 0 || F < 0 || B < 0 || R < 0) return CUPCFD_E_CONFIG_INVALID_VALUE;
-    if ((int64_t)F >= ((int64_t)1 << 30)) return CUPCFD_E_CONFIG_INVALID_VALUE;
-    if ((C && !v.cellVolume) || (F && (!v.faceCell1ID || !v.faceCell2ID || !v.faceBoundaryID || !v.faceLambda ||
-                                      !v.faceArea || !v.faceRLencos || !v.faceNorm.base || !v.faceCenter.base ||
-                                      !v.faceXpac.base || !v.faceXnac.base)) ||
-        (C && !v.cellCenter.base) || (B && (!v.boundaryFaceID || !v.boundaryRegionID || !v.boundaryDistance)) ||
-        (R && !v.regionType))
-        return CUPCFD_E_NULL_PTR;
+    if (int rc = view_check(v); rc != CUPCFD_E_SUCCESS) return rc;
 
     // ---- validation -------------------------------------------------------
     int Fi = 0;
@@ -505,52 +498,55 @@ template int *scalar_carry_src<float>(Mesh<float> &, bool, bool);
 
 using namespace cupcfd_hip;
 
-extern "C" int cupcfd_hip_mesh_create_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out) {
+namespace {
+
+// Build a device mesh from a view, on the host or on the device; the handle
+// keeps the view for drop-in state mirroring.
+template <class T, class View>
+int create_mesh(const View *view, int32_t flags, cupcfd_hip_mesh **out, bool on_device, View Mesh<T>::*keep) {
     if (!view || !out) return CUPCFD_E_NULL_PTR;
     *out = nullptr;
-    auto *h = new (std::nothrow) cupcfd_hip_mesh{64, new (std::nothrow) Mesh<double>(), nullptr};
-    if (!h || !h->m64) return CUPCFD_E_ERROR;
+    auto *M = new (std::nothrow) Mesh<T>();
+    if (!M) return CUPCFD_E_ERROR;
     int rc;
     try {
-        rc = build(*h->m64, *view, flags);
+        rc = on_device ? build_device(*M, *view, flags) : build(*M, *view, flags);
     } catch (const std::bad_alloc &) {
         rc = CUPCFD_E_ERROR;
     }
     if (rc == CUPCFD_E_SUCCESS) {
-        h->m64->view64 = *view;
-        rc = sync_state(*h->m64, *view, true);
+        M->*keep = *view;
+        rc = sync_state(*M, *view, true);
     }
-    if (rc != CUPCFD_E_SUCCESS) {
-        delete h->m64;
-        delete h;
-        return rc;
+    auto *h = rc == CUPCFD_E_SUCCESS ? new (std::nothrow) cupcfd_hip_mesh{} : nullptr;
+    if (!h) {
+        delete M;
+        return rc == CUPCFD_E_SUCCESS ? CUPCFD_E_ERROR : rc;
     }
+    if constexpr (sizeof(T) == 8) *h = cupcfd_hip_mesh{64, M, nullptr};
+    else *h = cupcfd_hip_mesh{32, nullptr, M};
     *out = h;
     return CUPCFD_E_SUCCESS;
 }
 
+}  // namespace
+
+extern "C" int cupcfd_hip_mesh_create_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out) {
+    return create_mesh<double>(view, flags, out, false, &Mesh<double>::view64);
+}
+
 extern "C" int cupcfd_hip_mesh_create_f32(const cupcfd_mesh_view_f32 *view, int32_t flags, cupcfd_hip_mesh **out) {
-    if (!view || !out) return CUPCFD_E_NULL_PTR;
-    *out = nullptr;
-    auto *h = new (std::nothrow) cupcfd_hip_mesh{32, nullptr, new (std::nothrow) Mesh<float>()};
-    if (!h || !h->m32) return CUPCFD_E_ERROR;
-    int rc;
-    try {
-        rc = build(*h->m32, *view, flags);
-    } catch (const std::bad_alloc &) {
-        rc = CUPCFD_E_ERROR;
-    }
-    if (rc == CUPCFD_E_SUCCESS) {
-        h->m32->view32 = *view;
-        rc = sync_state(*h->m32, *view, true);
-    }
-    if (rc != CUPCFD_E_SUCCESS) {
-        delete h->m32;
-        delete h;
-        return rc;
-    }
-    *out = h;
-    return CUPCFD_E_SUCCESS;
+    return create_mesh<float>(view, flags, out, false, &Mesh<float>::view32);
+}
+
+extern "C" int cupcfd_hip_mesh_create_device_f64(const cupcfd_mesh_view_f64 *view, int32_t flags,
+                                                 cupcfd_hip_mesh **out) {
+    return create_mesh<double>(view, flags, out, true, &Mesh<double>::view64);
+}
+
+extern "C" int cupcfd_hip_mesh_create_device_f32(const cupcfd_mesh_view_f32 *view, int32_t flags,
+                                                 cupcfd_hip_mesh **out) {
+    return create_mesh<float>(view, flags, out, true, &Mesh<float>::view32);
 }
 
 extern "C" int cupcfd_hip_mesh_destroy(cupcfd_hip_mesh *mesh) {
@@ -590,6 +586,129 @@ extern "C" int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *
     const auto &v = mesh->precision == 64 ? mesh->m64->face_order : mesh->m32->face_order;
     std::copy(v.begin(), v.end(), order);
     return CUPCFD_E_SUCCESS;
+}
+
+namespace {
+
+const char *const kDigestFields[] = {
+    "counts",  "fconn",   "fgeoS",   "fgeoX",   "fgeoP",    "fgeoN",   "frl",      "cgeo",    "cvol",
+    "row",     "ent",     "bface",   "bregion", "bcell",    "bdist",   "rtype",    "rstd",    "rflux",
+    "radiab",  "relog",   "rdensity", "rturbke", "rturbdiss", "rsplvl", "rden",     "rr",      "rt",
+    "ruvw",    "carry_src", "bcells", "blist",  "ocells",   "orow",    "oent",     "halo_count", "tinfo",
+    "thp",     "thf",     "thc",     "eslot",   "cell_order", "face_order", "cell_rank", "face_rank", "fconn_h",
+    "btype_h", "bregion_h", "radiab_h", "ntype"};
+constexpr int kNDigest = (int)(sizeof(kDigestFields) / sizeof(kDigestFields[0]));
+
+// 64-bit hash of a byte string, 8 bytes per step (FNV-1a constants)
+struct Hash64 {
+    uint64_t h = 0xcbf29ce484222325ull;
+    void add(const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            h = (h ^ w) * 0x100000001b3ull;
+            h ^= h >> 29;
+        }
+        for (; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+        h ^= n;
+    }
+};
+
+template <class T> int mesh_digest(const Mesh<T> &M, uint64_t *out) {
+    const DevMesh<T> &d = M.d;
+    std::vector<unsigned char> buf;
+    bool ok = true;
+    int i = 0;
+    auto dev = [&](const void *p, size_t bytes) {
+        Hash64 h;
+        if (ok && bytes) {
+            buf.resize(bytes);
+            ok = hipMemcpy(buf.data(), p, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+            h.add(buf.data(), bytes);
+        }
+        out[i++] = h.h;
+    };
+    auto host = [&](const void *p, size_t bytes) {
+        Hash64 h;
+        h.add(p, bytes);
+        out[i++] = h.h;
+    };
+    const size_t C = d.C, F = d.F, B = d.B, R = d.R, nnz = d.nnz;
+    const int32_t counts[] = {d.C,     d.F,     d.Fi,    d.B,          d.R,          d.nnz,        d.maxrow,
+                              d.rowlen, d.typed, d.nbc,   d.nblist[0],  d.nblist[1],  d.nblist[2],  d.nblist[3],
+                              d.nblist[4], d.noc, d.tcells, d.ntiles,   d.tslots,     d.tnnz};
+    host(counts, sizeof(counts));
+    dev(d.fconn, F * sizeof(int2));
+    dev(d.fgeoS, F * sizeof(V4<T>));
+    dev(d.fgeoX, F * sizeof(V4<T>));
+    dev(d.fgeoP, F * sizeof(V4<T>));
+    dev(d.fgeoN, F * sizeof(V2<T>));
+    dev(d.frl, F * sizeof(T));
+    dev(d.cgeo, C * sizeof(V4<T>));
+    dev(d.cvol, C * sizeof(T));
+    dev(d.row, (C + 1) * sizeof(int));
+    dev(d.ent, nnz * sizeof(int2));
+    dev(d.bface, B * sizeof(int));
+    dev(d.bregion, B * sizeof(int));
+    dev(d.bcell, B * sizeof(int));
+    dev(d.bdist, B * sizeof(T));
+    dev(d.rtype, R * sizeof(int));
+    dev(d.rstd, R);
+    dev(d.rflux, R);
+    dev(d.radiab, R);
+    for (const T *p : {d.relog, d.rdensity, d.rturbke, d.rturbdiss, d.rsplvl, d.rden, d.rr, d.rt}) dev(p, R * sizeof(T));
+    dev(d.ruvw, 3 * R * sizeof(T));
+    dev(d.carry_src, d.typed ? F * sizeof(int) : 0);
+    dev(d.bcells, d.nbc * sizeof(int));
+    {  // the five by-type lists as one sequence
+        Hash64 h;
+        for (int t = 0; t < 5 && ok; t++) {
+            buf.resize(d.nblist[t] * sizeof(int));
+            if (d.nblist[t])
+                ok = hipMemcpy(buf.data(), d.blist[t], buf.size(), hipMemcpyDeviceToHost) == hipSuccess;
+            h.add(buf.data(), buf.size());
+        }
+        out[i++] = h.h;
+    }
+    dev(d.ocells, d.noc * sizeof(int));
+    dev(d.orow, (d.noc + 1) * sizeof(int));
+    dev(d.oent, d.nblist[CUPCFD_RTYPE_OUTLET] * sizeof(int));
+    // tile arrays are read only when every tile fits its 15-bit slots
+    const size_t NT = d.ntiles;
+    int H = 0;
+    if (NT && ok) ok = hipMemcpy(&H, d.thp + NT, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    host(&H, sizeof(H));
+    dev(d.tinfo, NT * sizeof(int4));
+    dev(d.thp, NT ? (NT + 1) * sizeof(int) : 0);
+    dev(d.thf, H * sizeof(int));
+    dev(d.thc, H * sizeof(int2));
+    dev(d.eslot, NT ? nnz * sizeof(uint16_t) : 0);
+    host(M.cell_order.data(), M.cell_order.size() * sizeof(int32_t));
+    host(M.face_order.data(), M.face_order.size() * sizeof(int32_t));
+    host(M.cell_rank.data(), M.cell_rank.size() * sizeof(int32_t));
+    host(M.face_rank.data(), M.face_rank.size() * sizeof(int32_t));
+    host(M.fconn_h.data(), M.fconn_h.size() * sizeof(int32_t));
+    host(M.btype_h.data(), M.btype_h.size() * sizeof(int32_t));
+    host(M.bregion_h.data(), M.bregion_h.size() * sizeof(int32_t));
+    host(M.radiab_h.data(), M.radiab_h.size());
+    host(M.ntype, sizeof(M.ntype));
+    return ok && i == kNDigest ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+}
+
+}  // namespace
+
+extern "C" int cupcfd_hip_mesh_digest(const cupcfd_hip_mesh *mesh, uint64_t *digest, int32_t capacity,
+                                      int32_t *count) {
+    if (!mesh || !digest || !count) return CUPCFD_E_NULL_PTR;
+    *count = kNDigest;
+    if (capacity < kNDigest) return CUPCFD_E_ARRAY_SIZE_MISMATCH;
+    return mesh->precision == 64 ? mesh_digest(*mesh->m64, digest) : mesh_digest(*mesh->m32, digest);
+}
+
+extern "C" const char *cupcfd_hip_mesh_digest_field(int32_t i) {
+    return i >= 0 && i < kNDigest ? kDigestFields[i] : nullptr;
 }
 
 extern "C" int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value) {

@@ -125,6 +125,24 @@ template <class T> struct Mesh {
 template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns);
 int sync_state_any(struct ::cupcfd_hip_mesh *h, bool push);
 
+// Counts and pointers a MiniSoA view must carry before either repack reads it.
+template <class View> int view_check(const View &v) {
+    const int C = v.nCells, F = v.nFaces, B = v.nBoundaries, R = v.nRegions;
+    if (C < 0 || F < 0 || B < 0 || R < 0) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if ((int64_t)F >= ((int64_t)1 << 30)) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if ((C && !v.cellVolume) || (F && (!v.faceCell1ID || !v.faceCell2ID || !v.faceBoundaryID || !v.faceLambda ||
+                                      !v.faceArea || !v.faceRLencos || !v.faceNorm.base || !v.faceCenter.base ||
+                                      !v.faceXpac.base || !v.faceXnac.base)) ||
+        (C && !v.cellCenter.base) || (B && (!v.boundaryFaceID || !v.boundaryRegionID || !v.boundaryDistance)) ||
+        (R && !v.regionType))
+        return CUPCFD_E_NULL_PTR;
+    return CUPCFD_E_SUCCESS;
+}
+
+// The same repack as the host build, run on the device: every array of the
+// DevMesh and every host-side mirror of Mesh comes out bit-identical.
+template <class T, class View> int build_device(Mesh<T> &M, const View &v, int flags);
+
 }  // namespace cupcfd_hip
 
 struct cupcfd_hip_mesh {

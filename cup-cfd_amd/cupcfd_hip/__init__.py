@@ -112,7 +112,8 @@ HOST_SYMBOLS = [
     "cupcfd_host_mesh_destroy", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
-    "cupcfd_hip_mesh_autotune",
+    "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
+    "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field",
     "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
@@ -151,6 +152,11 @@ def _declare(L):
     L.cupcfd_host_mesh_destroy.restype = None
     L.cupcfd_hip_mesh_create_f64.argtypes = [ct.POINTER(MeshViewF64), ct.c_int32, ct.POINTER(ct.c_void_p)]
     L.cupcfd_hip_mesh_create_f32.argtypes = [ct.POINTER(MeshViewF32), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_create_device_f64.argtypes = [ct.POINTER(MeshViewF64), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_create_device_f32.argtypes = [ct.POINTER(MeshViewF32), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_digest.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_int32)]
+    L.cupcfd_hip_mesh_digest_field.argtypes = [ct.c_int32]
+    L.cupcfd_hip_mesh_digest_field.restype = ct.c_char_p
     L.cupcfd_hip_mesh_destroy.argtypes = [ct.c_void_p]
     L.cupcfd_hip_mesh_counts.argtypes = [ct.c_void_p] + [ct.POINTER(ct.c_int32)] * 4
     L.cupcfd_hip_mesh_cell_order.argtypes = [ct.c_void_p, ct.c_void_p]
@@ -314,9 +320,10 @@ def _n(a, per=1):
 
 
 class DeviceMesh:
-    """A mesh repacked into HBM (cupcfd_hip_mesh_create_f64 / _f32)."""
+    """A mesh repacked into HBM (cupcfd_hip_mesh_create_f64 / _f32, or with ``on_device`` the GPU repack
+    cupcfd_hip_mesh_create_device_f64 / _f32, which builds the same arrays)."""
 
-    def __init__(self, host: HostMesh, precision=64, reorder=True):
+    def __init__(self, host: HostMesh, precision=64, reorder=True, on_device=False):
         self.host = host
         self.precision = precision
         self.T = ct.c_double if precision == 64 else ct.c_float
@@ -326,12 +333,9 @@ class DeviceMesh:
         self._h = ct.c_void_p()
         L = lib()
         flags = MESH_REORDER if reorder else 0
-        if precision == 64:
-            check("cupcfd_hip_mesh_create_f64",
-                  L.cupcfd_hip_mesh_create_f64(ct.byref(host.view64), flags, ct.byref(self._h)))
-        else:
-            check("cupcfd_hip_mesh_create_f32",
-                  L.cupcfd_hip_mesh_create_f32(ct.byref(host.view32()), flags, ct.byref(self._h)))
+        fn = f"cupcfd_hip_mesh_create{'_device' if on_device else ''}_{self.sfx}"
+        view = host.view64 if precision == 64 else host.view32()
+        check(fn, getattr(L, fn)(ct.byref(view), flags, ct.byref(self._h)))
         c = [ct.c_int32() for _ in range(4)]
         check("cupcfd_hip_mesh_counts", L.cupcfd_hip_mesh_counts(self._h, *[ct.byref(x) for x in c]))
         self.nCells, self.nFaces, self.nBoundaries, self.nRegions = [x.value for x in c]
@@ -343,6 +347,14 @@ class DeviceMesh:
     @property
     def handle(self):
         return self._h
+
+    def digest(self):
+        """{field: 64-bit hash} of every device array and host-side order of this mesh (cupcfd_hip_mesh_digest)."""
+        L = lib()
+        out = np.zeros(128, np.uint64)
+        n = ct.c_int32()
+        check("cupcfd_hip_mesh_digest", L.cupcfd_hip_mesh_digest(self._h, _ptr(out), out.size, ct.byref(n)))
+        return {L.cupcfd_hip_mesh_digest_field(i).decode(): int(out[i]) for i in range(n.value)}
 
     def set_carry_init(self, v):
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))

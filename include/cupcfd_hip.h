@@ -148,6 +148,21 @@ typedef struct cupcfd_hip_mesh cupcfd_hip_mesh;
  * mesh state through them, so they must outlive the device mesh. */
 int cupcfd_hip_mesh_create_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out);
 int cupcfd_hip_mesh_create_f32(const cupcfd_mesh_view_f32 *view, int32_t flags, cupcfd_hip_mesh **out);
+/* The same repack run on the GPU (stable radix sorts and scans over the
+ * uploaded MiniSoA arrays) instead of the host: every device array and every
+ * order it reports are bit-identical to cupcfd_hip_mesh_create_*'s, and the
+ * errors are the same. It replaces the host-side map building of
+ * CupCfdSoAMesh::updateCellFaceMap (CupCfdSoAMesh.cpp:549-668), which the
+ * reference runs in finalize. */
+int cupcfd_hip_mesh_create_device_f64(const cupcfd_mesh_view_f64 *view, int32_t flags, cupcfd_hip_mesh **out);
+int cupcfd_hip_mesh_create_device_f32(const cupcfd_mesh_view_f32 *view, int32_t flags, cupcfd_hip_mesh **out);
+/* One 64-bit hash per array of the device mesh and per host-side order it
+ * keeps, in a fixed order (cupcfd_hip_mesh_digest_field names entry i), so two
+ * builds can be compared field by field. count receives the number of
+ * entries; capacity smaller than that returns CUPCFD_E_ARRAY_SIZE_MISMATCH.
+ * Synchronous; no counterpart in the reference. */
+int cupcfd_hip_mesh_digest(const cupcfd_hip_mesh *mesh, uint64_t *digest, int32_t capacity, int32_t *count);
+const char *cupcfd_hip_mesh_digest_field(int32_t i);
 int cupcfd_hip_mesh_destroy(cupcfd_hip_mesh *mesh);
 int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t *nFaces,
                            int32_t *nBoundaries, int32_t *nRegions);

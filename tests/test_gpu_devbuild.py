@@ -1,0 +1,165 @@
+"""The GPU mesh repack (cupcfd_hip_mesh_create_device_*) against the host repack.
+
+The host repack is what every kernel parity test runs on. The GPU repack must
+produce the same device mesh: every array, every count and every host-side
+order, compared field by field through cupcfd_hip_mesh_digest (integer and
+byte work: bit-exact). Kernel runs on both builds must give bit-identical
+outputs, and bad views must be refused with the same eCodes.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import cupcfd_hip as ch
+from cupcfd_hip import fvm
+from cupcfd_hip import workload as W
+from kernel_runs import run, seed_state
+
+MESHES = {
+    # the reference's fvm unit-test mesh: 5^3 StructGen over [-1,1]^3, one DEFAULT region
+    "ref5": dict(nx=5, bounds=(-1, 1, -1, 1, -1, 1)),
+    # six typed regions, relabelled cells and faces, non-zero xpac/xnac/rlencos
+    "typed_perm": dict(nx=7, ny=6, nz=5, typed=True, permute_cells=True, permute_faces=True, face_extras=True,
+                       seed=11),
+    "typed32": dict(nx=32, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=3),
+    "tets_typed": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, typed=True, permute_cells=True,
+                       permute_faces=True, face_extras=True, seed=5),
+    # 24576 jittered tets: many halo faces per tile
+    "tets16": dict(nx=16, tets=True, jitter=0.25, typed=True, permute_cells=True, permute_faces=True, seed=8),
+    # a slab one cell thick: every cell owns boundary faces
+    "slab": dict(nx=40, ny=3, nz=1, permute_faces=True, seed=2),
+    # one cell: no interior faces, no Morton sort
+    "one_cell": dict(nx=1, ny=1, nz=1, typed=True),
+    # a line of cells whose tiles have no halo faces on two axes
+    "line": dict(nx=200, ny=1, nz=1, typed=True, permute_cells=True, seed=4),
+}
+
+N_FIELDS = 49
+
+
+def _differ(a, b):
+    return [k for k in a if a[k] != b[k]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("reorder", [True, False])
+@pytest.mark.parametrize("meshname", list(MESHES))
+def test_device_build_matches_host(meshname, reorder, precision):
+    host = ch.HostMesh(**MESHES[meshname])
+    a = ch.DeviceMesh(host, precision, reorder=reorder)
+    b = ch.DeviceMesh(host, precision, reorder=reorder, on_device=True)
+    da, db = a.digest(), b.digest()
+    assert len(da) == N_FIELDS and list(da) == list(db)
+    assert _differ(da, db) == []
+    assert np.array_equal(a.cell_order, b.cell_order) and np.array_equal(a.face_order, b.face_order)
+
+
+@pytest.mark.gpu
+def test_device_build_matches_host_64cubed():
+    host = ch.HostMesh(64, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=4)
+    da = ch.DeviceMesh(host, 64).digest()
+    db = ch.DeviceMesh(host, 64, on_device=True).digest()
+    assert _differ(da, db) == []
+
+
+@pytest.mark.gpu
+def test_digest_sees_a_different_mesh():
+    """The digest is not blind: another relabelling changes the orders and the CSR."""
+    a = ch.DeviceMesh(ch.HostMesh(6, permute_cells=True, permute_faces=True, seed=1), 64, on_device=True).digest()
+    b = ch.DeviceMesh(ch.HostMesh(6, permute_cells=True, permute_faces=True, seed=2), 64, on_device=True).digest()
+    d = set(_differ(a, b))
+    assert {"cell_order", "face_order", "ent", "fconn"} <= d
+    assert "counts" not in d and "rtype" not in d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["gradient", "mass", "scalar", "uvw"])
+def test_kernels_agree_on_both_builds(kernel):
+    host = ch.HostMesh(**MESHES["typed_perm"])
+    outs = []
+    for on_device in (False, True):
+        dev = ch.DeviceMesh(host, 64, on_device=on_device)
+        host.set_state(seed_state(host))
+        rc, out = run(kernel, fvm, dev, W.Fields(host, seed=21).copy(), lambda: host.state(64))
+        assert rc == ch.E_SUCCESS
+        outs.append(out)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def _create(view, on_device, flags=ch.MESH_REORDER):
+    L = ch.lib()
+    h = ct.c_void_p()
+    fn = L.cupcfd_hip_mesh_create_device_f64 if on_device else L.cupcfd_hip_mesh_create_f64
+    rc = fn(ct.byref(view), flags, ct.byref(h))
+    return rc, h
+
+
+def _digest(h):
+    L = ch.lib()
+    out = np.zeros(64, np.uint64)
+    n = ct.c_int32()
+    assert L.cupcfd_hip_mesh_digest(h, out.ctypes.data_as(ct.c_void_p), out.size, ct.byref(n)) == ch.E_SUCCESS
+    return {L.cupcfd_hip_mesh_digest_field(i).decode(): int(out[i]) for i in range(n.value)}
+
+
+@pytest.mark.gpu
+def test_device_build_strided_view():
+    """3-vectors laid out as the reference's EuclideanVector<double,3> (32-byte stride)."""
+    host = ch.HostMesh(**MESHES["typed_perm"])
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    keep = []
+    for name in ("cellCenter", "faceNorm", "faceCenter", "faceXpac", "faceXnac"):
+        a = host.a[name]
+        s = np.full((a.shape[0], 4), np.nan)
+        s[:, 1:] = a
+        keep.append(s)
+        setattr(v, name, ch.V3F64(s.ctypes.data + 8, 32))
+    L = ch.lib()
+    rc_p, hp = _create(host.view64, False)
+    rc_s, hs = _create(v, True)
+    try:
+        assert rc_p == ch.E_SUCCESS and rc_s == ch.E_SUCCESS
+        assert _differ(_digest(hp), _digest(hs)) == []
+    finally:
+        L.cupcfd_hip_mesh_destroy(hp)
+        L.cupcfd_hip_mesh_destroy(hs)
+
+
+@pytest.mark.gpu
+def test_device_build_rejects_bad_views():
+    """The same eCodes as the host repack, and no device mesh."""
+    host = ch.HostMesh(3, typed=True)
+    L = ch.lib()
+    cases = []
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    bad = host.a["faceCell1"].copy()
+    bad[5] = host.nCells
+    v.faceCell1ID = bad.ctypes.data
+    cases.append((v, ch.E_INVALID_INDEX, bad))
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    bad2 = host.a["faceCell2"].copy()
+    bad2[np.flatnonzero(host.a["faceBnd"] < 0)[3]] = -2
+    v.faceCell2ID = bad2.ctypes.data
+    cases.append((v, ch.E_INVALID_INDEX, bad2))
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    badb = host.a["bndRegion"].copy()
+    badb[-1] = host.nRegions
+    v.boundaryRegionID = badb.ctypes.data
+    cases.append((v, ch.E_INVALID_INDEX, badb))
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    badr = host.a["regType"].copy()
+    badr[0] = 7
+    v.regionType = badr.ctypes.data
+    cases.append((v, 90, badr))
+    v = ch.MeshViewF64.from_buffer_copy(host.view64)
+    v.cellVolume = None
+    cases.append((v, ch.E_NULL_PTR, None))
+    for v, code, _ in cases:
+        for on_device in (False, True):
+            rc, h = _create(v, on_device)
+            assert rc == code, (on_device, rc, code)
+            assert not h.value
+    assert L.cupcfd_hip_mesh_create_device_f64(None, 0, None) == ch.E_NULL_PTR
