@@ -479,10 +479,20 @@ def bench_tets(torch, n, K, steps, warmup):
     import cupcfd_hip as ch
     from cupcfd_hip import workload as W
 
+    params = dict(seed=606, tets=True, jitter=0.2, permute_cells=True, permute_faces=True)
     t0 = time.time()
-    host = ch.HostMesh(n, seed=606, tets=True, jitter=0.2, permute_cells=True, permute_faces=True)
+    host = ch.HostMesh(n, **params)
+    t_gen = time.time() - t0
     dev = ch.DeviceMesh(host, 64, reorder=True)
     t_mesh = time.time() - t0
+    # the same mesh generated and repacked on the GPU
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, dgen = ch.generate_mesh(n, host_copy=False, **params)
+    t_dev = time.perf_counter() - t1
+    mesh_build = {"host_generate_s": t_gen, "host_repack_s": t_mesh - t_gen, "device_generate_s": t_dev,
+                  "device_generate_identical": dgen.digest() == dev.digest(), "speedup_setup": t_mesh / t_dev}
+    dgen = None
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
     f = W.Fields(host, seed=808)
     kernels = make_sweep(torch, host, dev, f)
@@ -492,7 +502,7 @@ def bench_tets(torch, n, K, steps, warmup):
     label = f"{n}^3 hexes x 6 jittered tets relabelled"
     out = {"workload": f"plain FP64 gradient+mass+scalar+UVW sweep, {label} ({C} cells, {F} faces), one DEFAULT "
                        f"region", "cells": C, "faces": F, "boundaries": B, "mesh_setup_s": t_mesh,
-           "gcell_sweeps_per_s": C / step_s / 1e9, "ms_per_step": step_s * 1e3,
+           "mesh_build": mesh_build, "gcell_sweeps_per_s": C / step_s / 1e9, "ms_per_step": step_s * 1e3,
            "kernels": kernel_report(C, F, F - B, B, per)}
     out["batched"] = bench_batched(torch, host, dev, K, steps=max(2, min(steps, 5)), warmup=1, label=label)
     torch.cuda.empty_cache()
@@ -501,24 +511,46 @@ def bench_tets(torch, n, K, steps, warmup):
     return out
 
 
-def bench_mesh_build(torch, host, dev_host, t_host_repack, label, reps=3):
-    """Host repack (cupcfd_hip_mesh_create_f64) against the GPU repack of the same view
-    (cupcfd_hip_mesh_create_device_f64): wall time of each create call, uploads and
-    host-side mirrors included, and whether the two device meshes are identical."""
+def bench_mesh_build(torch, host, dev_host, t_host_gen, t_host_repack, label, reps=3):
+    """Mesh setup on the host against the GPU, for the same mesh:
+      - host generator (cupcfd_structgen_create) + host repack (cupcfd_hip_mesh_create_f64);
+      - GPU repack of the host mesh's view (cupcfd_hip_mesh_create_device_f64);
+      - GPU generator + GPU repack (cupcfd_hip_mesh_generate_f64), with and without a host copy.
+    Wall time of each call (uploads, downloads and host-side mirrors included, best of `reps`), and
+    whether each device mesh is identical to the host-built one (cupcfd_hip_mesh_digest)."""
     import cupcfd_hip as ch
 
-    times = []
+    ref = dev_host.digest()
+    params = host.params
+    dims = dict(nx=params["nx"], ny=params["ny"], nz=params["nz"])
+    rest = {k: v for k, v in params.items() if k not in dims}
+
+    def best(make):
+        times, obj = [], None
+        for _ in range(reps):
+            obj = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            obj = make()
+            times.append(time.perf_counter() - t0)
+        return min(times), obj
+
+    t_rep, dev = best(lambda: ch.DeviceMesh(host, 64, reorder=True, on_device=True))
+    same_rep = dev.digest() == ref
     dev = None
-    for _ in range(reps):
-        dev = None
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        dev = ch.DeviceMesh(host, 64, reorder=True, on_device=True)
-        times.append(time.perf_counter() - t0)
-    same = dev.digest() == dev_host.digest()
-    return {"mesh": label, "cells": host.nCells, "faces": host.nFaces, "host_repack_s": t_host_repack,
-            "device_repack_s": min(times), "device_repack_first_s": times[0],
-            "speedup": t_host_repack / min(times), "identical": same}
+    t_gen, (_, dev) = best(lambda: ch.generate_mesh(host_copy=False, **dims, **rest))
+    same_gen = dev.digest() == ref
+    dev = None
+    t_gen_h, (hcopy, dev) = best(lambda: ch.generate_mesh(host_copy=True, **dims, **rest))
+    same_host = all(hcopy.a[k].tobytes() == host.a[k].tobytes() for k in host.a)
+    dev = hcopy = None
+    return {"mesh": label, "cells": host.nCells, "faces": host.nFaces,
+            "host_generate_s": t_host_gen, "host_repack_s": t_host_repack,
+            "device_repack_s": t_rep, "device_repack_identical": same_rep,
+            "device_generate_s": t_gen, "device_generate_identical": same_gen,
+            "device_generate_with_host_copy_s": t_gen_h, "host_copy_identical": same_host,
+            "speedup_repack": t_host_repack / t_rep,
+            "speedup_setup": (t_host_gen + t_host_repack) / t_gen}
 
 
 def main():
@@ -624,7 +656,7 @@ def main():
         "mesh_setup_s": t_mesh,
     }
     if not args.no_mesh_build:
-        out["mesh_build"] = bench_mesh_build(torch, host, dev, t_host_repack, f"StructGen {n}^3 relabelled")
+        out["mesh_build"] = bench_mesh_build(torch, host, dev, t_gen, t_host_repack, f"StructGen {n}^3 relabelled")
     if not args.no_batched:
         hb, db, lb = structgen(args.batch_n, 4242)
         out["batched"] = bench_batched(torch, hb, db, args.batch_k, steps=max(2, min(args.steps, 10)), warmup=1,

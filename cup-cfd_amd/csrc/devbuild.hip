@@ -66,7 +66,21 @@ inline unsigned bits_for(int64_t x) {  // bits that hold every value in [0, x]
 #define GRID_STRIDE(i, n) \
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)(n); i += (int64_t)gridDim.x * blockDim.x)
 
-__global__ void k_validate_faces(int F, int C, int B, const int *c1, const int *c2, const int *fb, int *stat) {
+// Block-level counters: threads combine in LDS, then one global atomic per
+// block and counter (a global atomic per thread serialises millions of
+// updates on one address).
+struct BlockCounters {
+    int *s;
+    __device__ explicit BlockCounters(int *lds, int n) : s(lds) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = 0;
+        __syncthreads();
+    }
+};
+
+__global__ void __launch_bounds__(kB) k_validate_faces(int F, int C, int B, const int *c1, const int *c2,
+                                                       const int *fb, int *stat) {
+    __shared__ int sc[2];
+    BlockCounters bc(sc, 2);
     int fi = 0, bad = 0;
     GRID_STRIDE(f, F) {
         const int a = c1[f], b = fb[f];
@@ -79,8 +93,13 @@ __global__ void k_validate_faces(int F, int C, int B, const int *c1, const int *
             else fi++;
         }
     }
-    if (bad) atomicOr(&stat[S_ERR], 1);
-    if (fi) atomicAdd(&stat[S_FI], fi);
+    if (bad) atomicOr(&sc[0], 1);
+    if (fi) atomicAdd(&sc[1], fi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (sc[0]) atomicOr(&stat[S_ERR], 1);
+        if (sc[1]) atomicAdd(&stat[S_FI], sc[1]);
+    }
 }
 
 __global__ void k_validate_bnds(int B, int F, int R, const int *bf, const int *br, int *stat) {
@@ -248,34 +267,54 @@ __global__ void k_csr_ent(int nnz, const int *slot, const int *frank, const int 
     }
 }
 
-__global__ void k_row_stats(int C, const int *row, int *stat) {
+__global__ void __launch_bounds__(kB) k_row_stats(int C, const int *row, int *stat) {
+    __shared__ int sc[2];
+    if (threadIdx.x == 0) sc[0] = 0, sc[1] = INT_MAX;
+    __syncthreads();
     int mx = 0, mn = INT_MAX;
     GRID_STRIDE(c, C) {
         const int l = row[c + 1] - row[c];
         mx = l > mx ? l : mx;
         mn = l < mn ? l : mn;
     }
-    atomicMax(&stat[S_MAXROW], mx);
-    atomicMin(&stat[S_MINROW], mn);
+    atomicMax(&sc[0], mx);
+    atomicMin(&sc[1], mn);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(&stat[S_MAXROW], sc[0]);
+        atomicMin(&stat[S_MINROW], sc[1]);
+    }
 }
 
-__global__ void k_bnd(int B, const int *bf, const int *br, const int *rtype, const int *frank, const int *c1,
-                      const int *crank, int *bface, int *bcell, unsigned *tkey, int *val, int *stat) {
+__device__ inline void flush_types(const int *sc, int *dst) {
+    __syncthreads();
+    if (threadIdx.x < 5 && sc[threadIdx.x]) atomicAdd(&dst[threadIdx.x], sc[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(kB) k_bnd(int B, const int *bf, const int *br, const int *rtype, const int *frank,
+                                            const int *c1, const int *crank, int *bface, int *bcell, unsigned *tkey,
+                                            int *val, int *stat) {
+    __shared__ int sc[5];
+    BlockCounters bc(sc, 5);
     GRID_STRIDE(b, B) {
         const int f = bf[b], t = rtype[br[b]];
         bface[b] = frank[f];
         bcell[b] = crank[c1[f]];
         tkey[b] = (unsigned)t;
         val[b] = (int)b;
-        atomicAdd(&stat[S_BTYPE + t], 1);
+        atomicAdd(&sc[t], 1);
     }
+    flush_types(sc, stat + S_BTYPE);
 }
 
-__global__ void k_ntype(int F, const int *fb, const int *br, const int *rtype, int *stat) {
+__global__ void __launch_bounds__(kB) k_ntype(int F, const int *fb, const int *br, const int *rtype, int *stat) {
+    __shared__ int sc[5];
+    BlockCounters bc(sc, 5);
     GRID_STRIDE(f, F) {
         const int b = fb[f];
-        if (b > -1) atomicAdd(&stat[S_NTYPE + rtype[br[b]]], 1);
+        if (b > -1) atomicAdd(&sc[rtype[br[b]]], 1);
     }
+    flush_types(sc, stat + S_NTYPE);
 }
 
 __global__ void k_owner_flags(int n, const int2 *fconn, unsigned char *flag) {
@@ -400,13 +439,12 @@ __global__ void k_eslot(int nnz, int TC, int Fi, const unsigned *ecell, const in
 // are freed when the build ends. The first HIP error sticks.
 struct Work {
     std::vector<void *> &keep;
-    std::vector<void *> tmp;
+    DeviceChunks tmp;
     void *prim = nullptr;
     size_t prim_bytes = 0;
     hipError_t err = hipSuccess;
     explicit Work(std::vector<void *> &k) : keep(k) {}
     ~Work() {
-        for (void *p : tmp) (void)hipFree(p);
         if (prim) (void)hipFree(prim);
     }
     void note(hipError_t e) {
@@ -416,14 +454,27 @@ struct Work {
         if (err != hipSuccess) return nullptr;
         if (n == 0) n = 1;  // keep every pointer valid
         void *p = nullptr;
-        note(hipMalloc(&p, n * sizeof(X)));
-        if (err != hipSuccess) return nullptr;
-        (persistent ? keep : tmp).push_back(p);
-        return (X *)p;
+        if (persistent) {
+            note(hipMalloc(&p, n * sizeof(X)));
+            if (err == hipSuccess) keep.push_back(p);
+        } else {
+            note(tmp.get(n * sizeof(X), &p));
+        }
+        return err == hipSuccess ? (X *)p : nullptr;
     }
     template <class X> X *upload(const X *src, size_t n, bool persistent) {
         X *d = alloc<X>(n, persistent);
         if (d && src && n) note(hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice));
+        return d;
+    }
+    // an input array of the view: read in place when the view is device-resident
+    // (copied only when the mesh keeps it), else uploaded
+    bool resident = false;
+    template <class X> X *input(const X *src, size_t n, bool persistent) {
+        if (!resident) return upload(src, n, persistent);
+        if (!persistent) return const_cast<X *>(src);
+        X *d = alloc<X>(n, true);
+        if (d && n) note(hipMemcpyAsync(d, src, n * sizeof(X), hipMemcpyDeviceToDevice, 0));
         return d;
     }
     template <class X> void download(std::vector<X> &h, const X *d, size_t n) {
@@ -462,9 +513,10 @@ template <class V3> const auto *v3at(const V3 &v, int64_t i, int k) {
     return (const S *)((const char *)v.base + i * stride) + k;
 }
 
-// a 3-vector array uploaded packed xyz
+// a 3-vector array, packed xyz on the device (a device-resident view is packed)
 template <class T, class V3> T *upload3(Work &W, const V3 &v, int64_t n, bool persistent) {
-    if (v.stride == 0 || v.stride == (int64_t)(3 * sizeof(T))) return W.upload((const T *)v.base, 3 * n, persistent);
+    if (v.stride == 0 || v.stride == (int64_t)(3 * sizeof(T)) || W.resident)
+        return W.input((const T *)v.base, 3 * n, persistent);
     std::vector<T> h(3 * (size_t)n);
     for (int64_t i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) h[3 * i + k] = *v3at(v, i, k);
@@ -473,19 +525,22 @@ template <class T, class V3> T *upload3(Work &W, const V3 &v, int64_t n, bool pe
 
 }  // namespace
 
-template <class T, class View> int build_device(Mesh<T> &M, const View &v, int flags) {
+template <class T, class View> int build_device(Mesh<T> &M, const View &v, int flags, bool resident) {
     if (int rc = view_check(v); rc != CUPCFD_E_SUCCESS) return rc;
     const int C = v.nCells, F = v.nFaces, B = v.nBoundaries, R = v.nRegions;
     const int TC = kTileCells;
     Work W(M.allocs);
+    W.resident = resident;
+    // the temporaries take about this much: uploads (host views), sort buffers, marks
+    W.tmp.min_chunk = std::max<size_t>(size_t(1) << 20, 96 * ((size_t)F + C + B));
     int st[S_COUNT];
 
     // ---- validation: every index in range, then region types --------------
-    int *c1 = W.upload(v.faceCell1ID, F, false);
-    int *c2 = W.upload(v.faceCell2ID, F, false);
-    int *fb = W.upload(v.faceBoundaryID, F, false);
-    int *bf = W.upload(v.boundaryFaceID, B, false);
-    int *br = W.upload(v.boundaryRegionID, B, true);
+    int *c1 = W.input(v.faceCell1ID, F, false);
+    int *c2 = W.input(v.faceCell2ID, F, false);
+    int *fb = W.input(v.faceBoundaryID, F, false);
+    int *bf = W.input(v.boundaryFaceID, B, false);
+    int *br = W.input(v.boundaryRegionID, B, true);
     int *stat;
     {
         int init[S_COUNT] = {};
@@ -505,7 +560,7 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
 
     // ---- cell order -------------------------------------------------------
     T *cc = upload3<T>(W, v.cellCenter, C, false);
-    T *vol = W.upload(v.cellVolume, C, false);
+    T *vol = W.input(v.cellVolume, C, false);
     int *corder = W.alloc<int>(C, false), *crank = W.alloc<int>(C, false);
     if ((flags & CUPCFD_MESH_REORDER) && C > 1) {
         const int nb = std::min(grid_for(C), 1024);
@@ -544,8 +599,8 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
     V2<T> *fN = W.alloc<V2<T>>(F, true);
     T *frl = W.alloc<T>(F, true);
     {
-        T *lam = W.upload(v.faceLambda, F, false), *rl = W.upload(v.faceRLencos, F, false);
-        T *area = W.upload(v.faceArea, F, false);
+        T *lam = W.input(v.faceLambda, F, false), *rl = W.input(v.faceRLencos, F, false);
+        T *area = W.input(v.faceArea, F, false);
         T *nrm = upload3<T>(W, v.faceNorm, F, false), *ctr = upload3<T>(W, v.faceCenter, F, false);
         T *xp = upload3<T>(W, v.faceXpac, F, false), *xn = upload3<T>(W, v.faceXnac, F, false);
         W.launch(F, k_face_streams<T>, F, (const int *)forder, (const int *)c1, (const int *)c2, (const int *)fb,
@@ -588,7 +643,7 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
             return rocprim::radix_sort_pairs(tmp, bytes, tkey, tskey, tval, blist, (size_t)B, 0, 3, (hipStream_t)0);
         });
     W.launch(F, k_ntype, F, (const int *)fb, (const int *)br, (const int *)rtype, stat);
-    T *bdist = W.upload(v.boundaryDistance, B, true);
+    T *bdist = W.input(v.boundaryDistance, B, true);
 
     // cells owning a boundary face, ascending
     unsigned char *has = W.alloc<unsigned char>(C, false);
@@ -717,7 +772,9 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
     if (W.err == hipSuccess) W.note(hipMemcpy(st, stat, sizeof(st), hipMemcpyDeviceToHost));
     if (W.err != hipSuccess) return CUPCFD_E_ERROR;
     for (int t = 0; t < 5; t++) M.ntype[t] = st[S_NTYPE + t];
-    M.bregion_h.assign(v.boundaryRegionID, v.boundaryRegionID + B);
+    if (resident) W.download(M.bregion_h, (const int32_t *)br, B);
+    else M.bregion_h.assign(v.boundaryRegionID, v.boundaryRegionID + B);
+    if (W.err != hipSuccess) return CUPCFD_E_ERROR;
 
     // ---- region arrays (a handful of entries: built on the host) -------------
     std::vector<uint8_t> rstd(R), rflux(R), radiab(R);
@@ -810,7 +867,7 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
     return W.err == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
 }
 
-template int build_device<double, cupcfd_mesh_view_f64>(Mesh<double> &, const cupcfd_mesh_view_f64 &, int);
-template int build_device<float, cupcfd_mesh_view_f32>(Mesh<float> &, const cupcfd_mesh_view_f32 &, int);
+template int build_device<double, cupcfd_mesh_view_f64>(Mesh<double> &, const cupcfd_mesh_view_f64 &, int, bool);
+template int build_device<float, cupcfd_mesh_view_f32>(Mesh<float> &, const cupcfd_mesh_view_f32 &, int, bool);
 
 }  // namespace cupcfd_hip

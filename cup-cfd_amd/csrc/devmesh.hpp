@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -139,9 +140,41 @@ template <class View> int view_check(const View &v) {
     return CUPCFD_E_SUCCESS;
 }
 
+// Device scratch carved from a few large allocations (256-byte aligned), all
+// freed together: a build's many temporaries cost a handful of hipMalloc /
+// hipFree calls instead of one pair each.
+struct DeviceChunks {
+    std::vector<void *> bufs;
+    char *cur = nullptr;
+    size_t left = 0, min_chunk = size_t(1) << 20;
+    DeviceChunks() = default;
+    DeviceChunks(const DeviceChunks &) = delete;
+    DeviceChunks &operator=(const DeviceChunks &) = delete;
+    ~DeviceChunks() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+    hipError_t get(size_t bytes, void **out) {
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        if (bytes > left) {
+            const size_t sz = std::max(bytes, min_chunk);
+            void *p = nullptr;
+            if (hipError_t e = hipMalloc(&p, sz); e != hipSuccess) return e;
+            bufs.push_back(p);
+            cur = (char *)p;
+            left = sz;
+        }
+        *out = cur;
+        cur += bytes;
+        left -= bytes;
+        return hipSuccess;
+    }
+};
+
 // The same repack as the host build, run on the device: every array of the
-// DevMesh and every host-side mirror of Mesh comes out bit-identical.
-template <class T, class View> int build_device(Mesh<T> &M, const View &v, int flags);
+// DevMesh and every host-side mirror of Mesh comes out bit-identical. With
+// `resident` the view's per-cell, per-face and per-boundary arrays are device
+// pointers (3-vectors packed) and its region arrays host pointers.
+template <class T, class View> int build_device(Mesh<T> &M, const View &v, int flags, bool resident = false);
 
 }  // namespace cupcfd_hip
 

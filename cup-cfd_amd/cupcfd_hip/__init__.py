@@ -113,7 +113,8 @@ HOST_SYMBOLS = [
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
-    "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field",
+    "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field", "cupcfd_hip_mesh_generate_f64",
+    "cupcfd_hip_mesh_generate_f32",
     "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
@@ -154,6 +155,9 @@ def _declare(L):
     L.cupcfd_hip_mesh_create_f32.argtypes = [ct.POINTER(MeshViewF32), ct.c_int32, ct.POINTER(ct.c_void_p)]
     L.cupcfd_hip_mesh_create_device_f64.argtypes = [ct.POINTER(MeshViewF64), ct.c_int32, ct.POINTER(ct.c_void_p)]
     L.cupcfd_hip_mesh_create_device_f32.argtypes = [ct.POINTER(MeshViewF32), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    for k in ("f64", "f32"):
+        getattr(L, f"cupcfd_hip_mesh_generate_{k}").argtypes = [ct.POINTER(StructGenParams), ct.c_int32,
+                                                               ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p)]
     L.cupcfd_hip_mesh_digest.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_int32)]
     L.cupcfd_hip_mesh_digest_field.argtypes = [ct.c_int32]
     L.cupcfd_hip_mesh_digest_field.restype = ct.c_char_p
@@ -204,19 +208,30 @@ def _np(ptr, n, dtype, shape=None, owner=None):
     return a.reshape(shape) if shape else a
 
 
+def structgen_params(nx, ny=None, nz=None, bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), seed=0, permute_cells=False,
+                     permute_faces=False, typed=False, face_extras=False, tets=False, jitter=0.0):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return StructGenParams(nx, ny, nz, *[float(b) for b in bounds], seed, int(permute_cells), int(permute_faces),
+                           int(typed), int(face_extras), int(tets), float(jitter))
+
+
 class HostMesh:
     """A StructGen-numbered mesh held by the library (cupcfd_structgen_create)."""
 
     def __init__(self, nx, ny=None, nz=None, bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), seed=0,
                  permute_cells=False, permute_faces=False, typed=False, face_extras=False, tets=False,
-                 jitter=0.0):
+                 jitter=0.0, _handle=None):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
-        p = StructGenParams(nx, ny, nz, *[float(b) for b in bounds], seed, int(permute_cells), int(permute_faces),
-                            int(typed), int(face_extras), int(tets), float(jitter))
-        self._h = ct.c_void_p()
         L = lib()
-        check("cupcfd_structgen_create", L.cupcfd_structgen_create(ct.byref(p), ct.byref(self._h)))
+        if _handle is None:
+            p = structgen_params(nx, ny, nz, bounds, seed, permute_cells, permute_faces, typed, face_extras, tets,
+                                 jitter)
+            self._h = ct.c_void_p()
+            check("cupcfd_structgen_create", L.cupcfd_structgen_create(ct.byref(p), ct.byref(self._h)))
+        else:  # a host mesh the library made (cupcfd_hip_mesh_generate_*)
+            self._h = _handle
         self.params = dict(nx=nx, ny=ny, nz=nz, bounds=bounds, seed=seed, permute_cells=permute_cells,
                            permute_faces=permute_faces, typed=typed, face_extras=face_extras, tets=tets,
                            jitter=jitter)
@@ -323,7 +338,7 @@ class DeviceMesh:
     """A mesh repacked into HBM (cupcfd_hip_mesh_create_f64 / _f32, or with ``on_device`` the GPU repack
     cupcfd_hip_mesh_create_device_f64 / _f32, which builds the same arrays)."""
 
-    def __init__(self, host: HostMesh, precision=64, reorder=True, on_device=False):
+    def __init__(self, host: HostMesh, precision=64, reorder=True, on_device=False, _handle=None):
         self.host = host
         self.precision = precision
         self.T = ct.c_double if precision == 64 else ct.c_float
@@ -333,9 +348,12 @@ class DeviceMesh:
         self._h = ct.c_void_p()
         L = lib()
         flags = MESH_REORDER if reorder else 0
-        fn = f"cupcfd_hip_mesh_create{'_device' if on_device else ''}_{self.sfx}"
-        view = host.view64 if precision == 64 else host.view32()
-        check(fn, getattr(L, fn)(ct.byref(view), flags, ct.byref(self._h)))
+        if _handle is not None:  # a device mesh the library made (cupcfd_hip_mesh_generate_*)
+            self._h = _handle
+        else:
+            fn = f"cupcfd_hip_mesh_create{'_device' if on_device else ''}_{self.sfx}"
+            view = host.view64 if precision == 64 else host.view32()
+            check(fn, getattr(L, fn)(ct.byref(view), flags, ct.byref(self._h)))
         c = [ct.c_int32() for _ in range(4)]
         check("cupcfd_hip_mesh_counts", L.cupcfd_hip_mesh_counts(self._h, *[ct.byref(x) for x in c]))
         self.nCells, self.nFaces, self.nBoundaries, self.nRegions = [x.value for x in c]
@@ -400,6 +418,23 @@ class DeviceMesh:
 
 def version():
     return lib().cupcfd_hip_version().decode()
+
+
+def generate_mesh(nx, ny=None, nz=None, precision=64, reorder=True, host_copy=True, **params):
+    """Generate the mesh HostMesh(nx, ny, nz, **params) describes on the GPU and repack it there
+    (cupcfd_hip_mesh_generate_f64 / _f32). Returns (HostMesh or None, DeviceMesh); the host copy,
+    when asked for, equals HostMesh(nx, ny, nz, **params) and the device mesh equals DeviceMesh of it."""
+    L = lib()
+    p = structgen_params(nx, ny, nz, **params)
+    hh, dh = ct.c_void_p(), ct.c_void_p()
+    fn = f"cupcfd_hip_mesh_generate_{'f64' if precision == 64 else 'f32'}"
+    check(fn, getattr(L, fn)(ct.byref(p), MESH_REORDER if reorder else 0, ct.byref(hh) if host_copy else None,
+                             ct.byref(dh)))
+    host = HostMesh(nx, ny, nz, _handle=hh, **params) if host_copy else None
+    dev = DeviceMesh(host, precision, reorder=reorder, _handle=dh)
+    if host is not None:
+        dev._keep_host = host  # the device mesh mirrors state through the host copy's view
+    return host, dev
 
 
 from .fvm import *  # noqa: E402,F401,F403  (kernel wrappers)

@@ -163,6 +163,22 @@ int cupcfd_hip_mesh_create_device_f32(const cupcfd_mesh_view_f32 *view, int32_t 
  * Synchronous; no counterpart in the reference. */
 int cupcfd_hip_mesh_digest(const cupcfd_hip_mesh *mesh, uint64_t *digest, int32_t capacity, int32_t *count);
 const char *cupcfd_hip_mesh_digest_field(int32_t i);
+/* Generate the mesh cupcfd_structgen_create(params) describes on the GPU and
+ * repack it there: no per-cell or per-face array crosses PCIe (the
+ * relabelling's two permutations are drawn on the host). The device mesh is
+ * bit-identical to cupcfd_hip_mesh_create_*(view of cupcfd_structgen_create's
+ * mesh, flags), and the errors are the generator's. host_out (may be NULL)
+ * receives a host copy equal to cupcfd_structgen_create's mesh; the device
+ * mesh then keeps that copy's view, as cupcfd_hip_mesh_create_* keeps the
+ * caller's. With host_out NULL the mesh keeps no view: its state starts as
+ * the generator leaves it and stays on the device (push/pull state do
+ * nothing). The caller destroys both objects, the device mesh first. It
+ * replaces MeshStructGenSource plus the finalize-time map building of
+ * CupCfdSoAMesh (MeshStructGenSource.cpp:33-1259, CupCfdSoAMesh.cpp:549-668). */
+int cupcfd_hip_mesh_generate_f64(const cupcfd_structgen_params *params, int32_t flags, cupcfd_host_mesh **host_out,
+                                 cupcfd_hip_mesh **out);
+int cupcfd_hip_mesh_generate_f32(const cupcfd_structgen_params *params, int32_t flags, cupcfd_host_mesh **host_out,
+                                 cupcfd_hip_mesh **out);
 int cupcfd_hip_mesh_destroy(cupcfd_hip_mesh *mesh);
 int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t *nFaces,
                            int32_t *nBoundaries, int32_t *nRegions);

@@ -163,3 +163,71 @@ def test_device_build_rejects_bad_views():
             assert rc == code, (on_device, rc, code)
             assert not h.value
     assert L.cupcfd_hip_mesh_create_device_f64(None, 0, None) == ch.E_NULL_PTR
+
+
+# ---- meshes generated on the GPU (cupcfd_hip_mesh_generate_*) -------------------
+GEN = dict(MESHES)
+GEN.update({
+    # uneven spacing and bounds on every axis, typed, extras, faces relabelled
+    "aniso": dict(nx=9, ny=4, nz=6, bounds=(0.0, 2.5, -1.0, 0.3, 0.1, 7.0), typed=True, face_extras=True,
+                  permute_faces=True, seed=99),
+    # strongly jittered tets on an uneven box, cells relabelled
+    "tets_aniso": dict(nx=3, ny=5, nz=2, tets=True, jitter=0.45, bounds=(-2.0, 1.0, 0.0, 0.5, 3.0, 4.0), typed=True,
+                       face_extras=True, permute_cells=True, seed=17),
+})
+
+
+def _same_bytes(a, b):
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("meshname", list(GEN))
+def test_generated_mesh_matches_host_generator(meshname, precision):
+    params = GEN[meshname]
+    ref = ch.HostMesh(**params)
+    host, dev = ch.generate_mesh(precision=precision, **params)
+    bad = [k for k in ref.a if not _same_bytes(ref.a[k], host.a[k])]
+    assert bad == []
+    dref = ch.DeviceMesh(ref, precision).digest()
+    assert _differ(dev.digest(), dref) == []
+    _, dev2 = ch.generate_mesh(precision=precision, host_copy=False, **params)
+    assert _differ(dev2.digest(), dref) == []
+
+
+@pytest.mark.gpu
+def test_generated_mesh_matches_host_generator_96cubed():
+    params = dict(nx=96, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=23)
+    ref = ch.HostMesh(**params)
+    host, dev = ch.generate_mesh(**params)
+    assert [k for k in ref.a if not _same_bytes(ref.a[k], host.a[k])] == []
+    assert _differ(dev.digest(), ch.DeviceMesh(ref, 64).digest()) == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["gradient", "scalar", "uvw"])
+def test_kernels_on_generated_mesh_without_host_copy(kernel):
+    """A generated mesh with no host copy starts from the generator's state and runs the drop-in
+    kernels bit-identically to a mesh built from the host generator's mesh."""
+    params = GEN["tets_typed"]
+    ref = ch.HostMesh(**params)
+    outs = []
+    for dev in (ch.DeviceMesh(ref, 64), ch.generate_mesh(host_copy=False, **params)[1]):
+        rc, out = run(kernel, fvm, dev, W.Fields(ref, seed=8).copy(), lambda: {})
+        assert rc == ch.E_SUCCESS
+        outs.append(out)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.gpu
+def test_generate_rejects_bad_params():
+    L = ch.lib()
+    for kw, code in ((dict(nx=0), 90), (dict(nx=2, ny=-1), 90), (dict(nx=2, tets=True, jitter=0.5), 90),
+                     (dict(nx=2, tets=True, jitter=-0.1), 90)):
+        p = ch.structgen_params(**kw)
+        hh, dh = ct.c_void_p(), ct.c_void_p()
+        assert L.cupcfd_hip_mesh_generate_f64(ct.byref(p), 1, ct.byref(hh), ct.byref(dh)) == code, kw
+        assert not hh.value and not dh.value
+    assert L.cupcfd_hip_mesh_generate_f64(None, 1, None, None) == ch.E_NULL_PTR
