@@ -325,13 +325,14 @@ template <class T> __device__ __forceinline__ void stage1(T *l, const T *p, int 
     const int i = threadIdx.x;
     if (i < nc) l[i] = p[c0 + i];
 }
-template <class T> __device__ __forceinline__ void stage3(T *l, const T *p, int c0, int nc) {
+template <class T> __device__ __forceinline__ void stage3(T *l, const T *p, int c0, int nc, int ld = kTileCells) {
     const int i = threadIdx.x;
-    if (i < 3 * nc) l[(i % 3) * kTileCells + i / 3] = p[3 * (size_t)c0 + i];
+    if (i < 3 * nc) l[(i % 3) * ld + i / 3] = p[3 * (size_t)c0 + i];
 }
-template <class T> __device__ __forceinline__ void stage_centre(T *l, const V4<T> *g, int c0, int nc) {
+template <class T>
+__device__ __forceinline__ void stage_centre(T *l, const V4<T> *g, int c0, int nc, int ld = kTileCells) {
     const int i = threadIdx.x;
-    if (i < 4 * nc && (i & 3) != 3) l[(i & 3) * kTileCells + (i >> 2)] = ((const T *)(g + c0))[i];
+    if (i < 4 * nc && (i & 3) != 3) l[(i & 3) * ld + (i >> 2)] = ((const T *)(g + c0))[i];
 }
 // the CSR slots of the tile's rows into LDS; returns the tile's first entry
 template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> &m, uint16_t *le, int c0, int nc) {
@@ -920,32 +921,36 @@ __global__ __launch_bounds__(LPC * 32) void k_scalar_vec(DevMesh<T> m, ScalarArg
     }
 }
 
-template <class T>
+template <class T, int WC>
 __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int TC = kTileCells, NF = 8;  // phi, VisEff, dPhidx xyz, centre xyz
-    const int S = m.tslots;
-    T *lc = (T *)smem;                        // [NF][TC]
-    T *lt = lc + NF * TC;                     // [4][S]
-    uint16_t *le = (uint16_t *)(lt + 4 * S);  // [tnnz]
-    uint8_t *lf = (uint8_t *)(le + m.tnnz);   // [S]
+    // phi, VisEff, dPhidx xyz, centre xyz: NF rows of W columns, columns
+    // [0, TC) the tile's cells, column TC + sl the cell of slot sl's face
+    // outside the tile. Phase B overwrites rows 0..3 of column TC + sl with the
+    // slot's terms once the thread of slot sl has read that cell. W is WC when
+    // the mesh's tiles fit it (a compile-time row stride), else TC + tslots.
+    constexpr int TC = kTileCells, NF = 8;
+    const int S = m.tslots, W = WC ? WC : TC + S;
+    T *lx = (T *)smem;                         // [NF][W]
+    uint16_t *le = (uint16_t *)(lx + NF * W);  // [tnnz]
+    uint8_t *lf = (uint8_t *)(le + m.tnnz);    // [S]
     const int t = xcd_block(blockIdx.x, per);
     if (t >= nblk) return;
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
     // A
-    stage1(lc, a.phi, c0, nc);
-    stage1(lc + TC, a.visEff, c0, nc);
-    stage3(lc + 2 * TC, a.dPhidx, c0, nc);
-    stage_centre(lc + 5 * TC, m.cgeo, c0, nc);
+    stage1(lx, a.phi, c0, nc);
+    stage1(lx + W, a.visEff, c0, nc);
+    stage3(lx + 2 * W, a.dPhidx, c0, nc, W);
+    stage_centre(lx + 5 * W, m.cgeo, c0, nc, W);
     const int eb = stage_eslots(m, le, c0, nc);
     int f = 0, out = -1;
     int2 cn = make_int2(0, 0);
     bool owned;
     V4<T> s{};
     T mf = 0, rl = 0;
-    ScalarCell<T> o{};
+    // slot sl's face and, for an interior face, its cell outside the tile into column TC + sl
     auto load = [&](int sl) {
         f = slot_conn(m, ts, sl, cn, owned);
         s = m.fgeoS[f];
@@ -954,27 +959,25 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
             mf = a.massFlux[f];
             rl = m.frl[f];
             out = outside_cell(cn, c0, nc);
-            if (out >= 0) o = scalar_cell(m, a, out);
+            if (out >= 0) {
+                const ScalarCell<T> o = scalar_cell(m, a, out);
+                T *q = lx + TC + sl;
+                q[0] = o.phi, q[W] = o.vis;
+                q[2 * W] = o.g.x, q[3 * W] = o.g.y, q[4 * W] = o.g.z;
+                q[5 * W] = o.cc.x, q[6 * W] = o.cc.y, q[7 * W] = o.cc.z;
+            }
         }
     };
     if (tid < ts.nslot) load(tid);
-    const bool fold = tid < nc;
-    int e0 = 0, n = 0;
-    T su = 0;
-    if (fold) {
-        row_range(m, c0 + tid, e0, n);
-        su = a.su[c0 + tid];
-    }
     __syncthreads();
     // B
-    auto cell = [&](int c) -> ScalarCell<T> {
-        if (c == out) return o;
-        const int l = c - c0;
-        return ScalarCell<T>{lc[l], lc[TC + l], v3<T>{lc[2 * TC + l], lc[3 * TC + l], lc[4 * TC + l]},
-                             v3<T>{lc[5 * TC + l], lc[6 * TC + l], lc[7 * TC + l]}};
-    };
     for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
-        if (sl != tid) load(sl);
+        if (sl != tid) load(sl);  // column TC + sl belongs to this thread alone
+        auto cell = [&](int c) -> ScalarCell<T> {
+            const T *q = lx + (c == out ? TC + sl : c - c0);
+            return ScalarCell<T>{q[0], q[W], v3<T>{q[2 * W], q[3 * W], q[4 * W]},
+                                 v3<T>{q[5 * W], q[6 * W], q[7 * W]}};
+        };
         T tt[4];
         bool fl = false;
         if (cn.y >= 0) {
@@ -983,18 +986,23 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
         } else {
             fl = scalar_boundary(m, a, f, s, cn.x, cell(cn.x), ~cn.y, tt);
         }
-        for (int q = 0; q < 4; q++) lt[q * S + sl] = tt[q];
+        T *q = lx + TC + sl;
+        for (int j = 0; j < 4; j++) q[j * W] = tt[j];
         lf[sl] = fl ? 1 : 0;
     }
     __syncthreads();
     // C: Su, then Au for cells with an Au-updating face
-    if (fold) {
+    if (tid < nc) {
         const int c = c0 + tid;
+        T su = a.su[c];
+        int e0, n;
+        row_range(m, c, e0, n);
+        const T *l0 = lx + TC, *l1 = l0 + W, *l2 = l1 + W, *l3 = l2 + W;
         bool any = false;
         for (int e = e0; e < e0 + n; e++) {
             const int es = le[e - eb];
             const int sl = es & 0x7fff;
-            T t0 = lt[sl], t1 = lt[S + sl], t2 = lt[2 * S + sl];
+            T t0 = l0[sl], t1 = l1[sl], t2 = l2[sl];
             if (es & 0x8000) {
                 t0 = -t0;
                 t1 = -t1;
@@ -1008,7 +1016,7 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
             T au = a.au[c];
             for (int e = e0; e < e0 + n; e++) {
                 const int sl = le[e - eb] & 0x7fff;
-                if (lf[sl]) au = au + lt[3 * S + sl];
+                if (lf[sl]) au = au + l3[sl];
             }
             a.au[c] = au;
         }
@@ -1310,14 +1318,21 @@ __global__ __launch_bounds__(LPC * 32) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T> a, int per, int nblk) {
+template <class T, int WC>
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_uvw_tile(DevMesh<T> m,
+                                                                                                 UvwArgs<T> a,
+                                                                                                 int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int TC = kTileCells, NF = 16;  // u v w VisEff, dudx dvdx dwdx centre (xyz each)
-    const int S = m.tslots;
-    T *lc = (T *)smem;                         // [NF][TC]
-    T *lt = lc + NF * TC;                      // [10][S]
-    uint16_t *le = (uint16_t *)(lt + 10 * S);  // [tnnz]
+    // u v w VisEff, dudx dvdx dwdx centre (xyz each): NF rows of W columns,
+    // columns [0, TC) the tile's cells, column TC + sl the cell of slot sl's
+    // face outside the tile. Phase B overwrites rows 0..9 of column TC + sl
+    // with the slot's terms once the thread of slot sl has read that cell. W
+    // is WC when the mesh's tiles fit it (a compile-time row stride, so LDS
+    // addresses become immediate offsets), else TC + tslots.
+    constexpr int TC = kTileCells, NF = 16;
+    const int S = m.tslots, W = WC ? WC : TC + S;
+    T *lx = (T *)smem;                         // [NF][W]
+    uint16_t *le = (uint16_t *)(lx + NF * W);  // [tnnz]
     uint8_t *lf = (uint8_t *)(le + m.tnnz);    // [S]
     const int t = xcd_block(blockIdx.x, per);
     if (t >= nblk) return;
@@ -1325,21 +1340,21 @@ __global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
     // A
-    stage1(lc, a.u, c0, nc);
-    stage1(lc + TC, a.v, c0, nc);
-    stage1(lc + 2 * TC, a.w, c0, nc);
-    stage1(lc + 3 * TC, a.visEff, c0, nc);
-    stage3(lc + 4 * TC, a.dudx, c0, nc);
-    stage3(lc + 7 * TC, a.dvdx, c0, nc);
-    stage3(lc + 10 * TC, a.dwdx, c0, nc);
-    stage_centre(lc + 13 * TC, m.cgeo, c0, nc);
+    stage1(lx, a.u, c0, nc);
+    stage1(lx + W, a.v, c0, nc);
+    stage1(lx + 2 * W, a.w, c0, nc);
+    stage1(lx + 3 * W, a.visEff, c0, nc);
+    stage3(lx + 4 * W, a.dudx, c0, nc, W);
+    stage3(lx + 7 * W, a.dvdx, c0, nc, W);
+    stage3(lx + 10 * W, a.dwdx, c0, nc, W);
+    stage_centre(lx + 13 * W, m.cgeo, c0, nc, W);
     const int eb = stage_eslots(m, le, c0, nc);
     int f = 0, out = -1;
     int2 cn = make_int2(0, 0);
     bool owned;
     V4<T> s{};
     T mf = 0, rl = 0;
-    UvwCell<T> o{};
+    // slot sl's face and, for an interior face, its cell outside the tile into column TC + sl
     auto load = [&](int sl) {
         f = slot_conn(m, ts, sl, cn, owned);
         s = m.fgeoS[f];
@@ -1348,36 +1363,33 @@ __global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T
             mf = a.massFlux[f];
             rl = m.frl[f];
             out = outside_cell(cn, c0, nc);
-            if (out >= 0) o = uvw_cell(m, a, out);
+            if (out >= 0) {
+                const UvwCell<T> o = uvw_cell(m, a, out);
+                T *q = lx + TC + sl;
+                q[0] = o.u, q[W] = o.v, q[2 * W] = o.w, q[3 * W] = o.vis;
+                q[4 * W] = o.du.x, q[5 * W] = o.du.y, q[6 * W] = o.du.z;
+                q[7 * W] = o.dv.x, q[8 * W] = o.dv.y, q[9 * W] = o.dv.z;
+                q[10 * W] = o.dw.x, q[11 * W] = o.dw.y, q[12 * W] = o.dw.z;
+                q[13 * W] = o.cc.x, q[14 * W] = o.cc.y, q[15 * W] = o.cc.z;
+            }
         }
     };
     if (tid < ts.nslot) load(tid);
-    // fold item tid < 3 * TC: component k of cell cl
-    const int k = tid / TC, cl = tid % TC;
-    const bool fold = tid < 3 * TC && cl < nc;
-    T *dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
-    int e0 = 0, n = 0;
-    T acc = 0;
-    if (fold) {
-        row_range(m, c0 + cl, e0, n);
-        acc = dst[c0 + cl];
-    }
     __syncthreads();
     // B
-    auto cell = [&](int c) -> UvwCell<T> {
-        if (c == out) return o;
-        const T *q = lc + (c - c0);
-        return UvwCell<T>{q[0],
-                          q[TC],
-                          q[2 * TC],
-                          q[3 * TC],
-                          v3<T>{q[4 * TC], q[5 * TC], q[6 * TC]},
-                          v3<T>{q[7 * TC], q[8 * TC], q[9 * TC]},
-                          v3<T>{q[10 * TC], q[11 * TC], q[12 * TC]},
-                          v3<T>{q[13 * TC], q[14 * TC], q[15 * TC]}};
-    };
     for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
-        if (sl != tid) load(sl);
+        if (sl != tid) load(sl);  // column TC + sl belongs to this thread alone
+        auto cell = [&](int c) -> UvwCell<T> {
+            const T *q = lx + (c == out ? TC + sl : c - c0);
+            return UvwCell<T>{q[0],
+                              q[W],
+                              q[2 * W],
+                              q[3 * W],
+                              v3<T>{q[4 * W], q[5 * W], q[6 * W]},
+                              v3<T>{q[7 * W], q[8 * W], q[9 * W]},
+                              v3<T>{q[10 * W], q[11 * W], q[12 * W]},
+                              v3<T>{q[13 * W], q[14 * W], q[15 * W]}};
+        };
         T tt[10];
         bool fl = false;
         if (cn.y >= 0) {
@@ -1386,15 +1398,21 @@ __global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T
         } else {
             fl = uvw_boundary(m, a, f, s, cell(cn.x), ~cn.y, tt);
         }
-        for (int q = 0; q < 10; q++) lt[q * S + sl] = tt[q];
+        T *q = lx + TC + sl;
+        for (int j = 0; j < 10; j++) q[j * W] = tt[j];
         lf[sl] = fl ? 1 : 0;
     }
     __syncthreads();
-    // C: su/sv/sw component k, then au/av/aw component k for cells with an
-    // a-updating face
-    if (fold) {
+    // C: fold item tid < 3 * TC, component k of cell cl: su/sv/sw, then
+    // au/av/aw for cells with an a-updating face
+    const int k = tid / TC, cl = tid % TC;
+    if (tid < 3 * TC && cl < nc) {
         const int c = c0 + cl;
-        const T *l0 = lt + (3 * k) * S, *l1 = l0 + S, *l2 = l1 + S;
+        T *dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
+        T acc = dst[c];
+        int e0, n;
+        row_range(m, c, e0, n);
+        const T *l0 = lx + TC + (3 * k) * W, *l1 = l0 + W, *l2 = l1 + W;
         bool any = false;
         for (int e = e0; e < e0 + n; e++) {
             const int es = le[e - eb];
@@ -1411,10 +1429,11 @@ __global__ __launch_bounds__(kTileBlock) void k_uvw_tile(DevMesh<T> m, UvwArgs<T
         dst[c] = acc;
         if (any) {
             T *dsta = k == 0 ? a.au : (k == 1 ? a.av : a.aw);
+            const T *l9 = lx + TC + 9 * W;
             T aa = dsta[c];
             for (int e = e0; e < e0 + n; e++) {
                 const int sl = le[e - eb] & 0x7fff;
-                if (lf[sl]) aa = aa + lt[9 * S + sl];
+                if (lf[sl]) aa = aa + l9[sl];
             }
             dsta[c] = aa;
         }
@@ -2210,6 +2229,23 @@ template <class T> size_t tile_lds(const DevMesh<T> &d, int nf, int nt, bool fla
     return b <= 65536 ? b : 0;
 }
 
+// Row width of the scalar and UVW tiles' [fields][W] LDS layout: a
+// compile-time width when the mesh's tiles have at most kTileSlots1 (every
+// tile of 4x4x4 hex cells has 240) or kTileSlots2 slots, else 0 (TC +
+// tslots at run time).
+constexpr int kTileSlots1 = 240, kTileSlots2 = 320;
+template <class T> int tile_width(const DevMesh<T> &d) {
+    if (d.tslots <= kTileSlots1) return kTileCells + kTileSlots1;
+    if (d.tslots <= kTileSlots2) return kTileCells + kTileSlots2;
+    return 0;
+}
+// its dynamic LDS: nf rows of W columns, the tile's CSR slots, a flag byte per slot
+template <class T> size_t tile_lds_cols(const DevMesh<T> &d, int nf) {
+    const int w = tile_width(d);
+    const size_t b = (size_t)nf * (w ? w : kTileCells + d.tslots) * sizeof(T) + 2 * (size_t)d.tnnz + d.tslots;
+    return b <= 65536 ? b : 0;
+}
+
 // Gather schedule of gradient, scalar and UVW: the kernel's autotuned schedule
 // if any, else the mesh's. Automatic (schedule 0)
 // and 64 take the cell-tile kernels when the mesh's tiles fit their slot
@@ -2223,7 +2259,8 @@ template <class T> int gather_schedule(const Mesh<T> &M, int kind, int nf, int n
     int l = M.schedule_k[kind] >= 0 ? M.schedule_k[kind] : M.schedule_lpc;
     lds = 0;
     if (l == kTile || l == 0) {
-        if (M.d.ntiles > 0) lds = tile_lds(M.d, nf, nt, flags);
+        // gradient: nf cell fields and nt term rows of tslots; scalar / UVW: nf rows of W columns
+        if (M.d.ntiles > 0) lds = kind == kGatherGradient ? tile_lds(M.d, nf, nt, flags) : tile_lds_cols(M.d, nf);
         if (lds) return kTile;
         l = 0;
     }
@@ -2390,11 +2427,18 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
     if (rc != CUPCFD_E_SUCCESS) return rc;
     if (d.C) {
         size_t lds;
-        const int lpc = gather_schedule(*M, kGatherScalar, 8, 4, true, lds);
+        const int lpc = gather_schedule(*M, kGatherScalar, 8, 0, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == kTile) hipLaunchKernelGGL(k_scalar_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
+        const int w = tile_width(d);
+        if (lpc == kTile && w == kTileCells + kTileSlots1)
+            hipLaunchKernelGGL((k_scalar_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk);
+        else if (lpc == kTile && w == kTileCells + kTileSlots2)
+            hipLaunchKernelGGL((k_scalar_tile<T, kTileCells + kTileSlots2>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk);
+        else if (lpc == kTile) hipLaunchKernelGGL((k_scalar_tile<T, 0>), gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
         else if (lpc == 8) hipLaunchKernelGGL((k_scalar_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 6) hipLaunchKernelGGL((k_scalar_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 4) hipLaunchKernelGGL((k_scalar_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
@@ -2514,11 +2558,18 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
                  gammaBlend, small};
     if (d.C) {
         size_t lds;
-        const int lpc = gather_schedule(*M, kGatherUvw, 16, 10, true, lds);
+        const int lpc = gather_schedule(*M, kGatherUvw, 16, 0, true, lds);
         const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
-        if (lpc == kTile) hipLaunchKernelGGL(k_uvw_tile<T>, gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
+        const int w = tile_width(d);
+        if (lpc == kTile && w == kTileCells + kTileSlots1)
+            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk);
+        else if (lpc == kTile && w == kTileCells + kTileSlots2)
+            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots2>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk);
+        else if (lpc == kTile) hipLaunchKernelGGL((k_uvw_tile<T, 0>), gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
         else if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 6) hipLaunchKernelGGL((k_uvw_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);

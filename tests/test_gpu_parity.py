@@ -169,6 +169,32 @@ def test_schedules_bit_identical(meshname):
                 assert compare(outs[(1, kernel)][k], outs[(lpc, kernel)][k])[0], (kernel, k, lpc)
 
 
+# unordered, these tiles have 342 (typed_perm), 384 (typed32), 228 (tets_typed) and 247 (tets_scattered) face
+# slots: the scalar and UVW tiles take their run-time row width, the 304-column and the 384-column layouts
+SCATTERED = {"tets_scattered": dict(nx=6, ny=5, nz=4, tets=True, jitter=0.3, typed=True, permute_cells=True,
+                                    permute_faces=True, face_extras=True, seed=9)}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["typed_perm", "typed32", "tets_typed", "tets_scattered"])
+def test_tiles_of_scattered_cells(meshname):
+    """Without the Morton renumbering a tile's 64 cells are scattered, so a tile can have more than 256 face
+    slots and threads take a second slot, whose outside cell is staged after the first barrier: still the bits
+    of one thread per cell, in every LDS row width."""
+    host = ch.HostMesh(**SCATTERED[meshname]) if meshname in SCATTERED else _setup(meshname, 64)[0]
+    outs = {}
+    for lpc in (1, 64):
+        dev = ch.DeviceMesh(host, 64, reorder=False)
+        dev.set_schedule(lpc)
+        for kernel in ("gradient", "scalar", "scalar_wall", "uvw"):
+            f = W.Fields(host, seed=13)
+            host.set_state(seed_state(host))
+            outs[(lpc, kernel)] = run(kernel, fvm, dev, f, lambda: host.state(64))[1]
+    for kernel in ("gradient", "scalar", "scalar_wall", "uvw"):
+        for k in outs[(1, kernel)]:
+            assert compare(outs[(1, kernel)][k], outs[(64, kernel)][k])[0], (kernel, k)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("meshname", ["typed_perm", "typed32", "tets_typed"])
 def test_autotune_keeps_bits_and_state(meshname):
