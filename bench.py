@@ -311,6 +311,22 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
                                                   stream))
     tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
                                          sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
+    # FP32-storage scalar flux (per-scalar arrays float, FP64 arithmetic) on the FP32-storage gradient: time, and
+    # max-norm error of Su, Au and PhiB against the FP64 chain on the FP64 inputs, from the same starting arrays
+    su32, au32, pB32 = su_b.float(), au_b.float(), phiB32.clone()
+    tbs32 = timed(lambda: R.scalar_batched_f32s(dev, K, phi32, pB32, sh["visEff"], au32, su32, sh["den"], mf,
+                                                sh["te"], sh["cpB"], sh["visEffB"], rface, g32, W.SCALAR, stream))
+    R.gradient_batched(dev, 1, K, phi_b, phiB_b, g_b, go_b, stream)
+    R.gradient_batched_f32s(dev, 1, K, phi32, phiB32, g32, go32, stream)
+    s64 = dict(su=torch.zeros((C, K), dtype=dt, device=cuda), au=torch.ones((C, K), dtype=dt, device=cuda),
+               pB=phiB_b.clone())
+    s32 = {k: v.float() for k, v in s64.items()}
+    R.scalar_batched(dev, K, phi_b, s64["pB"], sh["visEff"], s64["au"], s64["su"], sh["den"], mf, sh["te"],
+                     sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream)
+    R.scalar_batched_f32s(dev, K, phi32, s32["pB"], sh["visEff"], s32["au"], s32["su"], sh["den"], mf, sh["te"],
+                          sh["cpB"], sh["visEffB"], rface, g32, W.SCALAR, stream)
+    torch.cuda.synchronize()
+    err_s32 = {n: ((s32[n].double() - s64[n]).abs().max() / s64[n].abs().max()).item() for n in ("su", "au", "pB")}
     return {
         "workload": f"gradient + scalar flux of K={K} scalars, {label}, FP64, shared mass flux",
         "cells": C, "K": K,
@@ -319,6 +335,8 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
         "speedup": tp / tb, "bit_identical_sample": bool(same),
         "batched_gradient_ms": tbg * 1e3, "batched_scalar_ms": tbs * 1e3,
         "batched_gradient_f32_storage_ms": tbg32 * 1e3, "f32_storage_gradient_rel_err": err32,
+        "batched_scalar_f32_storage_ms": tbs32 * 1e3,
+        "f32_storage_scalar_rel_err": {"Su": err_s32["su"], "Au": err_s32["au"], "PhiB": err_s32["pB"]},
         "batched_adjoint_ms": tadj * 1e3, "batched_forward_adjoint_ratio": (tb + tadj) / tb,
     }
 

@@ -40,6 +40,12 @@ template <class T> __device__ __forceinline__ v3<T> ld3(const T *__restrict__ p,
 }
 // 3-vector i of an array of vectors whose components are `ks` apart
 // (components of one scalar in a scalar-innermost batch of ks scalars)
+template <class T> __device__ __forceinline__ v3<T> ld3s(const T *p, int i, int ks);
+// a strided 3-vector of storage type S, widened to T
+template <class T, class S> __device__ __forceinline__ v3<T> ld3s_as(const S *p, int i, int ks) {
+    const v3<S> v = ld3s(p, i, ks);
+    return v3<T>{(T)v.x, (T)v.y, (T)v.z};
+}
 template <class T> __device__ __forceinline__ v3<T> ld3s(const T *p, int i, int ks) {
     const size_t b = (size_t)3 * i * ks;
     return v3<T>{p[b], p[b + ks], p[b + 2 * (size_t)ks]};
@@ -653,9 +659,14 @@ template <class T> __global__ void k_mass_region(DevMesh<T> m, T *flowFact, cons
 // ===========================================================================
 // a9  FluxScalarDolfynFaceLoop — ScalarKernels.ipp:36-463
 // ===========================================================================
-template <class T> struct ScalarArgs {
-    const T *phi, *visEff, *den, *massFlux, *te, *cpB, *visEffB, *dPhidx;
-    T *phiB, *au, *su, *rface;
+// TS: storage type of the per-scalar arrays (phi, dPhidx, PhiB, Au, Su);
+// everything else, and all arithmetic, is T
+template <class T, class TS = T> struct ScalarArgs {
+    const TS *phi;
+    const T *visEff, *den, *massFlux, *te, *cpB, *visEffB;
+    const TS *dPhidx;
+    TS *phiB, *au, *su;
+    T *rface;
     const int *src;  // PhiFace source per face (typed meshes), else null
     // stride of the per-scalar arrays (phi, PhiB, Au, Su, dPhidx): 1 for one
     // scalar, K for scalar-innermost batches; write_state: this lane writes
@@ -671,7 +682,7 @@ template <class T> struct ScalarArgs {
     int visHasD2;
 };
 
-template <class T> __device__ __forceinline__ T scalar_visac(const ScalarArgs<T> &a, T Visac) {
+template <class T, class TS> __device__ __forceinline__ T scalar_visac(const ScalarArgs<T, TS> &a, T Visac) {
     if (a.solveTurb) {
         Visac = Visac - a.visLam;
         Visac = a.visLam + Visac / a.visD1;
@@ -682,8 +693,8 @@ template <class T> __device__ __forceinline__ T scalar_visac(const ScalarArgs<T>
 
 // Enthalpy wall branch on a non-adiabatic region (ScalarKernels.ipp:370-445):
 // returns PhiFace and the heat-transfer coefficient Hcoef and VisFace.
-template <class T>
-__device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, int ib, int ir, int ip,
+template <class T, class TS>
+__device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f, int ib, int ir, int ip,
                                  T &Hcoef, T &VisFace) {
     T PhiFlux = 0, PhiFace = 0;
     if (m.rflux[ir]) PhiFlux = m.rt[ir];
@@ -706,12 +717,12 @@ __device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T> &a, in
             Hcoef = 1.0 / (1.0 / VisFace + Resist * cp) * area;
         }
     }
-    if (m.rflux[ir]) PhiFace = a.phi[(size_t)ip * a.ks] + PhiFlux / (Hcoef * cp / area);
+    if (m.rflux[ir]) PhiFace = (T)a.phi[(size_t)ip * a.ks] + PhiFlux / (Hcoef * cp / area);
     return PhiFace;
 }
 
 // PhiFace as carried into face f: the value the last assigning face left
-template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarArgs<T> &a, int f) {
+template <class T, class TS> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f) {
     if (!a.src) return m.carry_init;
     const int g = a.src[f];
     if (g < 0) return m.carry_init;
@@ -722,7 +733,7 @@ template <class T> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarAr
     if (it == CUPCFD_RTYPE_INLET) return (T)(0);
     if (it == CUPCFD_RTYPE_OUTLET) {
         const v3<T> Xpn = sub3(xyz(m.fgeoX[g]), xyz(m.cgeo[ip]));
-        return a.phi[(size_t)ip * a.ks] + dot3(ld3s(a.dPhidx, ip, a.ks), Xpn);
+        return (T)a.phi[(size_t)ip * a.ks] + dot3(ld3s_as<T>(a.dPhidx, ip, a.ks), Xpn);
     }
     T H, V;
     return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
@@ -741,8 +752,8 @@ __device__ __forceinline__ ScalarCell<T> scalar_cell(const DevMesh<T> &m, const 
 // Interior face f of the scalar loop (ScalarKernels.ipp:135-217): terms of
 // the cell1 update Su[ip] = ((Su[ip] - blend) + fde1) - fdi; the cell2 update
 // is the same three terms negated. Writes RFace when `owner`.
-template <class T>
-__device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, const V4<T> &s,
+template <class T, class TS>
+__device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f, const V4<T> &s,
                                                 T mf, T rl,
                                                 const ScalarCell<T> &P, const ScalarCell<T> &N, bool owner,
                                                 T t[3]) {
@@ -772,8 +783,8 @@ __device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const Scala
 // Boundary face f of cell P = cell1 (ScalarKernels.ipp:219-447): t[0..2] the
 // additions to Su, t[3] to Au; returns whether Au is updated (INLET, enthalpy
 // WALL). Writes PhiB and the mesh's Q/H/T.
-template <class T>
-__device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const ScalarArgs<T> &a, int f, const V4<T> &s,
+template <class T, class TS>
+__device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f, const V4<T> &s,
                                                 int ip, const ScalarCell<T> &P, int ib, T t[4]) {
     const v3<T> S{s.x, s.y, s.z};
     t[0] = t[1] = t[2] = t[3] = nzero<T>();
@@ -793,7 +804,7 @@ __device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const Scala
         t[1] = -fdi;
         t[2] = -(fF * PhiFace);
         t[3] = -fF;
-        a.phiB[(size_t)ib * a.ks] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = (TS)(PhiFace);
         return true;
     }
     if (it == CUPCFD_RTYPE_OUTLET) {
@@ -804,17 +815,17 @@ __device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const Scala
         // Su = (Su + fde) - fdi
         t[0] = Visac * dot3(P.g, S);
         t[1] = -(VisFace * dot3(P.g, Xpn));
-        a.phiB[(size_t)ib * a.ks] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = (TS)(PhiFace);
         return false;
     }
     if (it == CUPCFD_RTYPE_SYMP) {
         const v3<T> ds = sub3(xyz(m.fgeoX[f]), P.cc);
-        a.phiB[(size_t)ib * a.ks] = P.phi + dot3(P.g, ds);
+        a.phiB[(size_t)ib * a.ks] = (TS)(P.phi + dot3(P.g, ds));
         return false;
     }
     if (it == CUPCFD_RTYPE_WALL && a.wallActive) {
         if (m.radiab[ir]) {
-            a.phiB[(size_t)ib * a.ks] = P.phi;
+            a.phiB[(size_t)ib * a.ks] = (TS)(P.phi);
             if (a.write_state) m.bq[ib] = 0.0;
             return false;
         }
@@ -822,7 +833,7 @@ __device__ __forceinline__ bool scalar_boundary(const DevMesh<T> &m, const Scala
         const T PhiFace = scalar_wall_phiface(m, a, f, ib, ir, ip, Hcoef, VisFace);
         const T area = m.fgeoX[f].w;
         const T cp = a.cpB[ib];
-        a.phiB[(size_t)ib * a.ks] = PhiFace;
+        a.phiB[(size_t)ib * a.ks] = (TS)(PhiFace);
         // Au = Au + Hcoef;  Su = Su + Hcoef * PhiFace
         t[0] = Hcoef * PhiFace;
         t[3] = Hcoef;
@@ -1809,8 +1820,8 @@ template <class T> struct ScalarFaceCoef {
     double fmn, fmx;
 };
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_scalar_coef(DevMesh<T> m, ScalarArgs<T> a, ScalarFaceCoef<T> *coef) {
+template <class T, class TS>
+__global__ __launch_bounds__(kBlock) void k_scalar_coef(DevMesh<T> m, ScalarArgs<T, TS> a, ScalarFaceCoef<T> *coef) {
     const int f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= m.Fi) return;
     const int2 cn = m.fconn[f];
@@ -1835,13 +1846,13 @@ __global__ __launch_bounds__(kBlock) void k_scalar_coef(DevMesh<T> m, ScalarArgs
     a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
 }
 
-template <class T, int G>
-__global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarArgs<T> a0,
+template <class T, class TS, int G>
+__global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarArgs<T, TS> a0,
                                                            const ScalarFaceCoef<T> *coef, int K, int per, int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     // this lane's view: per-scalar arrays offset to scalar k, stride K
-    ScalarArgs<T> a = a0;
+    ScalarArgs<T, TS> a = a0;
     a.phi += L.k;
     a.phiB += L.k;
     a.au += L.k;
@@ -1850,8 +1861,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarA
     a.ks = K;
     a.write_state = (L.k == K - 1);  // K plain calls leave the last scalar's mesh state
     const int c = batch_uniform<G>(L.c);
-    const ScalarCell<T> own{a.phi[(size_t)c * K], a.visEff[c], ld3s(a.dPhidx, c, K), xyz(m.cgeo[c])};
-    T su = a.su[(size_t)c * K];
+    const ScalarCell<T> own{(T)a.phi[(size_t)c * K], a.visEff[c], ld3s_as<T>(a.dPhidx, c, K), xyz(m.cgeo[c])};
+    T su = (T)a.su[(size_t)c * K];
     T au = 0;
     bool au_loaded = false;
     int e0, n;
@@ -1865,8 +1876,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarA
         if (en.y >= 0) {
             // scalar_interior's terms with the face's phi-free parts from coef
             const ScalarFaceCoef<T> q = coef[f];
-            const T ophi = a.phi[(size_t)en.y * K];
-            const v3<T> og = ld3s(a.dPhidx, en.y, K);
+            const T ophi = (T)a.phi[(size_t)en.y * K];
+            const v3<T> og = ld3s_as<T>(a.dPhidx, en.y, K);
             const bool in = (en.x & 1) != 0;
             const T phiP = in ? ophi : own.phi, phiN = in ? own.phi : ophi;
             const v3<T> gP = in ? og : own.g, gN = in ? own.g : og;
@@ -1893,14 +1904,14 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarA
         su = ((su + t[0]) + t[1]) + t[2];
         if (ta) {
             if (!au_loaded) {
-                au = a.au[(size_t)c * K];
+                au = (T)a.au[(size_t)c * K];
                 au_loaded = true;
             }
             au = au + t[3];
         }
     }
-    a.su[(size_t)c * K] = su;
-    if (au_loaded) a.au[(size_t)c * K] = au;
+    a.su[(size_t)c * K] = (TS)su;
+    if (au_loaded) a.au[(size_t)c * K] = (TS)au;
 }
 
 // ===========================================================================
@@ -2354,12 +2365,12 @@ int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, co
 
 // The scalar loop's arguments for one call (ks = 1: one scalar; batches set
 // their own stride in the kernel)
-template <class T>
-int scalar_args(Mesh<T> &M, ScalarArgs<T> &a, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den,
-                const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx, int SolveTurb,
-                int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k,
-                int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T TMCmu) {
-    a = ScalarArgs<T>{};
+template <class T, class TS>
+int scalar_args(Mesh<T> &M, ScalarArgs<T, TS> &a, const TS *Phi, TS *PhiB, const T *VisEff, TS *Au, TS *Su,
+                const T *Den, const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace, const TS *dPhidx,
+                int SolveTurb, int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
+                T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T TMCmu) {
+    a = ScalarArgs<T, TS>{};
     a.phi = Phi;
     a.visEff = VisEff;
     a.den = Den;
@@ -2516,9 +2527,10 @@ int dev_gradient_batched(cupcfd_hip_mesh *h, int nGradient, int K, const TI *phi
 }
 
 
-template <class T>
-int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const T *Phi, T *PhiB, const T *VisEff, T *Au, T *Su, const T *Den,
-                       const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace, const T *dPhidx,
+template <class T, class TS = T>
+int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, TS *PhiB, const T *VisEff, TS *Au, TS *Su,
+                       const T *Den, const T *MF, const T *TE, const T *CpB, const T *VisEffB, T *RFace,
+                       const TS *dPhidx,
                        int SolveTurb, int SolveEnthalpy, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
                        T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T Small, T Large,
                        T TMCmu, void *stream) {
@@ -2527,7 +2539,7 @@ int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const T *Phi, T *PhiB, const T
     if (!M) return CUPCFD_E_NULL_PTR;
     if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
     const DevMesh<T> &d = M->d;
-    ScalarArgs<T> a;
+    ScalarArgs<T, TS> a;
     const int rc = scalar_args(*M, a, Phi, PhiB, VisEff, Au, Su, Den, MF, TE, CpB, VisEffB, RFace, dPhidx, SolveTurb,
                                SolveEnthalpy, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e,
                                Sigma_s, Schmidt, GammaBlend, Small, TMCmu);
@@ -2537,13 +2549,13 @@ int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const T *Phi, T *PhiB, const T
     // the faces' phi-free coefficients, once for all K scalars (and RFace)
     ScalarFaceCoef<T> *coef = (ScalarFaceCoef<T> *)M->ws_get(46, (size_t)(d.Fi ? d.Fi : 1) * sizeof(ScalarFaceCoef<T>));
     if (!coef) return CUPCFD_E_ERROR;
-    if (d.Fi) hipLaunchKernelGGL(k_scalar_coef<T>, dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a, coef);
+    if (d.Fi) hipLaunchKernelGGL((k_scalar_coef<T, TS>), dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a, coef);
     const int G = batch_group(K);
     const Grid gr = batch_grid(d.C, G);
     const dim3 grid(gr.blocks, (K + G - 1) / G);
-    if (G == 4) hipLaunchKernelGGL((k_scalar_batched<T, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
-    else if (G == 16) hipLaunchKernelGGL((k_scalar_batched<T, 16>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
-    else hipLaunchKernelGGL((k_scalar_batched<T, 64>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    if (G == 4) hipLaunchKernelGGL((k_scalar_batched<T, TS, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    else if (G == 16) hipLaunchKernelGGL((k_scalar_batched<T, TS, 16>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    else hipLaunchKernelGGL((k_scalar_batched<T, TS, 64>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
     return launch_ok();
 }
 
@@ -2600,7 +2612,7 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
     const DevMesh<T> &d = M->d;
     if (d.typed) return CUPCFD_E_NOT_IMPLEMENTED;
     AdjArgs<T> a{};
-    const int rc = scalar_args<T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
+    const int rc = scalar_args<T, T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
                                nullptr, nullptr, nullptr, SolveTurb, 0, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE,
                                Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, (T)0, (T)1);
     if (rc != CUPCFD_E_SUCCESS) return rc;
@@ -2645,7 +2657,7 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T 
     const DevMesh<T> &d = M->d;
     if (d.typed) return CUPCFD_E_NOT_IMPLEMENTED;
     AdjBatchArgs<T> a{};
-    const int rc = scalar_args<T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
+    const int rc = scalar_args<T, T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, SolveTurb, 0, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE,
                                   Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, (T)0, (T)1);
     if (rc != CUPCFD_E_SUCCESS) return rc;
@@ -3113,6 +3125,20 @@ int cupcfd_dev_gradient_batched_f32s(cupcfd_hip_mesh *mesh, int32_t nGradient, i
                                      const float *phiBoundary, float *dPhidxCell, float *dPhidxoCell, void *stream) {
     return dev_gradient_batched<double, float, float>(mesh, nGradient, K, phiCell, phiBoundary, dPhidxCell,
                                                       dPhidxoCell, stream);
+}
+
+int cupcfd_dev_flux_scalar_batched_f32s(cupcfd_hip_mesh *mesh, int32_t K, const float *PhiCell, float *PhiBoundary,
+                                         const double *VisEff, float *Au, float *Su, const double *Den,
+                                         const double *MassFlux, const double *TE, const double *CpBoundary,
+                                         const double *visEffBoundary, double *RFace, const float *dPhidx,
+                                         int32_t SolveTurb, int32_t SolveEnthalpy, double vislam, int32_t ivar,
+                                         int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE, double Sigma_k,
+                                         int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
+                                         double GammaBlend, double Small, double Large, double TMCmu, void *stream) {
+    return dev_scalar_batched<double, float>(mesh, K, PhiCell, PhiBoundary, VisEff, Au, Su, Den, MassFlux, TE,
+                                             CpBoundary, visEffBoundary, RFace, dPhidx, SolveTurb, SolveEnthalpy,
+                                             vislam, ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e,
+                                             Sigma_s, Schmidt, GammaBlend, Small, Large, TMCmu, stream);
 }
 
 int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,

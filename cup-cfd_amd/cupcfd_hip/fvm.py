@@ -275,6 +275,23 @@ class resident:
             mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB), cls._t(g), cls._t(go), cls._s(stream)))
 
     @classmethod
+    def scalar_batched_f32s(cls, mesh, K, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx,
+                            params, stream=None):
+        """Batched scalar flux with float32 storage on a 64-bit mesh: phi, phiB, au, su, dphidx float32 (laid out as
+        for scalar_batched); the shared arrays and all arithmetic float64, Au/Su/PhiB rounded to float once."""
+        from . import lib, check
+        assert mesh.precision == 64
+        t = cls._t
+        p = params
+        D = ct.c_double
+        check("cupcfd_dev_flux_scalar_batched_f32s", lib().cupcfd_dev_flux_scalar_batched_f32s(
+            mesh.handle, _I(K), t(phi), t(phiB), t(visEff), t(au), t(su), t(den), t(mf), t(te), t(cpB), t(visEffB),
+            t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]), D(p["vislam"]), _I(p["ivar"]),
+            _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]), _I(p["VarED"]),
+            D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), D(p["Small"]), D(p["Large"]),
+            D(p["TMCmu"]), cls._s(stream)))
+
+    @classmethod
     def gradient_batched_set(cls, meshset, K, phi_ptrs, phiB_ptrs, g_ptrs, go_ptrs=None, stream=None):
         """Batched gradient (nGradient = 1) on every mesh of a DeviceMeshSet in one persistent launch.
         *_ptrs are int64 device tensors holding each member's device array address (data_ptr())."""

@@ -385,6 +385,24 @@ CUPCFD_BATCH_API(float, f32)
 int cupcfd_dev_gradient_batched_f32s(cupcfd_hip_mesh *mesh, int32_t nGradient, int32_t K,
     const float *phiCell, const float *phiBoundary, float *dPhidxCell, float *dPhidxoCell, void *stream);
 
+/* Batched scalar flux with FP32 storage and FP64 arithmetic, on a 64-bit mesh
+ * (no counterpart in the reference). The per-scalar arrays PhiCell,
+ * PhiBoundary, Au, Su and dPhidx are float, laid out as
+ * cupcfd_dev_flux_scalar_batched_f64's; the shared arrays (VisEff, Den,
+ * MassFlux, TE, CpBoundary, visEffBoundary, RFace), the geometry and every
+ * product and sum are double. Each Au, Su and PhiBoundary entry is widened,
+ * updated in double and rounded to float once, so the result is
+ * cupcfd_dev_flux_scalar_batched_f64's on the inputs widened to double,
+ * rounded to float (round to nearest even). */
+int cupcfd_dev_flux_scalar_batched_f32s(cupcfd_hip_mesh *mesh, int32_t K, const float *PhiCell,
+    float *PhiBoundary, const double *VisEff, float *Au, float *Su, const double *Den,
+    const double *MassFlux, const double *TE, const double *CpBoundary,
+    const double *visEffBoundary, double *RFace, const float *dPhidx, int32_t SolveTurb,
+    int32_t SolveEnthalpy, double vislam, int32_t ivar, int32_t VarT, double Sigma_T,
+    double Prandtl, int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e,
+    double Sigma_s, double Schmidt, double GammaBlend, double Small, double Large, double TMCmu,
+    void *stream);
+
 /* ---------------------------------------------------------------------------
  * Mesh sets: one persistent launch over many meshes of one precision (no
  * counterpart in the reference; a rank owning several partitions would make

@@ -163,6 +163,49 @@ def test_gradient_batched_f32_storage_is_rounded_f64(K, meshname, nG):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "tets_typed", "typed32"])
+@pytest.mark.parametrize("K", [3, 16, 64, 70])
+@pytest.mark.parametrize("variant", ["scalar", "scalar_wall"])
+def test_scalar_batched_f32_storage_is_rounded_f64(K, meshname, variant):
+    """FP32 storage of the per-scalar arrays, FP64 everything else: Su, Au and PhiB are the FP64 batched
+    results on the widened inputs, each rounded to float once; RFace and the mesh state are the FP64 call's."""
+    if meshname == "typed32" and K > 16:
+        pytest.skip("covered by the smaller meshes")
+    host, dev = _mesh(meshname, 64)
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    p = dict(W.SCALAR)
+    if variant == "scalar_wall":
+        p["ivar"] = p["VarT"]
+        p["SolveEnthalpy"] = 1
+    rng = np.random.default_rng(500 + K)
+    f = W.Fields(host, seed=9)
+    per = dict(phi=rng.uniform(-1, 1, (C, K)), phiB=rng.uniform(-1, 1, (B, K)), au=rng.uniform(0.5, 1.5, (C, K)),
+               su=rng.uniform(-1, 1, (C, K)), dphidx=rng.uniform(-1, 1, (C, 3, K)))
+    sh = {k: _dev(v, np.float64) for k, v in
+          dict(visEff=f.visEff, den=f.den, mf=f.massFlux, te=f.te, cpB=f.cpB, visEffB=f.visEffB).items()}
+    st0 = seed_state(host)
+    outs = []
+    for storage in (np.float32, np.float64):
+        host.set_state(st0, 64)
+        dev.push_state()
+        # the FP64 call reads the float inputs widened
+        a = {k: _dev(v.astype(np.float32).astype(storage), storage) for k, v in per.items()}
+        rface = torch.zeros(2 * F, dtype=torch.float64, device="cuda")
+        call = R.scalar_batched_f32s if storage == np.float32 else R.scalar_batched
+        call(dev, K, a["phi"], a["phiB"], sh["visEff"], a["au"], a["su"], sh["den"], sh["mf"], sh["te"], sh["cpB"],
+             sh["visEffB"], rface, a["dphidx"], p)
+        torch.cuda.synchronize()
+        dev.pull_state()
+        outs.append((a, rface, host.state(64)))
+    (a32, r32, st32), (a64, r64, st64) = outs
+    for n in ("su", "au", "phiB"):
+        assert _same(a32[n], a64[n].to(torch.float32)), n
+    assert _same(r32, r64)
+    for n in st32:
+        assert np.array_equal(st32[n], st64[n]), n
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("K", [3, 16, 64, 70])
 def test_gradient_batched_meshset_matches_per_mesh(K, precision):
