@@ -867,6 +867,111 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
     return W.err == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
 }
 
+// ---- order-preserving face colouring -----------------------------------------
+
+namespace {
+
+__global__ void k_entry_cell(int C, const int *row, int *ecell) {
+    GRID_STRIDE(c, C) {
+        for (int e = row[c]; e < row[c + 1]; e++) ecell[e] = (int)c;
+    }
+}
+
+// colour(face of entry e) >= colour(face of entry e-1 in the same row) + 1.
+// Colours only rise and never pass their fixed point (the longest chain of
+// faces ending at the face along shared rows), so repeating this until
+// nothing changes reaches exactly that fixed point in any update order.
+__global__ void __launch_bounds__(kB) k_colour_relax(int nnz, const int *row, const int *ecell, const int2 *ent,
+                                                     int *colour, int *changed) {
+    __shared__ int sc[1];
+    BlockCounters bc(sc, 1);
+    int any = 0;
+    GRID_STRIDE(e, nnz) {
+        if (e == row[ecell[e]]) continue;
+        const int p = ent[e - 1].x >> 1, k = ent[e].x >> 1;
+        const int cand = __hip_atomic_load(&colour[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (cand > __hip_atomic_load(&colour[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomicMax(&colour[k], cand);
+            any = 1;
+        }
+    }
+    if (any) atomicOr(&sc[0], 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && sc[0]) atomicOr(changed, 1);
+}
+
+__global__ void __launch_bounds__(kB) k_colour_stats(int F, const int *colour, unsigned *key, int *val, int *cmax) {
+    __shared__ int sc[1];
+    BlockCounters bc(sc, 1);
+    int mx = 0;
+    GRID_STRIDE(k, F) {
+        key[k] = (unsigned)colour[k];
+        val[k] = (int)k;
+        mx = colour[k] > mx ? colour[k] : mx;
+    }
+    atomicMax(&sc[0], mx);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(cmax, sc[0]);
+}
+
+__global__ void k_colour_count(int F, const unsigned *key, int *cnt) {
+    GRID_STRIDE(k, F) atomicAdd(&cnt[key[k]], 1);
+}
+
+}  // namespace
+
+template <class T> int build_face_colours(Mesh<T> &M) {
+    if (M.ncolours >= 0) return CUPCFD_E_SUCCESS;
+    const DevMesh<T> &d = M.d;
+    const int F = d.F, C = d.C, nnz = d.nnz;
+    Work W(M.allocs);
+    W.tmp.min_chunk = std::max<size_t>(size_t(1) << 20, 24 * ((size_t)F + nnz));
+    int *colour = W.alloc<int>(F, true);
+    int *ecell = W.alloc<int>(nnz, false);
+    int *flag = W.alloc<int>(2, false);
+    W.zero(colour, F);
+    W.launch(C, k_entry_cell, C, d.row, ecell);
+    // each pass lengthens every chain it has not finished by at least one face
+    for (int it = 0; it <= F && W.err == hipSuccess; it++) {
+        W.zero(flag, 1);
+        W.launch(nnz, k_colour_relax, nnz, d.row, (const int *)ecell, d.ent, colour, flag);
+        int changed = 0;
+        if (W.err == hipSuccess) W.note(hipMemcpy(&changed, flag, sizeof(int), hipMemcpyDeviceToHost));
+        if (!changed) break;
+    }
+    unsigned *key = W.alloc<unsigned>(F, false), *skey = W.alloc<unsigned>(F, false);
+    int *val = W.alloc<int>(F, false);
+    int *faces = W.alloc<int>(F, true);
+    W.zero(flag, 2);
+    W.launch(F, k_colour_stats, F, (const int *)colour, key, val, flag + 1);
+    int cmax = 0;
+    if (W.err == hipSuccess) W.note(hipMemcpy(&cmax, flag + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (W.err != hipSuccess) return CUPCFD_E_ERROR;
+    const int ncol = F ? cmax + 1 : 0;
+    if (F)
+        W.prim_call([&](void *tmp, size_t &bytes) {
+            return rocprim::radix_sort_pairs(tmp, bytes, key, skey, val, faces, (size_t)F, 0, bits_for(ncol),
+                                             (hipStream_t)0);
+        });
+    int *cnt = W.alloc<int>((size_t)ncol + 1, false), *off = W.alloc<int>((size_t)ncol + 1, false);
+    W.zero(cnt, (size_t)ncol + 1);
+    W.launch(F, k_colour_count, F, (const unsigned *)key, cnt);
+    W.prim_call([&](void *tmp, size_t &bytes) {
+        return rocprim::exclusive_scan(tmp, bytes, cnt, off, 0, (size_t)ncol + 1, rocprim::plus<int>(),
+                                       (hipStream_t)0);
+    });
+    W.download(M.colour_off, (const int *)off, (size_t)ncol + 1);
+    if (W.err == hipSuccess) W.note(hipDeviceSynchronize());
+    if (W.err != hipSuccess) return CUPCFD_E_ERROR;
+    M.colour_of = colour;
+    M.colour_faces = faces;
+    M.ncolours = ncol;
+    return CUPCFD_E_SUCCESS;
+}
+
+template int build_face_colours<double>(Mesh<double> &);
+template int build_face_colours<float>(Mesh<float> &);
+
 template int build_device<double, cupcfd_mesh_view_f64>(Mesh<double> &, const cupcfd_mesh_view_f64 &, int, bool);
 template int build_device<float, cupcfd_mesh_view_f32>(Mesh<float> &, const cupcfd_mesh_view_f32 &, int, bool);
 

@@ -730,8 +730,8 @@ extern "C" int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh) {
 
 extern "C" int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
-    if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 4 && lanes_per_cell != 6 &&
-        lanes_per_cell != 8 && lanes_per_cell != 64)
+    if (lanes_per_cell != 0 && lanes_per_cell != 1 && lanes_per_cell != 2 && lanes_per_cell != 4 &&
+        lanes_per_cell != 6 && lanes_per_cell != 8 && lanes_per_cell != 64)
         return CUPCFD_E_CONFIG_INVALID_VALUE;
     auto set = [&](auto *M) {
         M->schedule_lpc = lanes_per_cell;

@@ -116,6 +116,14 @@ template <class T> struct Mesh {
     int schedule_k[3] = {-1, -1, -1};
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    // order-preserving face colouring, built on first use of the colour-group
+    // schedule (-1: not built): colour(f) = 1 + the largest colour of the face
+    // before f in the rows of f's cells, so colours rise along every row and no
+    // two faces of one colour share a cell. Faces of colour q (internal ids,
+    // ascending) are colour_faces[colour_off[q] .. colour_off[q + 1]).
+    int ncolours = -1;
+    const int *colour_of = nullptr, *colour_faces = nullptr;
+    std::vector<int> colour_off;
     // workspace pool for drop-in staging and multi-sweep scratch
     std::vector<void *> ws;
     std::vector<size_t> ws_size;
@@ -169,6 +177,9 @@ struct DeviceChunks {
         return hipSuccess;
     }
 };
+
+// Build M's face colouring on the device from its CSR (no-op once built).
+template <class T> int build_face_colours(Mesh<T> &M);
 
 // The same repack as the host build, run on the device: every array of the
 // DevMesh and every host-side mirror of Mesh comes out bit-identical. With

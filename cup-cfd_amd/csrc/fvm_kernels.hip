@@ -271,6 +271,43 @@ __global__ __launch_bounds__(LPC * 32) void k_gradient_vec(DevMesh<T> m, GradArg
     }
 }
 
+// Colour-group schedule: one launch per colour of the mesh's order-preserving
+// face colouring. Within a colour no two faces share a cell, so each face adds
+// its terms to its cells' accumulators directly; colours rise along every
+// cell's row, so a cell receives its terms in row order, from the zero the
+// reference starts from: the gather's sums, bit for bit.
+template <class T> __global__ void k_fill_zero(size_t n, T *p) {
+    const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
+    if (i < n) p[i] = (T)0;
+}
+
+template <class T, bool SWEEP0>
+__global__ __launch_bounds__(kBlock) void k_gradient_colour(DevMesh<T> m, GradArgs<T> a, const int *faces, int n,
+                                                            T *acc) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int f = faces[i];
+    const int2 cn = m.fconn[f];
+    const V4<T> s = m.fgeoS[f];
+    T t[3];
+    if (cn.y >= 0) {
+        gradient_interior<T, SWEEP0>(SWEEP0 ? s : m.fgeoX[f], s, grad_cell<T, SWEEP0>(m, a, cn.x),
+                                     grad_cell<T, SWEEP0>(m, a, cn.y), t);
+        for (int k = 0; k < 3; k++) acc[3 * (size_t)cn.y + k] = acc[3 * (size_t)cn.y + k] + t[k];
+    } else {
+        gradient_boundary(a, ~cn.y, s, t);
+    }
+    for (int k = 0; k < 3; k++) acc[3 * (size_t)cn.x + k] = acc[3 * (size_t)cn.x + k] + t[k];
+}
+
+template <class T> __global__ __launch_bounds__(kBlock) void k_gradient_colour_store(DevMesh<T> m, GradArgs<T> a,
+                                                                                      const T *acc) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.C) return;
+    const T vol = m.cvol[c];
+    for (int k = 0; k < 3; k++) gradient_store(a, c, k, acc[3 * (size_t)c + k], vol);
+}
+
 // ---------------------------------------------------------------------------
 // Tile schedule (all three gathers): block b works on tile t = xcd_block(b),
 // the cells [64t, 64t + 64), in three phases separated by barriers:
@@ -2264,11 +2301,16 @@ template <class T> size_t tile_lds_cols(const DevMesh<T> &d, int nf) {
 // schedule if set (1 = one thread per cell), else the row length of uniform 4-
 // or 6-face meshes (every lane busy), else 8 if every row fits, else one
 // thread per cell. Returns kTile, 4, 6, 8, or 0 for one thread per cell.
-constexpr int kTile = 64;  // the tile schedule's code in cupcfd_hip_mesh_set_schedule
+constexpr int kTile = 64;   // the tile schedule's code in cupcfd_hip_mesh_set_schedule
+constexpr int kColour = 2;  // colour groups (gradient; the other gathers take the automatic schedule)
 enum { kGatherGradient = 0, kGatherScalar = 1, kGatherUvw = 2 };
 template <class T> int gather_schedule(const Mesh<T> &M, int kind, int nf, int nt, bool flags, size_t &lds) {
     int l = M.schedule_k[kind] >= 0 ? M.schedule_k[kind] : M.schedule_lpc;
     lds = 0;
+    if (l == kColour) {
+        if (kind == kGatherGradient) return kColour;
+        l = 0;
+    }
     if (l == kTile || l == 0) {
         // gradient: nf cell fields and nt term rows of tslots; scalar / UVW: nf rows of W columns
         if (M.d.ntiles > 0) lds = kind == kGatherGradient ? tile_lds(M.d, nf, nt, flags) : tile_lds_cols(M.d, nf);
@@ -2312,13 +2354,31 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
     const int lpc = gather_schedule(*M, kGatherGradient, 7, 3, false, lds);
     const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
     const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
+    T *acc = nullptr;
+    if (lpc == kColour) {
+        // the colouring is built on first use (allocates and synchronises)
+        if (build_face_colours(*M) != CUPCFD_E_SUCCESS) return CUPCFD_E_ERROR;
+        acc = (T *)M->ws_get(52, 3 * (size_t)d.C * sizeof(T));
+        if (!acc) return CUPCFD_E_ERROR;
+    }
     const T *prev = nullptr;
     for (int it = 0; it < nGradient; it++) {
         const bool last = (it == nGradient - 1);
         T *dst = last ? g : t[it & 1];
         GradArgs<T> ga{phi, phiB, prev, dst, last ? go : nullptr};
         const bool s0 = (it == 0);
-        if (lpc == kTile) {
+        if (lpc == kColour) {
+            hipLaunchKernelGGL(k_fill_zero<T>, dim3(blocks(3 * d.C)), dim3(kBlock), 0, s, 3 * (size_t)d.C, acc);
+            for (int q = 0; q < M->ncolours; q++) {
+                const int *fq = M->colour_faces + M->colour_off[q];
+                const int nq = M->colour_off[q + 1] - M->colour_off[q];
+                if (nq == 0) continue;
+                if (s0) hipLaunchKernelGGL((k_gradient_colour<T, true>), dim3(blocks(nq)), dim3(kBlock), 0, s, d, ga, fq, nq, acc);
+                else hipLaunchKernelGGL((k_gradient_colour<T, false>), dim3(blocks(nq)), dim3(kBlock), 0, s, d, ga, fq, nq, acc);
+            }
+            hipLaunchKernelGGL(k_gradient_colour_store<T>, dim3(blocks(d.C)), dim3(kBlock), 0, s, d, ga,
+                               (const T *)acc);
+        } else if (lpc == kTile) {
             if (s0) hipLaunchKernelGGL((k_gradient_tile<T, true>), gr.blocks, nthr, lds, s, d, ga, gr.per, gr.nblk);
             else hipLaunchKernelGGL((k_gradient_tile<T, false>), gr.blocks, nthr, lds, s, d, ga, gr.per, gr.nblk);
         } else if (lpc == 8) {
@@ -2702,12 +2762,13 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
     if (!M) return CUPCFD_E_NULL_PTR;
     const DevMesh<T> &d = M->d;
     const size_t C = d.C, F = d.F, B = d.B, R = d.R;
-    int cand[3];
+    int cand[4];
     int nc = 0;
     cand[nc++] = kTile;
     if (d.rowlen == 4 || d.rowlen == 6) cand[nc++] = d.rowlen;
     else if (d.maxrow <= 8) cand[nc++] = d.maxrow <= 4 ? 4 : 8;
     cand[nc++] = 1;
+    cand[nc++] = kColour;  // tried for the gradient only
     if (C == 0) {
         for (int k = 0; k < 3; k++) {
             M->schedule_k[k] = -1;
@@ -2762,6 +2823,7 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
         float best = 0;
         int pick = cand[0];
         for (int j = 0; j < nc && rc == CUPCFD_E_SUCCESS; j++) {
+            if (cand[j] == kColour && kind != kGatherGradient) continue;
             M->schedule_k[kind] = cand[j];
             rc = run(kind);  // warm-up (and lazily built tables)
             float tmin = 0;
@@ -3161,6 +3223,21 @@ int cupcfd_dev_scalar_adjoint_batched_f64(cupcfd_hip_mesh *mesh, int32_t K, cons
     return dev_scalar_adjoint_batched<double>(mesh, K, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam,
                                               ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s,
                                               Schmidt, GammaBlend, PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_hip_mesh_face_colours(cupcfd_hip_mesh *mesh, int32_t *colour, int32_t *ncolours) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    auto fetch = [&](auto *M) -> int {
+        if (build_face_colours(*M) != CUPCFD_E_SUCCESS) return CUPCFD_E_ERROR;
+        if (ncolours) *ncolours = M->ncolours;
+        if (!colour || M->d.F == 0) return CUPCFD_E_SUCCESS;
+        std::vector<int32_t> c(M->d.F);
+        if (hipMemcpy(c.data(), M->colour_of, c.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            return CUPCFD_E_ERROR;
+        for (int k = 0; k < M->d.F; k++) colour[M->face_order[k]] = c[k];  // mesh numbering
+        return CUPCFD_E_SUCCESS;
+    };
+    return mesh->precision == 64 ? fetch(mesh->m64) : fetch(mesh->m32);
 }
 
 int cupcfd_hip_meshset_create(cupcfd_hip_mesh *const *meshes, int32_t n, cupcfd_hip_meshset **out) {

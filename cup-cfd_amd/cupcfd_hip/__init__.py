@@ -114,7 +114,7 @@ HOST_SYMBOLS = [
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
     "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field", "cupcfd_hip_mesh_generate_f64",
-    "cupcfd_hip_mesh_generate_f32",
+    "cupcfd_hip_mesh_generate_f32", "cupcfd_hip_mesh_face_colours",
     "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",
@@ -158,6 +158,7 @@ def _declare(L):
     for k in ("f64", "f32"):
         getattr(L, f"cupcfd_hip_mesh_generate_{k}").argtypes = [ct.POINTER(StructGenParams), ct.c_int32,
                                                                ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hip_mesh_face_colours.argtypes = [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int32)]
     L.cupcfd_hip_mesh_digest.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_int32)]
     L.cupcfd_hip_mesh_digest_field.argtypes = [ct.c_int32]
     L.cupcfd_hip_mesh_digest_field.restype = ct.c_char_p
@@ -374,6 +375,13 @@ class DeviceMesh:
         check("cupcfd_hip_mesh_digest", L.cupcfd_hip_mesh_digest(self._h, _ptr(out), out.size, ct.byref(n)))
         return {L.cupcfd_hip_mesh_digest_field(i).decode(): int(out[i]) for i in range(n.value)}
 
+    def face_colours(self):
+        """(colour per face in mesh numbering, number of colours) of the order-preserving face colouring."""
+        out = np.zeros(self.nFaces, np.int32)
+        n = ct.c_int32()
+        check("cupcfd_hip_mesh_face_colours", lib().cupcfd_hip_mesh_face_colours(self._h, _ptr(out), ct.byref(n)))
+        return out, n.value
+
     def set_carry_init(self, v):
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
 
@@ -385,7 +393,8 @@ class DeviceMesh:
         return [int(x) for x in out]
 
     def set_schedule(self, lanes_per_cell):
-        """0 = automatic, 1 = one thread per cell, 4, 6 or 8 lanes per cell, 64 = 64-cell tiles."""
+        """0 = automatic, 1 = one thread per cell, 2 = colour groups (gradient), 4, 6 or 8 lanes per cell,
+        64 = 64-cell tiles."""
         check("set_schedule", lib().cupcfd_hip_mesh_set_schedule(self._h, int(lanes_per_cell)))
 
     def push_state(self):

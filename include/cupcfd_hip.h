@@ -189,12 +189,25 @@ int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
 /* Gather schedule of the per-cell face loops (gradient, scalar, UVW): 64 for
  * tiles of 64 cells whose values are staged in LDS and whose faces are each
  * evaluated once; lanes evaluating one cell's faces in parallel (4, 6 or 8; a
- * row longer than that falls back to one thread per cell); or 1 for one
- * thread per cell. 0 (default) takes tiles when the mesh's tiles fit (at most
+ * row longer than that falls back to one thread per cell); 1 for one
+ * thread per cell; or 2 for colour groups (gradient only; the scalar and UVW
+ * loops then take the automatic schedule): one launch per colour of the
+ * mesh's order-preserving face colouring (cupcfd_hip_mesh_face_colours), in
+ * which each face adds its terms straight to its cells. The colouring is
+ * built on the schedule's first call, which allocates and synchronises. 0 (default) takes tiles when the mesh's tiles fit (at most
  * 32767 face slots per tile and 64 KiB of LDS), else the row length of
  * all-tet / all-hex meshes, else 8. Every schedule gives bit-identical
  * results. */
 int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
+/* The mesh's order-preserving face colouring, built on the device on first
+ * use: colour(f) = 1 + the largest colour among the faces before f (in the
+ * mesh's face order) that share a cell with it, 0 if none. So no two faces of
+ * one colour share a cell, and each cell's faces have rising colours in face
+ * order: colour-by-colour accumulation adds a cell's terms in the
+ * reference's order. colour (nFaces entries, mesh face numbering; may be
+ * NULL) and ncolours (may be NULL) receive it. Synchronous; no counterpart in
+ * the reference (its loops are sequential). */
+int cupcfd_hip_mesh_face_colours(cupcfd_hip_mesh *mesh, int32_t *colour, int32_t *ncolours);
 /* Choose the gather schedule of the gradient, scalar and UVW loops separately
  * by timing each candidate (cell tiles, the lane-per-face width the mesh
  * allows, one thread per cell) on scratch inputs on a private stream; the

@@ -153,10 +153,10 @@ def test_size_errors_are_reported():
 @pytest.mark.gpu
 @pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32", "tets_typed"])
 def test_schedules_bit_identical(meshname):
-    """One thread per cell, 6 or 8 lanes per cell and 64-cell tiles give the same bits."""
+    """One thread per cell, colour groups, 6 or 8 lanes per cell and 64-cell tiles give the same bits."""
     host, _ = _setup(meshname, 64)
     outs = {}
-    for lpc in (1, 6, 8, 64):
+    for lpc in (1, 2, 6, 8, 64):
         dev = ch.DeviceMesh(host, 64)
         dev.set_schedule(lpc)
         for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
@@ -165,8 +165,40 @@ def test_schedules_bit_identical(meshname):
             outs[(lpc, kernel)] = run(kernel, fvm, dev, f, lambda: host.state(64))[1]
     for kernel in ("gradient", "gradient3", "scalar", "scalar_t", "scalar_wall", "uvw"):
         for k in outs[(1, kernel)]:
-            for lpc in (6, 8, 64):
+            for lpc in (2, 6, 8, 64):
                 assert compare(outs[(1, kernel)][k], outs[(lpc, kernel)][k])[0], (kernel, k, lpc)
+
+
+def _greedy_colours(host):
+    """The order-preserving colouring, sequentially: faces in mesh order, colour = 1 + the last colour given to
+    either of its cells (0 if none)."""
+    c1, c2 = host.a["faceCell1"], host.a["faceCell2"]
+    last = np.full(host.nCells, -1, np.int64)
+    col = np.empty(host.nFaces, np.int32)
+    for f in range(host.nFaces):
+        a, b = c1[f], c2[f]
+        q = (last[a] if b < 0 else max(last[a], last[b])) + 1
+        col[f] = q
+        last[a] = q
+        if b >= 0:
+            last[b] = q
+    return col
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "typed32", "tets_typed", "line"])
+def test_face_colouring_is_the_sequential_greedy(meshname):
+    """The device colouring equals its sequential twin, and no two faces of one colour share a cell."""
+    spec = {"line": dict(nx=150, ny=1, nz=1, permute_faces=True, seed=6)}
+    host = ch.HostMesh(**spec[meshname]) if meshname in spec else _setup(meshname, 64)[0]
+    col, n = ch.DeviceMesh(host, 64).face_colours()
+    ref = _greedy_colours(host)
+    assert n == ref.max() + 1
+    assert np.array_equal(col, ref)
+    c1, c2 = host.a["faceCell1"], host.a["faceCell2"]
+    for q in range(n):
+        cells = np.concatenate([c1[col == q], c2[(col == q) & (c2 >= 0)]])
+        assert np.unique(cells).size == cells.size, q
 
 
 # unordered, these tiles have 342 (typed_perm), 384 (typed32), 228 (tets_typed) and 247 (tets_scattered) face
@@ -206,7 +238,8 @@ def test_autotune_keeps_bits_and_state(meshname):
     dev.push_state()
     before = host.state(64)
     chosen = dev.autotune()
-    assert len(chosen) == 3 and all(c in (64, 4, 6, 8, 1) for c in chosen), chosen
+    assert len(chosen) == 3 and all(c in (64, 4, 6, 8, 1, 2) for c in chosen), chosen
+    assert chosen[1] != 2 and chosen[2] != 2  # colour groups are a gradient schedule
     dev.pull_state()
     after = host.state(64)
     for k in before:
