@@ -1883,6 +1883,65 @@ __global__ __launch_bounds__(kBlock) void k_scalar_coef(DevMesh<T> m, ScalarArgs
     a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
 }
 
+// Batched scalar flux on meshes whose regions are all DEFAULT, with RL faces
+// in every row: a boundary entry then adds -0.0 three times to Su and touches
+// nothing else (scalar_boundary), so only interior entries are evaluated; the
+// row's entries and neighbour values are all loaded before the first term (two
+// memory round trips per cell instead of two per entry). Terms and their
+// order are k_scalar_batched's.
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_scalar_batched_plain(DevMesh<T> m, ScalarArgs<T, TS> a,
+                                                                 const ScalarFaceCoef<T> *coef, int K, int per,
+                                                                 int nblk) {
+    const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
+    if (!L.live) return;
+    const int c = batch_uniform<G>(L.c);
+    const TS *phi = a.phi + L.k, *dphi = a.dPhidx + L.k;
+    TS *sup = a.su + L.k;
+    const T phiC = (T)phi[(size_t)c * K];
+    const v3<T> gC = ld3s_as<T>(dphi, c, K);
+    T su = (T)sup[(size_t)c * K];
+    const int e0 = c * RL;
+    int2 en[RL];
+    T ophi[RL];
+    v3<T> og[RL];
+#pragma unroll
+    for (int i = 0; i < RL; i++) en[i] = m.ent[e0 + i];
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        if (en[i].y >= 0) {
+            ophi[i] = (T)phi[(size_t)en[i].y * K];
+            og[i] = ld3s_as<T>(dphi, en[i].y, K);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        if (en[i].y < 0) continue;
+        const int f = en[i].x >> 1;
+        const V4<T> s = m.fgeoS[f];
+        const ScalarFaceCoef<T> q = coef[f];
+        const bool in = (en[i].x & 1) != 0;
+        const T phiP = in ? ophi[i] : phiC, phiN = in ? phiC : ophi[i];
+        const v3<T> gP = in ? og[i] : gC, gN = in ? gC : og[i];
+        const v3<T> S{s.x, s.y, s.z};
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const v3<T> g = lerp3(gN, facn, gP, facp);
+        const T fde1 = q.visac * dot3(g, S);
+        const T fci = q.fmn * (double)phiN + q.fmx * (double)phiP;
+        const T fdi = q.visface * dot3(g, v3<T>{q.xpx, q.xpy, q.xpz});
+        const T blend = a.gammaBlend * (q.fce - fci);
+        T t0 = -blend, t1 = fde1, t2 = -fdi;
+        if (in) {
+            t0 = -t0;
+            t1 = -t1;
+            t2 = -t2;
+        }
+        su = ((su + t0) + t1) + t2;
+    }
+    sup[(size_t)c * K] = (TS)su;
+}
+
 template <class T, class TS, int G>
 __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarArgs<T, TS> a0,
                                                            const ScalarFaceCoef<T> *coef, int K, int per, int nblk) {
@@ -2154,24 +2213,22 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef(DevMesh<T> m, AdjBatchArgs<
                                Visac * s.z - VisFace * Xpn.z, (T)fmin((double)mf, 0.0), (T)fmax((double)mf, 0.0)};
 }
 
-template <class T, int G>
+// RL > 0: every row has RL entries, all loaded (with the neighbours' values)
+// before the first term; RL = 0: rows walked one entry at a time
+template <class T, int G, int RL>
 __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     const int c = batch_uniform<G>(L.c), kk = L.k;
     T acc[4] = {0, 0, 0, 0};
-    int e0, n;
-    row_range(m, c, e0, n);
-    for (int e = e0; e < e0 + n; e++) {
-        const int2 en = m.ent[e];
-        if (en.y < 0) continue;  // boundary faces add nothing (the single-scalar fold adds -0.0)
+    // boundary faces add nothing (the single-scalar fold adds -0.0)
+    auto term = [&](int2 en, T sIp, T sNn) {
         const int f = en.x >> 1;
         const bool in = (en.x & 1) != 0;
-        const int ip = in ? en.y : c, nn = in ? c : en.y;
         const AdjFaceCoef<T> q = a.coef[f];
         const T facn = m.fgeoS[f].w;
         const T facp = 1.0 - facn;
-        const T D = a.subar[(size_t)ip * K + kk] - a.subar[(size_t)nn * K + kk];
+        const T D = sIp - sNn;
         const T Dg = D * a.s.gammaBlend;
         const T fac = in ? facn : facp;
         const T tt = D * fac;
@@ -2179,34 +2236,80 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
         acc[1] = acc[1] + tt * q.wy;
         acc[2] = acc[2] + tt * q.wz;
         acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+    };
+    if (RL) {
+        const T sc = a.subar[(size_t)c * K + kk];
+        int2 en[RL > 0 ? RL : 1];
+        T so[RL > 0 ? RL : 1];
+#pragma unroll
+        for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
+#pragma unroll
+        for (int i = 0; i < RL; i++)
+            if (en[i].y >= 0) so[i] = a.subar[(size_t)en[i].y * K + kk];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            if (en[i].y < 0) continue;
+            const bool in = (en[i].x & 1) != 0;
+            term(en[i], in ? so[i] : sc, in ? sc : so[i]);
+        }
+    } else {
+        int e0, n;
+        row_range(m, c, e0, n);
+        for (int e = e0; e < e0 + n; e++) {
+            const int2 en = m.ent[e];
+            if (en.y < 0) continue;
+            const bool in = (en.x & 1) != 0;
+            const int ip = in ? en.y : c, nn = in ? c : en.y;
+            term(en, a.subar[(size_t)ip * K + kk], a.subar[(size_t)nn * K + kk]);
+        }
     }
     const T fact = 1.0 / m.cvol[c];
     for (int j = 0; j < 3; j++) a.h[((size_t)c * 3 + j) * K + kk] = acc[j] * fact;
     a.dir[(size_t)c * K + kk] = acc[3];
 }
 
-template <class T, int G>
+template <class T, int G, int RL>
 __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     const int c = batch_uniform<G>(L.c), kk = L.k;
     T acc = a.dir[(size_t)c * K + kk];
-    int e0, n;
-    row_range(m, c, e0, n);
-    for (int e = e0; e < e0 + n; e++) {
-        const int2 en = m.ent[e];
-        if (en.y < 0) continue;
+    auto term = [&](int2 en, const v3<T> &hp, const v3<T> &hn) {
         const int f = en.x >> 1;
         const bool in = (en.x & 1) != 0;
-        const int ip = in ? en.y : c, nn = in ? c : en.y;
         const V4<T> s = m.fgeoS[f];
         const T facn = s.w;
         const T facp = 1.0 - facn;
         const T fac = in ? facn : facp;
-        const v3<T> hp = ld3s(a.h + kk, ip, K), hn = ld3s(a.h + kk, nn, K);
         const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
         const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
         acc = acc + fac * pF;
+    };
+    if (RL) {
+        const v3<T> hc = ld3s(a.h + kk, c, K);
+        int2 en[RL > 0 ? RL : 1];
+        v3<T> ho[RL > 0 ? RL : 1];
+#pragma unroll
+        for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
+#pragma unroll
+        for (int i = 0; i < RL; i++)
+            if (en[i].y >= 0) ho[i] = ld3s(a.h + kk, en[i].y, K);
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            if (en[i].y < 0) continue;
+            const bool in = (en[i].x & 1) != 0;
+            term(en[i], in ? ho[i] : hc, in ? hc : ho[i]);
+        }
+    } else {
+        int e0, n;
+        row_range(m, c, e0, n);
+        for (int e = e0; e < e0 + n; e++) {
+            const int2 en = m.ent[e];
+            if (en.y < 0) continue;
+            const bool in = (en.x & 1) != 0;
+            const int ip = in ? en.y : c, nn = in ? c : en.y;
+            term(en, ld3s(a.h + kk, ip, K), ld3s(a.h + kk, nn, K));
+        }
     }
     a.phibar[(size_t)c * K + kk] = acc;
 }
@@ -2613,9 +2716,22 @@ int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, TS *PhiB, const
     const int G = batch_group(K);
     const Grid gr = batch_grid(d.C, G);
     const dim3 grid(gr.blocks, (K + G - 1) / G);
-    if (G == 4) hipLaunchKernelGGL((k_scalar_batched<T, TS, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
-    else if (G == 16) hipLaunchKernelGGL((k_scalar_batched<T, TS, 16>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
-    else hipLaunchKernelGGL((k_scalar_batched<T, TS, 64>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk);
+    // untyped meshes with uniform rows: interior entries only, whole row preloaded
+    const int RL = d.typed ? 0 : (d.rowlen == 6 ? 6 : (d.rowlen == 4 ? 4 : 0));
+#define SCALAR_BATCHED_LAUNCH(GG)                                                                                     \
+    do {                                                                                                             \
+        if (RL == 6)                                                                                                 \
+            hipLaunchKernelGGL((k_scalar_batched_plain<T, TS, GG, 6>), grid, kBlock, 0, st, d, a, coef, K, gr.per,   \
+                               gr.nblk);                                                                             \
+        else if (RL == 4)                                                                                            \
+            hipLaunchKernelGGL((k_scalar_batched_plain<T, TS, GG, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per,   \
+                               gr.nblk);                                                                             \
+        else hipLaunchKernelGGL((k_scalar_batched<T, TS, GG>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk); \
+    } while (0)
+    if (G == 4) SCALAR_BATCHED_LAUNCH(4);
+    else if (G == 16) SCALAR_BATCHED_LAUNCH(16);
+    else SCALAR_BATCHED_LAUNCH(64);
+#undef SCALAR_BATCHED_LAUNCH
     return launch_ok();
 }
 
@@ -2735,16 +2851,24 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T 
         const int G = batch_group(K);
         const Grid gr = batch_grid(d.C, G);
         const dim3 grid(gr.blocks, (K + G - 1) / G);
-        if (G == 4) {
-            hipLaunchKernelGGL((k_adj_a_batched<T, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_batched<T, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-        } else if (G == 16) {
-            hipLaunchKernelGGL((k_adj_a_batched<T, 16>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_batched<T, 16>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-        } else {
-            hipLaunchKernelGGL((k_adj_a_batched<T, 64>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_batched<T, 64>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
-        }
+        // uniform 6- or 4-face rows: each row loaded whole before its first term
+        const int RL = d.rowlen == 6 ? 6 : (d.rowlen == 4 ? 4 : 0);
+#define ADJ_BATCHED_LAUNCH(GG, RR)                                                                               \
+    do {                                                                                                          \
+        hipLaunchKernelGGL((k_adj_a_batched<T, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
+        hipLaunchKernelGGL((k_adj_b_batched<T, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
+    } while (0)
+#define ADJ_BATCHED_ROWS(GG)                                                                                     \
+    do {                                                                                                          \
+        if (RL == 6) ADJ_BATCHED_LAUNCH(GG, 6);                                                                   \
+        else if (RL == 4) ADJ_BATCHED_LAUNCH(GG, 4);                                                              \
+        else ADJ_BATCHED_LAUNCH(GG, 0);                                                                           \
+    } while (0)
+        if (G == 4) ADJ_BATCHED_ROWS(4);
+        else if (G == 16) ADJ_BATCHED_ROWS(16);
+        else ADJ_BATCHED_ROWS(64);
+#undef ADJ_BATCHED_ROWS
+#undef ADJ_BATCHED_LAUNCH
     }
     if (d.F)
         hipLaunchKernelGGL(k_adj_faces_batched<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, K);

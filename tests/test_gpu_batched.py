@@ -26,6 +26,9 @@ MESHES = {
     "typed32": dict(nx=32, typed=True, permute_cells=True, permute_faces=True, face_extras=True, seed=3),
     "tets_typed": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, typed=True, permute_cells=True,
                        permute_faces=True, face_extras=True, seed=5),
+    # one DEFAULT region: the scalar flux evaluates interior entries only (4-face rows; ref5 has 6)
+    "tets_plain": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, permute_cells=True, permute_faces=True,
+                       face_extras=True, seed=6),
 }
 _cache = {}
 
@@ -163,7 +166,7 @@ def test_gradient_batched_f32_storage_is_rounded_f64(K, meshname, nG):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "tets_typed", "typed32"])
+@pytest.mark.parametrize("meshname", ["ref5", "typed_perm", "tets_typed", "tets_plain", "typed32"])
 @pytest.mark.parametrize("K", [3, 16, 64, 70])
 @pytest.mark.parametrize("variant", ["scalar", "scalar_wall"])
 def test_scalar_batched_f32_storage_is_rounded_f64(K, meshname, variant):
