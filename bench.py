@@ -309,6 +309,16 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
     mbar = torch.empty(F, dtype=dt, device=cuda)
     tadj = timed(lambda: R.scalar_adjoint_batched(dev, K, phi_b, sh["visEff"], mf, sbar, rbar, W.SCALAR, pbar, mbar,
                                                   stream))
+    # the same with FP32 storage of phi, SuBar and PhiBar: time, and max-norm error of PhiBar and MassFluxBar
+    # against the FP64 adjoint of the FP64 inputs
+    sbar32 = sbar.float()
+    pbar32 = torch.empty((C, K), dtype=torch.float32, device=cuda)
+    mbar32 = torch.empty(F, dtype=dt, device=cuda)
+    tadj32 = timed(lambda: R.scalar_adjoint_batched_f32s(dev, K, phi32, sh["visEff"], mf, sbar32, rbar, W.SCALAR,
+                                                         pbar32, mbar32, stream))
+    torch.cuda.synchronize()
+    err_adj32 = {"PhiBar": ((pbar32.double() - pbar).abs().max() / pbar.abs().max()).item(),
+                 "MassFluxBar": ((mbar32 - mbar).abs().max() / mbar.abs().max()).item()}
     tbs = timed(lambda: R.scalar_batched(dev, K, phi_b, pB_b, sh["visEff"], au_b, su_b, sh["den"], mf, sh["te"],
                                          sh["cpB"], sh["visEffB"], rface, g_b, W.SCALAR, stream))
     # FP32-storage scalar flux (per-scalar arrays float, FP64 arithmetic) on the FP32-storage gradient: time, and
@@ -338,6 +348,7 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
         "batched_scalar_f32_storage_ms": tbs32 * 1e3,
         "f32_storage_scalar_rel_err": {"Su": err_s32["su"], "Au": err_s32["au"], "PhiB": err_s32["pB"]},
         "batched_adjoint_ms": tadj * 1e3, "batched_forward_adjoint_ratio": (tb + tadj) / tb,
+        "batched_adjoint_f32_storage_ms": tadj32 * 1e3, "f32_storage_adjoint_rel_err": err_adj32,
     }
 
 

@@ -2188,15 +2188,20 @@ template <class T> struct AdjFaceCoef {
     T wx, wy, wz, fmn, fmx;
 };
 
-template <class T> struct AdjBatchArgs {
-    ScalarArgs<T> s;  // phi ([c*K + k]), visEff, massFlux and the diffusivity parameters
-    const T *subar, *rbar;
-    T *h, *dir, *phibar, *mfbar;
+// TS: storage type of the per-scalar arrays (phi, SuBar, PhiBar); the shared
+// arrays, the scratch fields h and dir, and all arithmetic are T
+template <class T, class TS = T> struct AdjBatchArgs {
+    ScalarArgs<T, TS> s;  // phi ([c*K + k]), visEff, massFlux and the diffusivity parameters
+    const TS *subar;
+    const T *rbar;
+    T *h, *dir;
+    TS *phibar;
+    T *mfbar;
     AdjFaceCoef<T> *coef;
 };
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_coef(DevMesh<T> m, AdjBatchArgs<T> a) {
+template <class T, class TS>
+__global__ __launch_bounds__(kBlock) void k_adj_coef(DevMesh<T> m, AdjBatchArgs<T, TS> a) {
     const int f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= m.Fi) return;
     const int2 cn = m.fconn[f];
@@ -2215,8 +2220,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_coef(DevMesh<T> m, AdjBatchArgs<
 
 // RL > 0: every row has RL entries, all loaded (with the neighbours' values)
 // before the first term; RL = 0: rows walked one entry at a time
-template <class T, int G, int RL>
-__global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
+                                                          int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     const int c = batch_uniform<G>(L.c), kk = L.k;
@@ -2238,14 +2244,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
         acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
     };
     if (RL) {
-        const T sc = a.subar[(size_t)c * K + kk];
+        const T sc = (T)a.subar[(size_t)c * K + kk];
         int2 en[RL > 0 ? RL : 1];
         T so[RL > 0 ? RL : 1];
 #pragma unroll
         for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
 #pragma unroll
         for (int i = 0; i < RL; i++)
-            if (en[i].y >= 0) so[i] = a.subar[(size_t)en[i].y * K + kk];
+            if (en[i].y >= 0) so[i] = (T)a.subar[(size_t)en[i].y * K + kk];
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             if (en[i].y < 0) continue;
@@ -2260,7 +2266,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
             if (en.y < 0) continue;
             const bool in = (en.x & 1) != 0;
             const int ip = in ? en.y : c, nn = in ? c : en.y;
-            term(en, a.subar[(size_t)ip * K + kk], a.subar[(size_t)nn * K + kk]);
+            term(en, (T)a.subar[(size_t)ip * K + kk], (T)a.subar[(size_t)nn * K + kk]);
         }
     }
     const T fact = 1.0 / m.cvol[c];
@@ -2268,8 +2274,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
     a.dir[(size_t)c * K + kk] = acc[3];
 }
 
-template <class T, int G, int RL>
-__global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K, int per, int nblk) {
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
+                                                          int nblk) {
     const BatchLane L = batch_lane<T, G>(m, K, per, nblk);
     if (!L.live) return;
     const int c = batch_uniform<G>(L.c), kk = L.k;
@@ -2311,12 +2318,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
             term(en, ld3s(a.h + kk, ip, K), ld3s(a.h + kk, nn, K));
         }
     }
-    a.phibar[(size_t)c * K + kk] = acc;
+    a.phibar[(size_t)c * K + kk] = (TS)acc;
 }
 
 // one thread per face, adding its K scalars' terms in order after the RFace term
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_faces_batched(DevMesh<T> m, AdjBatchArgs<T> a, int K) {
+template <class T, class TS>
+__global__ __launch_bounds__(kBlock) void k_adj_faces_batched(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K) {
     const int f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= m.F) return;
     const int2 cn = m.fconn[f];
@@ -2327,13 +2334,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces_batched(DevMesh<T> m, AdjB
     const T mf = a.s.massFlux[f];
     const T PhiFace = m.carry_init;
     T acc = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
-    const T *sp = a.subar + (size_t)cn.x * K, *sn = a.subar + (size_t)cn.y * K;
-    const T *pu = mf < 0 ? a.s.phi + (size_t)cn.y * K : a.s.phi + (size_t)cn.x * K;  // the upwind cell's phi
+    const TS *sp = a.subar + (size_t)cn.x * K, *sn = a.subar + (size_t)cn.y * K;
+    const TS *pu = mf < 0 ? a.s.phi + (size_t)cn.y * K : a.s.phi + (size_t)cn.x * K;  // the upwind cell's phi
     const bool up = mf < 0 || mf > 0;
     for (int kk = 0; kk < K; kk++) {
-        const T D = sp[kk] - sn[kk];
+        const T D = (T)sp[kk] - (T)sn[kk];
         const T Dg = D * a.s.gammaBlend;
-        const T dfci = up ? pu[kk] : (T)0.0;
+        const T dfci = up ? (T)pu[kk] : (T)0.0;
         acc = acc + (-(Dg * (PhiFace - dfci)));
     }
     a.mfbar[f] = acc;
@@ -2822,18 +2829,18 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
     return launch_ok();
 }
 
-template <class T>
-int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T *VisEff, const T *MF, const T *SuBar,
-                               const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl,
-                               int VarTE, T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend,
-                               T *PhiBar, T *MFBar, void *stream) {
+template <class T, class TS = T>
+int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T *VisEff, const T *MF,
+                               const TS *SuBar, const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT,
+                               T Sigma_T, T Prandtl, int VarTE, T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt,
+                               T GammaBlend, TS *PhiBar, T *MFBar, void *stream) {
     Mesh<T> *M = get<T>(h);
     if (!M) return CUPCFD_E_NULL_PTR;
     if (K < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
     const DevMesh<T> &d = M->d;
     if (d.typed) return CUPCFD_E_NOT_IMPLEMENTED;
-    AdjBatchArgs<T> a{};
-    const int rc = scalar_args<T, T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
+    AdjBatchArgs<T, TS> a{};
+    const int rc = scalar_args<T, TS>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, nullptr, MF, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, SolveTurb, 0, vislam, ivar, VarT, Sigma_T, Prandtl, VarTE,
                                   Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, (T)0, (T)1);
     if (rc != CUPCFD_E_SUCCESS) return rc;
@@ -2847,7 +2854,7 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T 
         a.dir = (T *)M->ws_get(50, (size_t)d.C * K * sizeof(T));
         a.coef = (AdjFaceCoef<T> *)M->ws_get(51, (size_t)(d.Fi ? d.Fi : 1) * sizeof(AdjFaceCoef<T>));
         if (!a.h || !a.dir || !a.coef) return CUPCFD_E_ERROR;
-        if (d.Fi) hipLaunchKernelGGL(k_adj_coef<T>, dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a);
+        if (d.Fi) hipLaunchKernelGGL((k_adj_coef<T, TS>), dim3(blocks(d.Fi)), dim3(kBlock), 0, st, d, a);
         const int G = batch_group(K);
         const Grid gr = batch_grid(d.C, G);
         const dim3 grid(gr.blocks, (K + G - 1) / G);
@@ -2855,8 +2862,8 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T 
         const int RL = d.rowlen == 6 ? 6 : (d.rowlen == 4 ? 4 : 0);
 #define ADJ_BATCHED_LAUNCH(GG, RR)                                                                               \
     do {                                                                                                          \
-        hipLaunchKernelGGL((k_adj_a_batched<T, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
-        hipLaunchKernelGGL((k_adj_b_batched<T, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
+        hipLaunchKernelGGL((k_adj_a_batched<T, TS, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
+        hipLaunchKernelGGL((k_adj_b_batched<T, TS, GG, RR>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);         \
     } while (0)
 #define ADJ_BATCHED_ROWS(GG)                                                                                     \
     do {                                                                                                          \
@@ -2871,7 +2878,7 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const T *Phi, const T 
 #undef ADJ_BATCHED_LAUNCH
     }
     if (d.F)
-        hipLaunchKernelGGL(k_adj_faces_batched<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, K);
+        hipLaunchKernelGGL((k_adj_faces_batched<T, TS>), dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, K);
     return launch_ok();
 }
 
@@ -3362,6 +3369,19 @@ int cupcfd_hip_mesh_face_colours(cupcfd_hip_mesh *mesh, int32_t *colour, int32_t
         return CUPCFD_E_SUCCESS;
     };
     return mesh->precision == 64 ? fetch(mesh->m64) : fetch(mesh->m32);
+}
+
+int cupcfd_dev_scalar_adjoint_batched_f32s(cupcfd_hip_mesh *mesh, int32_t K, const float *PhiCell,
+                                            const double *VisEff, const double *MassFlux, const float *SuBar,
+                                            const double *RFaceBar, int32_t SolveTurb, double vislam, int32_t ivar,
+                                            int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
+                                            double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s,
+                                            double Schmidt, double GammaBlend, float *PhiBar, double *MassFluxBar,
+                                            void *stream) {
+    return dev_scalar_adjoint_batched<double, float>(mesh, K, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb,
+                                                     vislam, ivar, VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED,
+                                                     Sigma_e, Sigma_s, Schmidt, GammaBlend, PhiBar, MassFluxBar,
+                                                     stream);
 }
 
 int cupcfd_hip_meshset_create(cupcfd_hip_mesh *const *meshes, int32_t n, cupcfd_hip_meshset **out) {

@@ -313,3 +313,17 @@ class resident:
             _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
             _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
             t(mfbar), cls._s(stream)))
+
+    @classmethod
+    def scalar_adjoint_batched_f32s(cls, mesh, K, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None):
+        """Batched adjoint with float32 storage of phi, subar and phibar (the rest float64), on a 64-bit mesh."""
+        from . import lib, check
+        assert mesh.precision == 64
+        t = cls._t
+        p = params
+        D = ct.c_double
+        check("cupcfd_dev_scalar_adjoint_batched_f32s", lib().cupcfd_dev_scalar_adjoint_batched_f32s(
+            mesh.handle, _I(K), t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
+            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
+            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
+            t(mfbar), cls._s(stream)))

@@ -460,6 +460,17 @@ int cupcfd_dev_scalar_adjoint_batched_f64(cupcfd_hip_mesh *mesh, int32_t K, cons
     int32_t SolveTurb, double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl,
     int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
     double GammaBlend, double *PhiBar, double *MassFluxBar, void *stream);
+/* The batched adjoint with FP32 storage of the per-scalar arrays (PhiCell,
+ * SuBar, PhiBar float, laid out as in cupcfd_dev_scalar_adjoint_batched_f64);
+ * VisEff, MassFlux, RFaceBar, MassFluxBar and all arithmetic are double. The
+ * result is cupcfd_dev_scalar_adjoint_batched_f64's on the inputs widened to
+ * double, with PhiBar rounded to float once (MassFluxBar exactly). No
+ * counterpart in the reference. */
+int cupcfd_dev_scalar_adjoint_batched_f32s(cupcfd_hip_mesh *mesh, int32_t K, const float *PhiCell,
+    const double *VisEff, const double *MassFlux, const float *SuBar, const double *RFaceBar,
+    int32_t SolveTurb, double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl,
+    int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
+    double GammaBlend, float *PhiBar, double *MassFluxBar, void *stream);
 int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
     const double *MassFlux, const double *SuBar, const double *RFaceBar, int32_t SolveTurb,
     double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,

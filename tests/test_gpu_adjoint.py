@@ -219,3 +219,29 @@ def test_batched_adjoint_matches_single(name, K):
         assert torch.equal(pb[:, k].contiguous(), p1), k
         want = want + m1.cpu().numpy()
     assert np.array_equal(mb.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("K", [3, 16, 70])
+@pytest.mark.parametrize("name", ["perm12", "tets"])
+def test_batched_adjoint_f32_storage_is_rounded_f64(name, K):
+    """FP32 storage of phi, SuBar and PhiBar: PhiBar is the FP64 batched adjoint on the widened inputs, rounded
+    once; MassFluxBar is that call's exactly."""
+    c = case(name)
+    rng = np.random.default_rng(70 + K)
+    phi = rng.uniform(-1, 1, (c.C, K)).astype(np.float32)
+    sb = rng.normal(size=(c.C, K)).astype(np.float32)
+    rb = rng.normal(size=2 * c.F)
+    mf, vis = c.faces(c.f.massFlux), c.cells(c.f.visEff)
+    out = []
+    for dt in (torch.float32, torch.float64):
+        pb = torch.empty((c.C, K), dtype=dt, device="cuda")
+        mb = torch.empty(c.F, dtype=torch.float64, device="cuda")
+        ph, s_ = c.cells(phi.astype(np.float64)).to(dt), c.cells(sb.astype(np.float64)).to(dt)
+        call = R.scalar_adjoint_batched_f32s if dt == torch.float32 else R.scalar_adjoint_batched
+        call(c.dev, K, ph.contiguous(), vis, mf, s_.contiguous(), c.faces(rb, 2), P, pb, mb)
+        torch.cuda.synchronize()
+        out.append((pb, mb))
+    (pb32, mb32), (pb64, mb64) = out
+    assert torch.equal(pb32, pb64.to(torch.float32))
+    assert torch.equal(mb32, mb64)

@@ -101,3 +101,23 @@ def test_adjoint_batched_mixed_rows(K):
         assert torch.equal(pb[:, k], p1), k
         want = want + m1.cpu().numpy()
     assert np.array_equal(mb.cpu().numpy(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("K", [3, 70])
+def test_adjoint_batched_f32_storage_mixed_rows(K):
+    dev = _mesh()
+    C, F = dev.nCells, dev.nFaces
+    rng = np.random.default_rng(300 + K)
+    phi = rng.uniform(-1, 1, (C, K)).astype(np.float32)
+    sb = rng.normal(size=(C, K)).astype(np.float32)
+    rb, mf, vis = _cuda(rng.normal(size=2 * F)), _cuda(rng.uniform(-1, 1, F)), _cuda(rng.uniform(1e-3, 2e-3, C))
+    pb32 = torch.empty((C, K), dtype=torch.float32, device="cuda")
+    pb64 = torch.empty((C, K), dtype=torch.float64, device="cuda")
+    mb32 = torch.empty(F, dtype=torch.float64, device="cuda")
+    mb64 = torch.empty_like(mb32)
+    R.scalar_adjoint_batched_f32s(dev, K, _cuda(phi), vis, mf, _cuda(sb), rb, W.SCALAR, pb32, mb32)
+    R.scalar_adjoint_batched(dev, K, _cuda(phi.astype(np.float64)), vis, mf, _cuda(sb.astype(np.float64)), rb,
+                             W.SCALAR, pb64, mb64)
+    torch.cuda.synchronize()
+    assert torch.equal(pb32, pb64.to(torch.float32)) and torch.equal(mb32, mb64)
