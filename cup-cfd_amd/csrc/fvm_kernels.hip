@@ -57,6 +57,13 @@ template <class T> __device__ __forceinline__ void st3(T *__restrict__ p, int i,
 }
 template <class T> __device__ __forceinline__ v3<T> xyz(const V4<T> &a) { return v3<T>{a.x, a.y, a.z}; }
 
+// Per-face outputs written once per sweep (RFace, the mass flux) and a tile's
+// final cell sums: non-temporal stores, which keep them from evicting the cell
+// values that neighbouring faces and tiles read again from L2. (Non-temporal
+// loads of the face records were measured slower, and so were non-temporal
+// stores of the gradient, whose outputs the next kernels read back.)
+template <class T> __device__ __forceinline__ void st_stream(T *p, T v) { __builtin_nontemporal_store(v, p); }
+
 // EuclideanVector::dotProduct (EuclideanVector.cpp:83-90): from 0.0, in order
 template <class T> __device__ __forceinline__ T dot3(v3<T> a, v3<T> b) {
     T d = 0.0;
@@ -515,9 +522,9 @@ template <class T> __device__ void mass_interior(const DevMesh<T> &m, const Mass
     const T dpy = (gp_in.y * facn + gp_ip.y * facp) * xpn.y;
     const T dpz = (gp_in.z * facn + gp_ip.z * facp) * xpn.z;
     const T fact = apv;
-    a.rface[2 * (size_t)f] = -fact;
-    a.rface[2 * (size_t)f + 1] = -fact;
-    a.massFlux[f] = mf - fact * ((pin - pip) - dpx - dpy - dpz);
+    st_stream(a.rface + 2 * (size_t)f, -fact);
+    st_stream(a.rface + 2 * (size_t)f + 1, -fact);
+    st_stream(a.massFlux + f, mf - fact * ((pin - pip) - dpx - dpy - dpz));
 }
 
 // boundary faces of one owning cell, in face order (MassKernels.ipp:197-305)
@@ -812,8 +819,8 @@ __device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const Scala
     t[1] = fde1;
     t[2] = -fdi;
     if (owner) {
-        a.rface[2 * (size_t)f] = -VisFace - fmax((double)mf, 0.0);
-        a.rface[2 * (size_t)f + 1] = -VisFace + fmin((double)mf, 0.0);
+        st_stream(a.rface + 2 * (size_t)f, (T)(-VisFace - fmax((double)mf, 0.0)));
+        st_stream(a.rface + 2 * (size_t)f + 1, (T)(-VisFace + fmin((double)mf, 0.0)));
     }
 }
 
@@ -1059,7 +1066,7 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
             su = ((su + t0) + t1) + t2;
             any |= lf[sl] != 0;
         }
-        a.su[c] = su;
+        st_stream(a.su + c, su);
         if (any) {
             T au = a.au[c];
             for (int e = e0; e < e0 + n; e++) {
@@ -1158,8 +1165,8 @@ __device__ __forceinline__ void uvw_interior(const DevMesh<T> &m, const UvwArgs<
     t[3] = -blendV, t[4] = fvde, t[5] = -fvdi;
     t[6] = -blendW, t[7] = fwde, t[8] = -fwdi;
     if (owner) {
-        a.rface[2 * (size_t)f] = -visFace - smax(mf, (T)(0.0));
-        a.rface[2 * (size_t)f + 1] = -visFace + smin(mf, (T)(0.0));
+        st_stream(a.rface + 2 * (size_t)f, -visFace - smax(mf, (T)(0.0)));
+        st_stream(a.rface + 2 * (size_t)f + 1, -visFace + smin(mf, (T)(0.0)));
     }
 }
 
@@ -1474,7 +1481,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
             acc = ((acc + t0) + t1) + t2;
             any |= lf[sl] != 0;
         }
-        dst[c] = acc;
+        st_stream(dst + c, acc);
         if (any) {
             T *dsta = k == 0 ? a.au : (k == 1 ? a.av : a.aw);
             const T *l9 = lx + TC + 9 * W;
