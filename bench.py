@@ -80,6 +80,20 @@ def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
     return 2 * nbytes / t / 1e9
 
 
+def stream_read_gbs(torch, nbytes=1 << 30, reps=10):
+    """Read-only streaming rate: a sum over nbytes (torch's reduction, one read of every byte)."""
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    a.sum()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        a.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    return nbytes / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
+
+
 def cpu_baseline(host, fields, budget_s=20.0):
     """Time the oracle (one core) on the same mesh and inputs: one sweep of
     the four kernels, repeated until about budget_s seconds of CPU work."""
@@ -651,6 +665,7 @@ def main():
     kern = kernel_report(C, F, Fi, B, per)
     dom = max(per, key=per.get)
     copy_gbs = stream_copy_gbs(torch)
+    read_gbs = stream_read_gbs(torch)
 
     traffic = None
     # HBM bytes per launch of the dominant kernel, from the committed
@@ -681,6 +696,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["algorithmic_GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kern[dom]["hbm_frac"], "traffic": traffic},
         "stream_copy_GBps": copy_gbs,
+        "stream_read_GBps": read_gbs,
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
