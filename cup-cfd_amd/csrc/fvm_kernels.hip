@@ -22,7 +22,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "devmesh.hpp"
@@ -3430,9 +3435,95 @@ int cupcfd_dev_gradient_batched_set_f32(cupcfd_hip_meshset *set, int32_t K, cons
     return dev_gradient_batched_set<float>(set, K, phiCell, phiBoundary, dPhidxCell, dPhidxoCell, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Autotuned schedules, cached per (GPU architecture, precision, mesh class)
+// for the process and, when CUPCFD_HIP_TUNE_CACHE names a file, across
+// processes: one "key gradient scalar uvw" line per entry. The mesh class is
+// what the choice depends on: row lengths, whether tiles apply and at which
+// LDS width, and the cell count to the nearest power of two.
+std::mutex tune_mutex;
+std::map<std::string, std::vector<int>> tune_cache;
+
+template <class T> std::string tune_key(const Mesh<T> &M) {
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    std::string arch(prop.gcnArchName);
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    const DevMesh<T> &d = M.d;
+    int lg = 0;
+    while (lg < 31 && ((int64_t)1 << (lg + 1)) <= (int64_t)d.C) lg++;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s|p%d|rl%d|mr%d|tiles%d|tw%d|typed%d|c2^%d", arch.c_str(),
+                  (int)(8 * sizeof(T)), d.rowlen, d.maxrow, d.ntiles > 0 ? 1 : 0, tile_width(d), d.typed, lg);
+    return buf;
+}
+
+void tune_load(const char *path) {
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return;
+    char key[160];
+    int g, sc, u;
+    while (std::fscanf(fp, "%159s %d %d %d", key, &g, &sc, &u) == 4) tune_cache[key] = {g, sc, u};
+    std::fclose(fp);
+}
+
+void tune_save(const char *path) {
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *fp = std::fopen(tmp.c_str(), "w");
+    if (!fp) return;
+    for (const auto &e : tune_cache) std::fprintf(fp, "%s %d %d %d\n", e.first.c_str(), e.second[0], e.second[1], e.second[2]);
+    if (std::fclose(fp) == 0) (void)std::rename(tmp.c_str(), path);
+}
+
+template <class T> int autotune_cached(cupcfd_hip_mesh *h, int32_t *chosen) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const std::string key = tune_key(*M);
+    const char *path = std::getenv("CUPCFD_HIP_TUNE_CACHE");
+    {
+        std::lock_guard<std::mutex> lock(tune_mutex);
+        if (path && *path) tune_load(path);
+        auto it = tune_cache.find(key);
+        if (it != tune_cache.end()) {
+            for (int k = 0; k < 3; k++) {
+                M->schedule_k[k] = it->second[k];
+                if (chosen) chosen[k] = it->second[k];
+            }
+            return CUPCFD_E_SUCCESS;
+        }
+    }
+    int32_t pick[3];
+    const int rc = autotune<T>(h, pick);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    if (chosen)
+        for (int k = 0; k < 3; k++) chosen[k] = pick[k];
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    tune_cache[key] = {pick[0], pick[1], pick[2]};
+    if (path && *path) tune_save(path);
+    return CUPCFD_E_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
-    return mesh->precision == 64 ? autotune<double>(mesh, schedules) : autotune<float>(mesh, schedules);
+    return mesh->precision == 64 ? autotune_cached<double>(mesh, schedules) : autotune_cached<float>(mesh, schedules);
+}
+
+// Drop the process's autotune cache (the next cupcfd_hip_mesh_autotune of
+// each mesh class times again, or reads CUPCFD_HIP_TUNE_CACHE).
+int cupcfd_hip_autotune_forget(void) {
+    std::lock_guard<std::mutex> lock(tune_mutex);
+    tune_cache.clear();
+    return CUPCFD_E_SUCCESS;
 }
 
 const char *cupcfd_hip_version(void) { return "cupcfd_hip 0.1 gfx950"; }

@@ -114,7 +114,7 @@ HOST_SYMBOLS = [
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
     "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field", "cupcfd_hip_mesh_generate_f64",
-    "cupcfd_hip_mesh_generate_f32", "cupcfd_hip_mesh_face_colours",
+    "cupcfd_hip_mesh_generate_f32", "cupcfd_hip_mesh_face_colours", "cupcfd_hip_autotune_forget",
     "cupcfd_hip_mesh_push_state",
     "cupcfd_hip_mesh_pull_state", "cupcfd_dev_cells_to_internal", "cupcfd_dev_cells_from_internal",
     "cupcfd_dev_faces_to_internal", "cupcfd_dev_faces_from_internal", "cupcfd_hip_version",

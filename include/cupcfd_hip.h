@@ -210,11 +210,20 @@ int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
 int cupcfd_hip_mesh_face_colours(cupcfd_hip_mesh *mesh, int32_t *colour, int32_t *ncolours);
 /* Choose the gather schedule of the gradient, scalar and UVW loops separately
  * by timing each candidate (cell tiles, the lane-per-face width the mesh
- * allows, one thread per cell) on scratch inputs on a private stream; the
+ * allows, one thread per cell, and colour groups for the gradient) on scratch
+ * inputs on a private stream; the
  * mesh's boundary and region state is left as it was. schedules (may be NULL)
  * receives the three choices. A later cupcfd_hip_mesh_set_schedule replaces
- * them. Synchronous; no counterpart in the reference. */
+ * them. The choice is cached per (GPU architecture, precision, mesh class:
+ * row lengths, tile applicability and LDS width, typed regions, cell count
+ * to the nearest power of two), so a later call on a mesh of the same class
+ * takes it without timing; with the environment variable
+ * CUPCFD_HIP_TUNE_CACHE naming a file, the cache is read from and written to
+ * that file ("key gradient scalar uvw" per line), so it also persists across
+ * processes. Synchronous; no counterpart in the reference. */
 int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules);
+/* Forget the process's autotune cache (the file, if any, is left alone). */
+int cupcfd_hip_autotune_forget(void);
 /* copy mutable mesh state view → device (push) or device → view (pull) */
 int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh);
 int cupcfd_hip_mesh_pull_state(cupcfd_hip_mesh *mesh);

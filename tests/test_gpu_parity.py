@@ -228,6 +228,32 @@ def test_tiles_of_scattered_cells(meshname):
 
 
 @pytest.mark.gpu
+def test_autotune_cache_per_mesh_class(tmp_path, monkeypatch):
+    """Autotune results are kept per (architecture, precision, mesh class) and, with CUPCFD_HIP_TUNE_CACHE set,
+    in that file: a mesh of the same class takes the cached choice, and a fresh cache reads the file."""
+    L = ch.lib()
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("CUPCFD_HIP_TUNE_CACHE", str(path))
+    L.cupcfd_hip_autotune_forget()
+    try:
+        a = ch.DeviceMesh(ch.HostMesh(12, permute_cells=True, seed=1), 64)
+        ca = a.autotune()
+        lines = [ln.split() for ln in path.read_text().splitlines() if ln.strip()]
+        assert len(lines) == 1 and [int(x) for x in lines[0][1:]] == ca
+        b = ch.DeviceMesh(ch.HostMesh(12, permute_cells=True, permute_faces=True, seed=2), 64)
+        assert b.autotune() == ca
+        assert len([ln for ln in path.read_text().splitlines() if ln.strip()]) == 1
+        # a cache entry written by another process is what the next call takes
+        path.write_text(f"{lines[0][0]} 1 1 1\n")
+        L.cupcfd_hip_autotune_forget()
+        c = ch.DeviceMesh(ch.HostMesh(12, seed=3), 64)
+        assert c.autotune() == [1, 1, 1]
+    finally:
+        monkeypatch.delenv("CUPCFD_HIP_TUNE_CACHE")
+        L.cupcfd_hip_autotune_forget()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("meshname", ["typed_perm", "typed32", "tets_typed"])
 def test_autotune_keeps_bits_and_state(meshname):
     """cupcfd_hip_mesh_autotune picks one valid schedule per gather kernel, leaves the mesh state as it was,
