@@ -13,7 +13,9 @@
 //
 // The device mesh for a host mesh is built on first use and cached by the
 // mesh's address; cupcfd_hip::shim::forget(mesh) drops it (call it after
-// CupCfdSoAMesh::finalize changes the mesh). Kernels that set mesh state
+// CupCfdSoAMesh::finalize changes the mesh). The build runs on the GPU
+// (cupcfd_hip_mesh_create_device_*) unless cupcfd_hip::shim::device_repack()
+// is set to false, which selects the host repack; both give identical meshes. Kernels that set mesh state
 // (boundary shear, Q/H/T, y+/u+, region yLog and force tangent) write it
 // back into the mesh's vectors before returning, as the reference does.
 #pragma once
@@ -33,15 +35,17 @@ template <class T> struct Abi;
 template <> struct Abi<double> {
     using V3 = cupcfd_v3_f64;
     using View = cupcfd_mesh_view_f64;
-    static int create(const View *v, int32_t flags, cupcfd_hip_mesh **out) {
-        return cupcfd_hip_mesh_create_f64(v, flags, out);
+    static int create(const View *v, int32_t flags, bool on_device, cupcfd_hip_mesh **out) {
+        return on_device ? cupcfd_hip_mesh_create_device_f64(v, flags, out)
+                         : cupcfd_hip_mesh_create_f64(v, flags, out);
     }
 };
 template <> struct Abi<float> {
     using V3 = cupcfd_v3_f32;
     using View = cupcfd_mesh_view_f32;
-    static int create(const View *v, int32_t flags, cupcfd_hip_mesh **out) {
-        return cupcfd_hip_mesh_create_f32(v, flags, out);
+    static int create(const View *v, int32_t flags, bool on_device, cupcfd_hip_mesh **out) {
+        return on_device ? cupcfd_hip_mesh_create_device_f32(v, flags, out)
+                         : cupcfd_hip_mesh_create_f32(v, flags, out);
     }
 };
 
@@ -75,6 +79,12 @@ inline std::map<const void *, std::unique_ptr<Entry>> &cache() {
 }
 
 inline void forget(const void *mesh) { cache().erase(mesh); }
+
+// true (the default): device meshes are repacked on the GPU; false: on the host
+inline bool &device_repack() {
+    static bool on = true;
+    return on;
+}
 
 // Device mesh of `mesh` (built and cached on first use); nullptr on failure.
 template <class T, class Mesh> cupcfd_hip_mesh *device(Mesh &mesh, int *rc_out = nullptr,
@@ -133,7 +143,7 @@ template <class T, class Mesh> cupcfd_hip_mesh *device(Mesh &mesh, int *rc_out =
     v.regionUVW = v3<T>(mesh.regionUVW);
     static_assert(sizeof(mesh.faceCell1ID[0]) == 4, "int32 ids expected (I = L = int)");
     static_assert(sizeof(mesh.regionType[0]) == 4, "RType stored as a 4-byte enum");
-    int rc = Abi<T>::create(&v, flags, &e->dev);
+    int rc = Abi<T>::create(&v, flags, device_repack(), &e->dev);
     if (rc_out) *rc_out = rc;
     if (rc != CUPCFD_E_SUCCESS) {
         e->dev = nullptr;

@@ -62,3 +62,33 @@ def test_driver_graph_mode(tmp_path, precision):
         g = b["graph"]
         assert g["ok"] and g["replay_matches_eager"], b
         assert g["graph_ms_per_repetition"] > 0 and g["eager_ms_per_repetition"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_driver_mesh_build_host_and_device(tmp_path, precision):
+    """"MeshBuild": the device mesh is repacked on the GPU by default and on the host with "host"; both builds
+    hash to the same mesh digest, the kernels run without failures on either, and the repack time is reported
+    apart from the kernels'."""
+    reps = {}
+    for build in ("host", "device"):
+        cfg = json.load(open(CFG))
+        cfg["DataTypes"]["FloatDataType"] = precision
+        for b in cfg["Benchmarks"]:
+            if "BenchmarkKernels" in b:
+                b["BenchmarkKernels"]["Repetitions"] = 2
+                if build == "host":
+                    b["BenchmarkKernels"]["MeshBuild"] = "host"
+        p = tmp_path / f"cupcfd_{build}.json"
+        p.write_text(json.dumps(cfg))
+        r = subprocess.run([BIN, str(p)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        reps[build] = json.loads(r.stdout)["benchmarks"]
+    for h, d in zip(reps["host"], reps["device"]):
+        assert h["BenchmarkName"] == d["BenchmarkName"]
+        assert (h["mesh_build"], d["mesh_build"]) == ("host", "device")
+        assert h["mesh_digest"] == d["mesh_digest"], h["BenchmarkName"]
+        assert h["failures"] == 0 and d["failures"] == 0
+        assert h["mesh_setup_s"] > 0 and d["mesh_setup_s"] > 0
+    # the plain and the typed relabelled meshes differ
+    assert reps["device"][0]["mesh_digest"] != reps["device"][1]["mesh_digest"]
