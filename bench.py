@@ -29,6 +29,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
        [--batched-only] [--tets-only] [--schedule S]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -65,7 +66,15 @@ def algorithmic_bytes(C, F, Fi, B, s):
     }
 
 
+def quiesce():
+    """Collect Python garbage now, before a timed region. main() keeps the collector off otherwise: a
+    generation-2 collection in this process takes 20-50 ms, and one that lands inside a timed region leaves
+    the GPU idle behind the host, which the region's events then count as kernel time."""
+    gc.collect()
+
+
 def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
+    quiesce()
     a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
@@ -82,6 +91,7 @@ def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
 
 def stream_read_gbs(torch, nbytes=1 << 30, reps=10):
     """Read-only streaming rate: a sum over nbytes (torch's reduction, one read of every byte)."""
+    quiesce()
     a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     a.sum()
     torch.cuda.synchronize()
@@ -205,6 +215,7 @@ def make_sweep(torch, host, dev, f):
 
 def time_sweep(torch, kernels, steps, warmup):
     """Mean seconds per kernel and per sweep over ``steps`` timed sweeps (HIP events on the launch stream)."""
+    quiesce()
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         for _, k in kernels:
@@ -289,6 +300,7 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
                      sh["visEffB"], rface, o["g"], W.SCALAR, stream)
 
     def timed(fn):
+        quiesce()
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -406,6 +418,7 @@ def bench_adjoint(torch, host, dev, steps, warmup, label):
         R.scalar_adjoint(dev, phi, visEff, mf, sb, rb, W.SCALAR, pbar, mbar, stream)
 
     def timed(fn):
+        quiesce()
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -468,6 +481,7 @@ def bench_partitions(torch, nparts, K, steps, warmup):
         R.gradient_batched_set(mset, K, pp, pb, pg, pgo)
 
     def timed(fn):
+        quiesce()
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -628,6 +642,8 @@ def main():
 
     import torch
 
+    # no automatic collections from here on; quiesce() collects before each timed region
+    gc.disable()
     import cupcfd_hip as ch
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
