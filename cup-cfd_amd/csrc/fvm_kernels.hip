@@ -1005,6 +1005,7 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
     stage3(lx + 2 * W, a.dPhidx, c0, nc, W);
     stage_centre(lx + 5 * W, m.cgeo, c0, nc, W);
     const int eb = stage_eslots(m, le, c0, nc);
+    const T su0 = tid < nc ? a.su[c0 + tid] : (T)0;  // phase C's starting value
     int f = 0, out = -1;
     int2 cn = make_int2(0, 0);
     bool owned;
@@ -1054,7 +1055,7 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
     // C: Su, then Au for cells with an Au-updating face
     if (tid < nc) {
         const int c = c0 + tid;
-        T su = a.su[c];
+        T su = su0;
         int e0, n;
         row_range(m, c, e0, n);
         const T *l0 = lx + TC, *l1 = l0 + W, *l2 = l1 + W, *l3 = l2 + W;
@@ -1409,6 +1410,11 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
     stage3(lx + 10 * W, a.dwdx, c0, nc, W);
     stage_centre(lx + 13 * W, m.cgeo, c0, nc, W);
     const int eb = stage_eslots(m, le, c0, nc);
+    // phase C's fold item tid < 3 * TC: component k of cell cl, and its starting value
+    const int k = tid / TC, cl = tid % TC;
+    const bool fold = tid < 3 * TC && cl < nc;
+    T *const dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
+    const T acc0 = fold ? dst[c0 + cl] : (T)0;
     int f = 0, out = -1;
     int2 cn = make_int2(0, 0);
     bool owned;
@@ -1463,13 +1469,10 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
         lf[sl] = fl ? 1 : 0;
     }
     __syncthreads();
-    // C: fold item tid < 3 * TC, component k of cell cl: su/sv/sw, then
-    // au/av/aw for cells with an a-updating face
-    const int k = tid / TC, cl = tid % TC;
-    if (tid < 3 * TC && cl < nc) {
+    // C: su/sv/sw, then au/av/aw for cells with an a-updating face
+    if (fold) {
         const int c = c0 + cl;
-        T *dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
-        T acc = dst[c];
+        T acc = acc0;
         int e0, n;
         row_range(m, c, e0, n);
         const T *l0 = lx + TC + (3 * k) * W, *l1 = l0 + W, *l2 = l1 + W;
