@@ -104,15 +104,14 @@ def stream_read_gbs(torch, nbytes=1 << 30, reps=10):
     return nbytes / (e0.elapsed_time(e1) / 1e3 / reps) / 1e9
 
 
-def cpu_baseline(host, fields, budget_s=20.0):
-    """Time the oracle (one core) on the same mesh and inputs: one sweep of
-    the four kernels, repeated until about budget_s seconds of CPU work."""
+def oracle_sweeps(host, fields, budget_s, max_reps=5):
+    """Median seconds per kernel of the oracle's gradient+mass+scalar+uvw sweep on ``host``, repeated until
+    about budget_s seconds of CPU work or max_reps sweeps; returns (per-kernel seconds, sweeps run)."""
     import pyoracle as po
     from kernel_runs import run
 
     m = po.OracleMesh(host, 64)
     times = {}
-    t_sweep = 0.0
     reps = 0
     t_start = time.perf_counter()
     while True:
@@ -120,12 +119,17 @@ def cpu_baseline(host, fields, budget_s=20.0):
             f = fields.copy()
             t0 = time.perf_counter()
             run(k, po, m, f, lambda: {})
-            dt = time.perf_counter() - t0
-            times.setdefault(k, []).append(dt)
+            times.setdefault(k, []).append(time.perf_counter() - t0)
         reps += 1
-        if time.perf_counter() - t_start > budget_s or reps >= 5:
+        if time.perf_counter() - t_start > budget_s or reps >= max_reps:
             break
-    per = {k: float(np.median(v)) for k, v in times.items()}
+    return {k: float(np.median(v)) for k, v in times.items()}, reps
+
+
+def cpu_baseline(host, fields, budget_s=20.0):
+    """Time the oracle (one core) on the same mesh and inputs: one sweep of
+    the four kernels, repeated until about budget_s seconds of CPU work."""
+    per, reps = oracle_sweeps(host, fields, budget_s)
     t_sweep = sum(per.values())
     return {
         "value": host.nCells / t_sweep / 1e9,
@@ -135,6 +139,69 @@ def cpu_baseline(host, fields, budget_s=20.0):
         "sample": f"{reps} sweep(s) of gradient+mass+scalar+uvw on the same {host.params['nx']}^3 relabelled "
                   f"mesh, median per kernel (includes per-call input copies outside the timed calls)",
         "per_kernel_s": per,
+    }
+
+
+def cpu_worker(spec):
+    """One partition of the all-cores CPU baseline, run in a process of its own (no torch, no GPU): the
+    oracle sweep on a relabelled StructGen mesh of nx*ny*nz cells; prints its cells and median sweep time."""
+    import cupcfd_hip as ch
+    from cupcfd_hip import workload as W
+
+    nx, ny, nz, seed, budget = spec.split(",")
+    host = ch.HostMesh(int(nx), int(ny), int(nz), seed=int(seed), permute_cells=True, permute_faces=True)
+    fields = W.Fields(host, seed=777)
+    print("ready", flush=True)
+    sys.stdin.readline()  # every partition starts sweeping at the same time
+    per, reps = oracle_sweeps(host, fields, float(budget), max_reps=1000)
+    print(json.dumps({"cells": host.nCells, "sweep_s": sum(per.values()), "reps": reps}), flush=True)
+
+
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline_all_cores(n, budget_s):
+    """The oracle on every host core this process may use (at most 16), one partition per core as the
+    reference runs one MPI rank per core: the n^3 grid is cut into P slabs of n*n*(n/P) cells, each swept
+    by its own process at the same time. Aggregate rate = all cells / the slowest partition's sweep."""
+    import subprocess
+
+    P = max(1, min(len(os.sched_getaffinity(0)), 16))
+    nz = max(1, round(n / P))
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               f"{n},{n},{nz},{1000 + i},{budget_s}"], stdin=subprocess.PIPE,
+                              stdout=subprocess.PIPE, text=True)
+             for i in range(P)]
+    for pr in procs:
+        if pr.stdout.readline().strip() != "ready":
+            raise RuntimeError("CPU baseline worker failed to start")
+    for pr in procs:
+        pr.stdin.write("go\n")
+        pr.stdin.flush()
+    res = []
+    for pr in procs:
+        out, _ = pr.communicate(timeout=600)
+        if pr.returncode != 0:
+            raise RuntimeError(f"CPU baseline worker exited with {pr.returncode}")
+        res.append(json.loads(out.strip().splitlines()[-1]))
+    cells = sum(r["cells"] for r in res)
+    slowest = max(r["sweep_s"] for r in res)
+    return {
+        "value": cells / slowest / 1e9,
+        "unit": "Gcell-sweeps/s",
+        "cores": P,
+        "kind": "port",
+        "sample": f"{P} concurrent processes, each sweeping gradient+mass+scalar+uvw on its own relabelled "
+                  f"{n}x{n}x{nz} StructGen partition ({cells} cells in all), all started together and sweeping for "
+                  f"about {budget_s:g} s, median per kernel",
+        "slowest_partition_sweep_s": slowest,
     }
 
 
@@ -618,6 +685,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched K-scalar comparison")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched K-scalar comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint comparison")
@@ -639,6 +707,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding design is documented, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker)
+        return
 
     import torch
 
@@ -734,7 +806,9 @@ def main():
         out["tets"] = bench_tets(torch, args.tets_n, args.tets_k, steps=max(3, min(args.steps, 10)), warmup=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host, f, args.cpu_budget)
-        out["cpu_baseline"]["cores"] = 1
+        out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(n, args.cpu_budget / 2)
+        out["cpu_baseline"]["host_cpu"] = host_cpu_model()
+        out["cpu_baseline"]["nproc"] = os.cpu_count()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
