@@ -222,24 +222,26 @@ def batched_fields(host, K, seed):
     return phi, phiB
 
 
-def make_sweep(torch, host, dev, f):
-    """The four face-loop kernels of one sweep on resident copies of ``f`` (FP64, internal numbering), as
-    (name, callable) pairs launching on the current stream."""
+def make_sweep(torch, host, dev, f, arrays=False):
+    """The four face-loop kernels of one sweep on resident copies of ``f`` (internal numbering, in the
+    precision of ``dev``), as (name, callable) pairs launching on the current stream; with ``arrays`` also
+    the dict of device arrays they read and write."""
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
     C, F = host.nCells, host.nFaces
     co, fo = dev.cell_order, dev.face_order
     cuda = torch.device("cuda:0")
+    dt = torch.float64 if dev.precision == 64 else torch.float32
 
     def cells(a):
-        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda)
+        return torch.from_numpy(np.ascontiguousarray(a[co])).to(cuda, dt)
 
     def faces(a, k=1):
-        return torch.from_numpy(np.ascontiguousarray(a.reshape(F, k)[fo].ravel())).to(cuda)
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(F, k)[fo].ravel())).to(cuda, dt)
 
     def bnd(a):
-        return torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+        return torch.from_numpy(np.ascontiguousarray(a)).to(cuda, dt)
 
     d = dict(phi=cells(f.phi), phiB=bnd(f.phiB), u=cells(f.u), v=cells(f.v), w=cells(f.w), den=cells(f.den),
              p=cells(f.p), ar=cells(f.ar), visEff=cells(f.visEff), visEffB=bnd(f.visEffB),
@@ -247,11 +249,11 @@ def make_sweep(torch, host, dev, f):
              mf=faces(f.massFlux), te=cells(f.te), ed=cells(f.ed), t=cells(f.t), teB=bnd(f.teB), edB=bnd(f.edB),
              tB=bnd(f.tB), denB=bnd(f.denB), cpB=bnd(f.cpB), uB=bnd(f.uB), vB=bnd(f.vB), wB=bnd(f.wB),
              su=cells(f.su), sv=cells(f.sv), sw=cells(f.sw), au=cells(f.au), av=cells(f.av), aw=cells(f.aw))
-    d["g"] = torch.empty(C * 3, dtype=torch.float64, device=cuda)
+    d["g"] = torch.empty(C * 3, dtype=dt, device=cuda)
     d["go"] = torch.empty_like(d["g"])
-    d["rface"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
-    d["rface2"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
-    d["rface3"] = torch.zeros(2 * F, dtype=torch.float64, device=cuda)
+    d["rface"] = torch.zeros(2 * F, dtype=dt, device=cuda)
+    d["rface2"] = torch.zeros(2 * F, dtype=dt, device=cuda)
+    d["rface3"] = torch.zeros(2 * F, dtype=dt, device=cuda)
     d["mf_out"] = d["mf"].clone()
     d["Su"] = d["su"].clone()
     d["Au"] = d["au"].clone()
@@ -277,7 +279,55 @@ def make_sweep(torch, host, dev, f):
               d["sw"], d["au"], d["av"], d["aw"], stream)
 
     kernels = [("gradient", k_gradient), ("mass", k_mass), ("scalar", k_scalar), ("uvw", k_uvw)]
-    return kernels
+    return (kernels, d) if arrays else kernels
+
+
+# outputs of one sweep compared between precisions: (kernel, output, array name in make_sweep's dict)
+SWEEP_OUTPUTS = [("gradient", "dPhidx", "g"), ("mass", "massFlux", "mf_out"), ("mass", "RFace", "rface"),
+                 ("scalar", "Su", "Su"), ("scalar", "RFace", "rface2"), ("scalar", "PhiB", "phiB2"),
+                 ("uvw", "su", "su"), ("uvw", "sv", "sv"), ("uvw", "sw", "sw"), ("uvw", "rFace", "rface3")]
+
+
+def bench_f32(torch, host, f, steps, warmup, schedule):
+    """The same sweep in FP32 (T = float throughout: the reference's float instantiation, FP32 mesh and
+    fields): per-kernel time and algorithmic-byte fraction of the HBM peak at s = 4, and the max-norm relative
+    error of each output against the FP64 sweep of the same inputs after one sweep from the same state."""
+    import cupcfd_hip as ch
+
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    Fi = int((host.a["faceBnd"] < 0).sum())
+    d32 = ch.DeviceMesh(host, 32, reorder=True)
+    d64 = ch.DeviceMesh(host, 64, reorder=True)
+    if schedule >= 0:
+        d32.set_schedule(schedule)
+        d64.set_schedule(schedule)
+        sched = [schedule] * 3
+    else:
+        sched = d32.autotune()
+        d64.autotune()
+    k32, a32 = make_sweep(torch, host, d32, f, arrays=True)
+    k64, a64 = make_sweep(torch, host, d64, f, arrays=True)
+    for (_, k), (_, k2) in zip(k32, k64):
+        k()
+        k2()
+    torch.cuda.synchronize()
+    err = {}
+    for kern, name, key in SWEEP_OUTPUTS:
+        ref = a64[key]
+        scale = ref.abs().max().item()
+        err[f"{kern}.{name}"] = ((a32[key].double() - ref).abs().max().item() / scale) if scale > 0 else 0.0
+    del k64, a64, d64
+    per, step_s, _ = time_sweep(torch, k32, steps, warmup)
+    ab = algorithmic_bytes(C, F, Fi, B, 4)
+    kern = {}
+    for name, t in per.items():
+        gbs = ab[name] / t / 1e9
+        kern[name] = {"ms": t * 1e3, "gcell_sweeps_per_s": C / t / 1e9, "algorithmic_bytes": ab[name],
+                      "algorithmic_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    return {"workload": f"the same sweep in FP32 (float mesh, fields and arithmetic), StructGen "
+                        f"{host.params['nx']}^3 relabelled", "gcell_sweeps_per_s": C / step_s / 1e9,
+            "ms_per_step": step_s * 1e3, "gather_schedules": dict(zip(("gradient", "scalar", "uvw"), sched)),
+            "kernels": kern, "rel_err_vs_f64": err}
 
 
 def time_sweep(torch, kernels, steps, warmup):
@@ -692,6 +742,7 @@ def main():
     ap.add_argument("--no-tets", action="store_true", help="skip the tetrahedral stress mesh")
     ap.add_argument("--no-partitions", action="store_true", help="skip the many-partition batched launch")
     ap.add_argument("--no-mesh-build", action="store_true", help="skip the host vs GPU mesh repack timing")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 sweep")
     ap.add_argument("--partitions-only", action="store_true", help="run only the many-partition batched launch")
     ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
     ap.add_argument("--tets-n", type=int, default=100)
@@ -788,6 +839,8 @@ def main():
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
+    if not args.no_f32:
+        out["f32"] = bench_f32(torch, host, f, steps=max(5, min(args.steps, 20)), warmup=2, schedule=args.schedule)
     if not args.no_mesh_build:
         out["mesh_build"] = bench_mesh_build(torch, host, dev, t_gen, t_host_repack, f"StructGen {n}^3 relabelled")
     if not args.no_batched:
