@@ -365,6 +365,106 @@ def kernel_report(C, F, Fi, B, per):
     return out
 
 
+def loop_bytes(host, s):
+    """Algorithmic bytes of the boundary, region and viscosity loops of one BenchmarkKernels repetition: the
+    ids and values each touches per boundary of the region type(s) it acts on (4-byte ids), per region and
+    per cell; read-modify-write counted twice, v = 3s."""
+    v = 3 * s
+    C, B, R = host.nCells, host.nBoundaries, host.nRegions
+    bt = host.a["regType"][host.a["bndRegion"]]
+    n_in, n_out, n_wall = (int((bt == t).sum()) for t in (1, 2, 4))
+    return {
+        "mass_bl1": n_in * (8 + s),                                     # INLET list, face id; massFlux
+        "mass_bl2": n_out * (12 + s) + 2 * s * R,                       # + flowRegion read-modify-write
+        "mass_bl3": n_out * (8 + s),                                    # area
+        "mass_bl4": n_out * (12 + s + v + s + s + 3 * s) + s * R,       # area, S, denB; massFlux, uB/vB/wB out
+        "mass_bl5": n_out * (12 + 2 * s + 6 * s + 2 * s + s),           # massFlux, uB/vB/wB, su RMW; flowFact
+        "mass_region": R * (4 + 3 * s),
+        "uvw_region1": R * v,
+        "uvw_bnds1": n_wall * (8 + v) + 2 * v * R,
+        "visc_cell1": 5 * s * C,                                        # TE, ED, den; visEff RMW
+        "visc_region": R * (4 + 4 * s),
+        "visc_boundary": B * (12 + 4 * s),                              # ids; cell TE, den, visEff; visEffB out
+        "visc_cell2": 2 * s * (C + B),                                  # visEff, visEffB RMW
+    }
+
+
+def bench_all_kernels(torch, n, steps, warmup, schedule):
+    """All 16 kernels of BenchmarkKernels::runBenchmark (BenchmarkKernels.cpp:55-115), resident, each timed on
+    its own (HIP events around `steps` back-to-back calls) on a typed n^3 relabelled StructGen mesh, FP64.
+    Typed (INLET x-low, OUTLET x-high, WALL y, SYMP z) so the boundary and region loops have work. The O(B)
+    and O(R) loops move a few hundred KB and are latency-bound; their fractions are not bandwidth figures."""
+    import cupcfd_hip as ch
+    from cupcfd_hip import workload as W
+    from cupcfd_hip.fvm import resident as R
+
+    host = ch.HostMesh(n, seed=31, typed=True, permute_cells=True, permute_faces=True)
+    dev = ch.DeviceMesh(host, 64, reorder=True)
+    if schedule >= 0:
+        dev.set_schedule(schedule)
+    else:
+        dev.autotune()
+    f = W.Fields(host, seed=321)
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    Fi = int((host.a["faceBnd"] < 0).sum())
+    sweep, d = make_sweep(torch, host, dev, f, arrays=True)
+    cuda = torch.device("cuda:0")
+
+    def one(x):
+        return torch.full((1,), x, dtype=torch.float64, device=cuda)
+
+    plain = {k: torch.from_numpy(np.ascontiguousarray(getattr(f, k))).to(cuda)
+             for k in ("flowRegion", "flowFact")}
+    fl = [one(0.0) for _ in range(4)]
+    q, vc = W.MASS_LOOPS, W.VISC
+    ve2 = d["visEff"].clone()
+    vb2 = d["visEffB"].clone()
+    calls = dict(sweep)
+    calls.update({
+        "mass_bl1": lambda: R.mass_bl1(dev, d["mf_out"], fl[0]),
+        "mass_bl2": lambda: R.mass_bl2(dev, d["mf_out"], plain["flowRegion"], fl[1]),
+        "mass_bl3": lambda: R.mass_bl3(dev, q["flowin"], fl[2]),
+        "mass_bl4": lambda: R.mass_bl4(dev, d["mf_out"], d["uB"], d["vB"], d["wB"], d["denB"], q["ratearea"], fl[1]),
+        "mass_bl5": lambda: R.mass_bl5(dev, d["mf_out"], d["su"], d["uB"], d["vB"], d["wB"], q["fact"], 1, 1, 1,
+                                       plain["flowFact"], fl[3]),
+        "mass_region": lambda: R.mass_region(dev, plain["flowFact"], plain["flowRegion"], 0.0),
+        "uvw_region1": lambda: R.uvw_region1(dev),
+        "uvw_bnds1": lambda: R.uvw_bnds1(dev),
+        "visc_cell1": lambda: R.visc_cell1(dev, vc["cell1"]["small"], vc["cell1"]["cmu"], vc["cell1"]["visURF"],
+                                           vc["cell1"]["visLam"], d["te"], d["ed"], d["den"], ve2),
+        "visc_region": lambda: R.visc_region(dev, vc["region"]["kappa"]),
+        "visc_boundary": lambda: R.visc_boundary(dev, vc["boundary"]["tmcmu"], vc["boundary"]["large"],
+                                                 vc["boundary"]["small"], vc["boundary"]["kappa"],
+                                                 vc["boundary"]["visLam"], d["te"], d["den"], ve2, vb2),
+        "visc_cell2": lambda: R.visc_cell2(dev, vc["cell2"]["visLam"], ve2, vb2),
+    })
+    order = ["gradient", "mass", "mass_bl1", "mass_bl2", "mass_bl3", "mass_bl4", "mass_bl5", "mass_region", "scalar",
+             "uvw", "uvw_region1", "uvw_bnds1", "visc_cell1", "visc_region", "visc_boundary", "visc_cell2"]
+    nbytes = dict(algorithmic_bytes(C, F, Fi, B, 8))
+    nbytes.update(loop_bytes(host, 8))
+    stream = torch.cuda.current_stream()
+    quiesce()
+    out = {}
+    for name in order:
+        fn = calls[name]
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / steps
+        gbs = nbytes[name] / t / 1e9
+        out[name] = {"us": t * 1e6, "algorithmic_bytes": nbytes[name], "algorithmic_GBps": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS, "bound": "hbm" if nbytes[name] > (32 << 20) else "latency"}
+    return {"workload": f"the 16 BenchmarkKernels kernels, resident, one at a time, StructGen {n}^3 relabelled "
+                        f"with INLET/OUTLET/WALL/SYMP regions, FP64", "cells": C, "boundaries": B,
+            "repetition_us": sum(v["us"] for v in out.values()), "kernels": out}
+
+
 def bench_batched(torch, host, dev, K, steps, warmup, label):
     """K scalars through gradient + scalar flux on ``host`` (FP64): one batched call of each against K
     single-scalar calls on K separate arrays. The scalars share the mass flux of one FluxMassDolfynFaceLoop
@@ -743,6 +843,7 @@ def main():
     ap.add_argument("--no-partitions", action="store_true", help="skip the many-partition batched launch")
     ap.add_argument("--no-mesh-build", action="store_true", help="skip the host vs GPU mesh repack timing")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 sweep")
+    ap.add_argument("--no-all-kernels", action="store_true", help="skip the 16-kernel typed-mesh timing")
     ap.add_argument("--partitions-only", action="store_true", help="run only the many-partition batched launch")
     ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
     ap.add_argument("--tets-n", type=int, default=100)
@@ -839,6 +940,9 @@ def main():
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
+    if not args.no_all_kernels:
+        out["all_kernels"] = bench_all_kernels(torch, n, steps=max(5, min(args.steps, 20)), warmup=2,
+                                               schedule=args.schedule)
     if not args.no_f32:
         out["f32"] = bench_f32(torch, host, f, steps=max(5, min(args.steps, 20)), warmup=2, schedule=args.schedule)
     if not args.no_mesh_build:

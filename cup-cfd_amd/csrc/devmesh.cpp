@@ -16,6 +16,8 @@ template <class T> Mesh<T>::~Mesh() {
     for (void *p : ws) (void)hipFree(p);
     for (int k = 0; k < 4; k++)
         if (sc_src[k]) (void)hipFree(sc_src[k]);
+    if (rbnd) (void)hipFree(rbnd);
+    if (rbnd_off) (void)hipFree(rbnd_off);
 }
 
 template <class T> void *Mesh<T>::ws_get(int slot, size_t bytes) {
@@ -493,6 +495,34 @@ template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wa
 
 template int *scalar_carry_src<double>(Mesh<double> &, bool, bool);
 template int *scalar_carry_src<float>(Mesh<float> &, bool, bool);
+
+// A stable counting sort of the boundary ids by region: each region's
+// boundaries in ascending id, the order the reference's boundary loops add
+// them in (one pass over b = 0 .. B-1).
+template <class T> bool region_boundaries(Mesh<T> &M) {
+    if (M.rbnd && M.rbnd_off) return true;
+    const int B = M.d.B, R = M.d.R;
+    std::vector<int> off(R + 1, 0), lst(B ? B : 1);
+    for (int b = 0; b < B; b++) off[M.bregion_h[b] + 1]++;
+    for (int r = 0; r < R; r++) off[r + 1] += off[r];
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int b = 0; b < B; b++) lst[pos[M.bregion_h[b]]++] = b;
+    void *pl = nullptr, *po = nullptr;
+    if (hipMalloc(&pl, lst.size() * sizeof(int)) != hipSuccess) return false;
+    if (hipMalloc(&po, off.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(pl, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(po, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(pl);
+        if (po) (void)hipFree(po);
+        return false;
+    }
+    M.rbnd = (int *)pl;
+    M.rbnd_off = (int *)po;
+    return true;
+}
+
+template bool region_boundaries<double>(Mesh<double> &);
+template bool region_boundaries<float>(Mesh<float> &);
 
 }  // namespace cupcfd_hip
 

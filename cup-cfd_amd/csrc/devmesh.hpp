@@ -116,6 +116,10 @@ template <class T> struct Mesh {
     int schedule_k[3] = {-1, -1, -1};
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    // boundaries grouped by region, ascending id within each region (region r:
+    // rbnd[rbnd_off[r] .. rbnd_off[r + 1])), built on first use by the
+    // per-region ordered sums
+    int *rbnd = nullptr, *rbnd_off = nullptr;
     // order-preserving face colouring, built on first use of the colour-group
     // schedule (-1: not built): colour(f) = 1 + the largest colour of the face
     // before f in the rows of f's cells, so colours rise along every row and no
@@ -132,6 +136,8 @@ template <class T> struct Mesh {
 };
 
 template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns);
+// M.rbnd / M.rbnd_off, built on first call; false if they cannot be allocated
+template <class T> bool region_boundaries(Mesh<T> &M);
 int sync_state_any(struct ::cupcfd_hip_mesh *h, bool push);
 
 // Counts and pointers a MiniSoA view must carry before either repack reads it.

@@ -108,16 +108,45 @@ template <class T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) 
 template <class T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 
 // ---------------------------------------------------------------------------
-// Sequential-order wave reduction: returns acc + v[0] + v[1] + ... + v[n-1]
-// evaluated left to right, as the reference's boundary loops do. Call with
-// one wavefront (64 lanes); every lane returns the same value.
+// Ordered sums (the boundary loops' flow sums, region flows and force
+// tangents): for each component c < NV, init_c + v_c(0) + v_c(1) + ... +
+// v_c(n-1), added left to right as the reference's sequential loops do, so
+// the result is the reference's, bit for bit. One workgroup of kSeqBlock
+// threads: wave c adds component c, reading each value from LDS at a
+// wave-uniform address (a broadcast, so the chain of dependent adds is the
+// only serial work), while the other waves gather the next chunk of values
+// into the other half of a double buffer. Returns component `wave` of the
+// sum to the waves c < NV.
 // ---------------------------------------------------------------------------
-template <class T, class Get> __device__ T wave_seq_sum(T acc, int n, Get get) {
-    const int lane = threadIdx.x;
-    for (int base = 0; base < n; base += 64) {
-        T v = (base + lane < n) ? get(base + lane) : T(0);
-        const int cnt = min(64, n - base);
-        for (int j = 0; j < cnt; j++) acc = acc + __shfl(v, j, 64);
+constexpr int kSeqBlock = 1024;
+constexpr int kSeqLds = 3072;  // values per half of the double buffer (all components)
+
+template <class T, int NV, class Get> __device__ T block_ordered_sum(T init, int n, Get get, T *buf) {
+    constexpr int CH = kSeqLds / NV;  // items per chunk
+    static_assert(CH % 64 == 0 && NV * 64 < kSeqBlock, "chunk of whole waves; staging waves left over");
+    const int tid = threadIdx.x, wave = tid >> 6;
+    auto stage = [&](int base, T *dst, int t0, int nt) {
+        const int cnt = min(CH, n - base);
+        for (int i = t0; i < cnt; i += nt) {
+            T v[NV];
+            get(base + i, v);
+            for (int c = 0; c < NV; c++) dst[c * CH + i] = v[c];
+        }
+    };
+    T acc = init;
+    if (n > 0) stage(0, buf, tid, kSeqBlock);
+    __syncthreads();
+    int k = 0;
+    for (int base = 0; base < n; base += CH, k ^= 1) {
+        if (wave >= NV) {
+            if (base + CH < n) stage(base + CH, buf + (k ^ 1) * NV * CH, tid - NV * 64, kSeqBlock - NV * 64);
+        } else {
+            const T *col = buf + k * NV * CH + wave * CH;
+            const int cnt = min(CH, n - base);
+#pragma unroll 32
+            for (int i = 0; i < cnt; i++) acc = acc + col[i];
+        }
+        __syncthreads();
     }
     return acc;
 }
@@ -594,48 +623,44 @@ __global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, in
 }
 
 // a3 FluxMassDolfynBoundaryLoop1 (MassKernels.ipp:311-336)
-template <class T> __global__ void k_mass_bl1(DevMesh<T> m, const T *__restrict__ mf, T *flowin) {
+template <class T>
+__global__ __launch_bounds__(kSeqBlock) void k_mass_bl1(DevMesh<T> m, const T *__restrict__ mf, T *flowin) {
+    __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_INLET];
-    T s = wave_seq_sum<T>(T(0.0), m.nblist[CUPCFD_RTYPE_INLET],
-                          [&](int i) { return mf[m.bface[lst[i]]]; });
+    const T s = block_ordered_sum<T, 1>(
+        T(0.0), m.nblist[CUPCFD_RTYPE_INLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
     if (threadIdx.x == 0) *flowin = s;
 }
 
-// a4 FluxMassDolfynBoundaryLoop2 (MassKernels.ipp:338-368)
-template <class T> __global__ void k_mass_bl2(DevMesh<T> m, const T *__restrict__ mf, T *flowRegion, T *flowout) {
-    const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
-    const int n = m.nblist[CUPCFD_RTYPE_OUTLET];
-    const int lane = threadIdx.x;
-    T s = wave_seq_sum<T>(T(0.0), n, [&](int i) { return mf[m.bface[lst[i]]]; });
-    if (lane == 0) *flowout = s;
-    // flowRegion[ir] += mf in boundary order; lane r owns region rbase + r
-    for (int rbase = 0; rbase < m.R; rbase += 64) {
-        const int myr = rbase + lane;
-        T acc = myr < m.R ? flowRegion[myr] : T(0);
-        for (int base = 0; base < n; base += 64) {
-            T v = 0;
-            int r = -1;
-            if (base + lane < n) {
-                const int b = lst[base + lane];
-                v = mf[m.bface[b]];
-                r = m.bregion[b];
-            }
-            const int cnt = min(64, n - base);
-            for (int j = 0; j < cnt; j++) {
-                const T vj = __shfl(v, j, 64);
-                const int rj = __shfl(r, j, 64);
-                if (rj == myr) acc = acc + vj;
-            }
-        }
-        if (myr < m.R) flowRegion[myr] = acc;
+// a4 FluxMassDolfynBoundaryLoop2 (MassKernels.ipp:338-368): block 0 sums
+// flowout over the OUTLET boundaries; block 1 + r adds region r's OUTLET
+// boundaries to flowRegion[r], in ascending boundary id
+template <class T>
+__global__ __launch_bounds__(kSeqBlock) void k_mass_bl2(DevMesh<T> m, const T *__restrict__ mf, T *flowRegion,
+                                                        T *flowout, const int *__restrict__ rbnd,
+                                                        const int *__restrict__ roff) {
+    __shared__ T buf[2 * kSeqLds];
+    if (blockIdx.x == 0) {
+        const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
+        const T s = block_ordered_sum<T, 1>(
+            T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
+        if (threadIdx.x == 0) *flowout = s;
+        return;
     }
+    const int r = blockIdx.x - 1;
+    if (m.rtype[r] != CUPCFD_RTYPE_OUTLET) return;
+    const int b0 = roff[r];
+    const T s = block_ordered_sum<T, 1>(
+        flowRegion[r], roff[r + 1] - b0, [&](int i, T *v) { v[0] = mf[m.bface[rbnd[b0 + i]]]; }, buf);
+    if (threadIdx.x == 0) flowRegion[r] = s;
 }
 
 // a5 FluxMassDolfynBoundaryLoop3 (MassKernels.ipp:371-393)
-template <class T> __global__ void k_mass_bl3(DevMesh<T> m, T flowin, T *ratearea) {
+template <class T> __global__ __launch_bounds__(kSeqBlock) void k_mass_bl3(DevMesh<T> m, T flowin, T *ratearea) {
+    __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
-    T areaout = wave_seq_sum<T>(T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET],
-                                [&](int i) { return m.fgeoX[m.bface[lst[i]]].w; });
+    const T areaout = block_ordered_sum<T, 1>(
+        T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = m.fgeoX[m.bface[lst[i]]].w; }, buf);
     if (threadIdx.x == 0) *ratearea = -flowin / areaout;
 }
 
@@ -662,10 +687,14 @@ __global__ __launch_bounds__(kBlock) void k_mass_bl4_faces(DevMesh<T> m, T *mass
 
 // sum of massFlux over OUTLET boundaries in boundary order, starting from
 // *acc_in (0 for loop 4, the caller's flowout2 for loop 5)
-template <class T> __global__ void k_outlet_sum(DevMesh<T> m, const T *__restrict__ mf, T *acc, int from_zero) {
+template <class T>
+__global__ __launch_bounds__(kSeqBlock) void k_outlet_sum(DevMesh<T> m, const T *__restrict__ mf, T *acc,
+                                                          int from_zero) {
+    __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
-    T a0 = from_zero ? T(0.0) : *acc;
-    T s = wave_seq_sum<T>(a0, m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i) { return mf[m.bface[lst[i]]]; });
+    const T a0 = from_zero ? T(0.0) : *acc;
+    const T s = block_ordered_sum<T, 1>(
+        a0, m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
     if (threadIdx.x == 0) *acc = s;
 }
 
@@ -1511,30 +1540,23 @@ template <class T> __global__ void k_uvw_region1(DevMesh<T> m) {
 
 // a12 FluxUVWDolfynBndsLoop1 (UVWKernels.ipp:527-544): WALL shear summed per
 // region in boundary order; lane r owns region rbase + r
-template <class T> __global__ void k_uvw_bnds1(DevMesh<T> m) {
-    const int *lst = m.blist[CUPCFD_RTYPE_WALL];
-    const int n = m.nblist[CUPCFD_RTYPE_WALL];
-    const int lane = threadIdx.x;
-    for (int rbase = 0; rbase < m.R; rbase += 64) {
-        const int myr = rbase + lane;
-        v3<T> acc = myr < m.R ? ld3((const T *)m.rforcet, myr) : v3<T>{0, 0, 0};
-        for (int base = 0; base < n; base += 64) {
-            v3<T> v{0, 0, 0};
-            int r = -1;
-            if (base + lane < n) {
-                const int b = lst[base + lane];
-                v = ld3((const T *)m.bshear, b);
-                r = m.bregion[b];
-            }
-            const int cnt = min(64, n - base);
-            for (int j = 0; j < cnt; j++) {
-                const T x = __shfl(v.x, j, 64), y = __shfl(v.y, j, 64), z = __shfl(v.z, j, 64);
-                const int rj = __shfl(r, j, 64);
-                if (rj == myr) acc = v3<T>{acc.x + x, acc.y + y, acc.z + z};
-            }
-        }
-        if (myr < m.R) st3(m.rforcet, myr, acc);
-    }
+// block r adds the boundary shear of WALL region r's boundaries, in ascending
+// id, to its force tangent; wave c < 3 adds component c
+template <class T>
+__global__ __launch_bounds__(kSeqBlock) void k_uvw_bnds1(DevMesh<T> m, const int *__restrict__ rbnd,
+                                                         const int *__restrict__ roff) {
+    __shared__ T buf[2 * kSeqLds];
+    const int r = blockIdx.x;
+    if (m.rtype[r] != CUPCFD_RTYPE_WALL) return;
+    const int b0 = roff[r], c = min((int)threadIdx.x >> 6, 2);
+    const T s = block_ordered_sum<T, 3>(
+        m.rforcet[3 * r + c], roff[r + 1] - b0,
+        [&](int i, T *v) {
+            const v3<T> x = ld3((const T *)m.bshear, rbnd[b0 + i]);
+            v[0] = x.x, v[1] = x.y, v[2] = x.z;
+        },
+        buf);
+    if (threadIdx.x < 192 && (threadIdx.x & 63) == 0) m.rforcet[3 * r + c] = s;
 }
 
 // ===========================================================================
@@ -3175,21 +3197,24 @@ int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void 
                                                 void *stream) {                                                    \
         Mesh<T> *M = get<T>(mesh);                                                                                 \
         if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
-        hipLaunchKernelGGL(k_mass_bl1<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, massFlux, flowin);      \
+        hipLaunchKernelGGL(k_mass_bl1<T>, dim3(1), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d, massFlux,        \
+                           flowin);                                                                                \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_flux_mass_boundary_loop2_##S(cupcfd_hip_mesh *mesh, const T *massFlux, T *flowRegion,           \
                                                 T *flowout, void *stream) {                                        \
         Mesh<T> *M = get<T>(mesh);                                                                                 \
         if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
-        hipLaunchKernelGGL(k_mass_bl2<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, massFlux, flowRegion,   \
-                           flowout);                                                                               \
+        if (!region_boundaries(*M)) return CUPCFD_E_ERROR;                                                         \
+        hipLaunchKernelGGL(k_mass_bl2<T>, dim3(M->d.R + 1), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d,         \
+                           massFlux, flowRegion, flowout, (const int *)M->rbnd, (const int *)M->rbnd_off);         \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_flux_mass_boundary_loop3_##S(cupcfd_hip_mesh *mesh, T flowin, T *ratearea, void *stream) {      \
         Mesh<T> *M = get<T>(mesh);                                                                                 \
         if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
-        hipLaunchKernelGGL(k_mass_bl3<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, flowin, ratearea);      \
+        hipLaunchKernelGGL(k_mass_bl3<T>, dim3(1), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d, flowin,          \
+                           ratearea);                                                                              \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_flux_mass_boundary_loop4_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *uBoundary, T *vBoundary,    \
@@ -3201,8 +3226,8 @@ int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void 
         if (n)                                                                                                     \
             hipLaunchKernelGGL(k_mass_bl4_faces<T>, dim3(blocks(n)), dim3(kBlock), 0, (hipStream_t)stream, M->d,   \
                                massFlux, uBoundary, vBoundary, wBoundary, denBoundary, ratearea);                  \
-        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, (const T *)massFlux,  \
-                           flowout, 1);                                                                            \
+        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d,                \
+                           (const T *)massFlux, flowout, 1);                                                       \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_flux_mass_boundary_loop5_##S(cupcfd_hip_mesh *mesh, T *massFlux, T *su, T *uBoundary,           \
@@ -3215,8 +3240,8 @@ int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void 
             hipLaunchKernelGGL(k_mass_bl5_cells<T>, dim3(blocks(M->d.noc)), dim3(kBlock), 0, (hipStream_t)stream,  \
                                M->d, massFlux, su, uBoundary, vBoundary, wBoundary, fact, solveU, solveV, solveW,  \
                                flowFact);                                                                          \
-        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d, (const T *)massFlux,  \
-                           flowout2, 0);                                                                           \
+        hipLaunchKernelGGL(k_outlet_sum<T>, dim3(1), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d,                \
+                           (const T *)massFlux, flowout2, 0);                                                      \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_flux_mass_region_loop_##S(cupcfd_hip_mesh *mesh, T *flowFact, const T *flowRegion, T flowIn,    \
@@ -3259,7 +3284,10 @@ int cupcfd_dev_faces_from_internal(cupcfd_hip_mesh *mesh, const void *src, void 
     int cupcfd_dev_flux_uvw_bnds_loop1_##S(cupcfd_hip_mesh *mesh, void *stream) {                                  \
         Mesh<T> *M = get<T>(mesh);                                                                                 \
         if (!M) return CUPCFD_E_NULL_PTR;                                                                          \
-        hipLaunchKernelGGL(k_uvw_bnds1<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, M->d);                       \
+        if (!region_boundaries(*M)) return CUPCFD_E_ERROR;                                                         \
+        if (M->d.R)                                                                                                \
+            hipLaunchKernelGGL(k_uvw_bnds1<T>, dim3(M->d.R), dim3(kSeqBlock), 0, (hipStream_t)stream, M->d,        \
+                               (const int *)M->rbnd, (const int *)M->rbnd_off);                                    \
         return launch_ok();                                                                                        \
     }                                                                                                              \
     int cupcfd_dev_viscosity_cell_loop1_##S(cupcfd_hip_mesh *mesh, T small, T cmu, T visURF, T visLam,             \

@@ -327,3 +327,78 @@ class resident:
             _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
             _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
             t(mfbar), cls._s(stream)))
+
+    # the boundary, region and viscosity loops; scalar outputs (flowin, flowout, ratearea, ...) are one-value
+    # device tensors
+    @classmethod
+    def mass_bl1(cls, mesh, mf, flowin, stream=None):
+        """MassKernels.h:90-94"""
+        cls.call(mesh, "flux_mass_boundary_loop1", mesh.handle, cls._t(mf), cls._t(flowin), cls._s(stream))
+
+    @classmethod
+    def mass_bl2(cls, mesh, mf, flowRegion, flowout, stream=None):
+        """MassKernels.h:103-108"""
+        t = cls._t
+        cls.call(mesh, "flux_mass_boundary_loop2", mesh.handle, t(mf), t(flowRegion), t(flowout), cls._s(stream))
+
+    @classmethod
+    def mass_bl3(cls, mesh, flowin, ratearea, stream=None):
+        """MassKernels.h:117-120 (flowin: a host value)"""
+        cls.call(mesh, "flux_mass_boundary_loop3", mesh.handle, mesh.T(flowin), cls._t(ratearea), cls._s(stream))
+
+    @classmethod
+    def mass_bl4(cls, mesh, mf, uB, vB, wB, denB, ratearea, flowout, stream=None):
+        """MassKernels.h:129-138 (ratearea: a host value)"""
+        t = cls._t
+        cls.call(mesh, "flux_mass_boundary_loop4", mesh.handle, t(mf), t(uB), t(vB), t(wB), t(denB),
+                 mesh.T(ratearea), t(flowout), cls._s(stream))
+
+    @classmethod
+    def mass_bl5(cls, mesh, mf, su, uB, vB, wB, fact, solveU, solveV, solveW, flowFact, flowout2, stream=None):
+        """MassKernels.h:147-157"""
+        t = cls._t
+        cls.call(mesh, "flux_mass_boundary_loop5", mesh.handle, t(mf), t(su), t(uB), t(vB), t(wB), mesh.T(fact),
+                 _I(solveU), _I(solveV), _I(solveW), t(flowFact), t(flowout2), cls._s(stream))
+
+    @classmethod
+    def mass_region(cls, mesh, flowFact, flowRegion, flowIn, stream=None):
+        """MassKernels.h:166-171 (flowIn: a host value)"""
+        cls.call(mesh, "flux_mass_region_loop", mesh.handle, cls._t(flowFact), cls._t(flowRegion), mesh.T(flowIn),
+                 cls._s(stream))
+
+    @classmethod
+    def uvw_region1(cls, mesh, stream=None):
+        """UVWKernels.h:72-73"""
+        cls.call(mesh, "flux_uvw_region_loop1", mesh.handle, cls._s(stream))
+
+    @classmethod
+    def uvw_bnds1(cls, mesh, stream=None):
+        """UVWKernels.h:78-79"""
+        cls.call(mesh, "flux_uvw_bnds_loop1", mesh.handle, cls._s(stream))
+
+    @classmethod
+    def visc_cell1(cls, mesh, small, cmu, visURF, visLam, te, ed, den, visEff, stream=None):
+        """ViscosityKernels.h:38-48"""
+        t, T = cls._t, mesh.T
+        cls.call(mesh, "viscosity_cell_loop1", mesh.handle, T(small), T(cmu), T(visURF), T(visLam), t(te), t(ed),
+                 t(den), t(visEff), cls._s(stream))
+
+    @classmethod
+    def visc_region(cls, mesh, kappa, stream=None):
+        """ViscosityKernels.h:50-51 (the mesh is the second argument there)"""
+        from . import lib, check
+        fn = f"cupcfd_dev_viscosity_region_loop_{mesh.sfx}"
+        check(fn, getattr(lib(), fn)(mesh.T(kappa), mesh.handle, cls._s(stream)))
+
+    @classmethod
+    def visc_boundary(cls, mesh, tmcmu, large, small, kappa, visLam, te, den, visEff, visEffB, stream=None):
+        """ViscosityKernels.h:53-60"""
+        t, T = cls._t, mesh.T
+        cls.call(mesh, "viscosity_boundary_loop", mesh.handle, T(tmcmu), T(large), T(small), T(kappa), T(visLam),
+                 t(te), t(den), t(visEff), t(visEffB), cls._s(stream))
+
+    @classmethod
+    def visc_cell2(cls, mesh, visLam, visEff, visEffB, stream=None):
+        """ViscosityKernels.h:62-67"""
+        cls.call(mesh, "viscosity_cell_loop2", mesh.handle, mesh.T(visLam), cls._t(visEff), cls._t(visEffB),
+                 cls._s(stream))
