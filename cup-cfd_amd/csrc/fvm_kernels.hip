@@ -746,6 +746,7 @@ template <class T, class TS = T> struct ScalarArgs {
     TS *phiB, *au, *su;
     T *rface;
     const int *src;  // PhiFace source per face (typed meshes), else null
+    const T *cval;   // PhiFace each source face leaves (k_scalar_carry_vals), for the tile schedule; else null
     // stride of the per-scalar arrays (phi, PhiB, Au, Su, dPhidx): 1 for one
     // scalar, K for scalar-innermost batches; write_state: this lane writes
     // the mesh's boundary Q/H/T
@@ -799,11 +800,9 @@ __device__ T scalar_wall_phiface(const DevMesh<T> &m, const ScalarArgs<T, TS> &a
     return PhiFace;
 }
 
-// PhiFace as carried into face f: the value the last assigning face left
-template <class T, class TS> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f) {
-    if (!a.src) return m.carry_init;
-    const int g = a.src[f];
-    if (g < 0) return m.carry_init;
+// PhiFace as boundary face g leaves it for the faces after it
+template <class T, class TS>
+__device__ T scalar_carry_from(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int g) {
     const int ib = ~m.fconn[g].y;
     const int ip = m.fconn[g].x;
     const int ir = m.bregion[ib];
@@ -815,6 +814,30 @@ template <class T, class TS> __device__ T scalar_carry(const DevMesh<T> &m, cons
     }
     T H, V;
     return scalar_wall_phiface(m, a, g, ib, ir, ip, H, V);
+}
+
+// PhiFace as carried into face f: the value the last assigning face left
+template <class T, class TS> __device__ T scalar_carry(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f) {
+    if (!a.src) return m.carry_init;
+    const int g = a.src[f];
+    return g < 0 ? m.carry_init : scalar_carry_from(m, a, g);
+}
+
+// The value each boundary face that can carry PhiFace leaves (INLET when it
+// assigns, OUTLET, WALL when it assigns and its region is not adiabatic; the
+// faces a.src names), for the tile schedule to read with one load per face:
+// cval[g - Fi] for boundary face g.
+template <class T> __global__ __launch_bounds__(kBlock) void k_scalar_carry_vals(DevMesh<T> m, ScalarArgs<T> a,
+                                                                               T *cval) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.F - m.Fi) return;
+    const int g = m.Fi + k;
+    const int ib = ~m.fconn[g].y;
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    const bool source = it == CUPCFD_RTYPE_OUTLET || (it == CUPCFD_RTYPE_INLET && a.inletAssigns) ||
+                        (it == CUPCFD_RTYPE_WALL && a.wallActive && !m.radiab[ir]);
+    cval[k] = source ? scalar_carry_from(m, a, g) : m.carry_init;
 }
 
 // the cell values an interior face of the scalar loop reads
@@ -834,7 +857,7 @@ template <class T, class TS>
 __device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const ScalarArgs<T, TS> &a, int f, const V4<T> &s,
                                                 T mf, T rl,
                                                 const ScalarCell<T> &P, const ScalarCell<T> &N, bool owner,
-                                                T t[3]) {
+                                                T PhiFace, T t[3]) {
     const v3<T> S{s.x, s.y, s.z};
     const T facn = s.w;
     const T facp = 1.0 - facn;
@@ -843,7 +866,6 @@ __device__ __forceinline__ void scalar_interior(const DevMesh<T> &m, const Scala
     const v3<T> g = lerp3(N.g, facn, P.g, facp);
     const v3<T> Xpn = sub3(N.cc, P.cc);
     const T VisFace = Visac * rl;
-    const T PhiFace = m.typed ? scalar_carry(m, a, f) : m.carry_init;
     const T fce = mf * PhiFace;
     const T fde1 = Visac * dot3(g, S);
     const T fci = fmin((double)mf, 0.0) * (double)N.phi + fmax((double)mf, 0.0) * (double)P.phi;
@@ -936,7 +958,7 @@ __device__ __forceinline__ bool scalar_terms(const DevMesh<T> &m, const ScalarAr
     if (cn.y >= 0) {
         const bool in = (en.x & 1) != 0;
         scalar_interior(m, a, f, s, a.massFlux[f], m.frl[f], scalar_cell(m, a, cn.x), scalar_cell(m, a, cn.y), !in,
-                        t);
+                        m.typed ? scalar_carry(m, a, f) : m.carry_init, t);
         if (in) {
             t[0] = -t[0];
             t[1] = -t[1];
@@ -1039,8 +1061,9 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
     int2 cn = make_int2(0, 0);
     bool owned;
     V4<T> s{};
-    T mf = 0, rl = 0;
-    // slot sl's face and, for an interior face, its cell outside the tile into column TC + sl
+    T mf = 0, rl = 0, pf = m.carry_init;
+    // slot sl's face and, for an interior face, its carried PhiFace and its
+    // cell outside the tile into column TC + sl
     auto load = [&](int sl) {
         f = slot_conn(m, ts, sl, cn, owned);
         s = m.fgeoS[f];
@@ -1048,6 +1071,10 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
         if (cn.y >= 0) {
             mf = a.massFlux[f];
             rl = m.frl[f];
+            if (a.cval) {
+                const int g = a.src[f];
+                pf = g < 0 ? m.carry_init : a.cval[g - m.Fi];
+            }
             out = outside_cell(cn, c0, nc);
             if (out >= 0) {
                 const ScalarCell<T> o = scalar_cell(m, a, out);
@@ -1071,7 +1098,7 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
         T tt[4];
         bool fl = false;
         if (cn.y >= 0) {
-            scalar_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, tt);
+            scalar_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, pf, tt);
             tt[3] = nzero<T>();
         } else {
             fl = scalar_boundary(m, a, f, s, cn.x, cell(cn.x), ~cn.y, tt);
@@ -1120,7 +1147,16 @@ template <class T> struct UvwArgs {
     const T *u, *v, *w, *visEff, *visEffB, *dudx, *dvdx, *dwdx;
     T *uB, *vB, *wB, *massFlux, *rface, *su, *sv, *sw, *au, *av, *aw;
     T gammaBlend, small;
+    const T *cuvw;  // uFace/vFace/wFace each source face leaves (k_uvw_carry_vals), for the tile schedule; else null
 };
+
+// uFace/vFace/wFace as INLET or OUTLET face g leaves them
+template <class T> __device__ v3<T> uvw_carry_from(const DevMesh<T> &m, const UvwArgs<T> &a, int g) {
+    const int2 cg = m.fconn[g];
+    const int ir = m.bregion[~cg.y];
+    if (m.rtype[ir] == CUPCFD_RTYPE_INLET) return ld3(m.ruvw, ir);
+    return v3<T>{a.u[cg.x], a.v[cg.x], a.w[cg.x]};
+}
 
 // uFace/vFace/wFace carried into interior face f (set by INLET and OUTLET faces)
 template <class T> __device__ v3<T> uvw_carry(const DevMesh<T> &m, const UvwArgs<T> &a, int f) {
@@ -1128,10 +1164,21 @@ template <class T> __device__ v3<T> uvw_carry(const DevMesh<T> &m, const UvwArgs
     if (!m.typed) return v3<T>{ci, ci, ci};
     const int g = m.carry_src[f];
     if (g < 0) return v3<T>{ci, ci, ci};
-    const int2 cg = m.fconn[g];
-    const int ir = m.bregion[~cg.y];
-    if (m.rtype[ir] == CUPCFD_RTYPE_INLET) return ld3(m.ruvw, ir);
-    return v3<T>{a.u[cg.x], a.v[cg.x], a.w[cg.x]};
+    return uvw_carry_from(m, a, g);
+}
+
+// uFace/vFace/wFace each INLET and OUTLET face leaves (the faces carry_src
+// names), for the tile schedule to read with one load per face: cuvw[3 (g -
+// Fi) + k] for boundary face g.
+template <class T> __global__ __launch_bounds__(kBlock) void k_uvw_carry_vals(DevMesh<T> m, UvwArgs<T> a, T *cuvw) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.F - m.Fi) return;
+    const int g = m.Fi + k;
+    const int it = m.rtype[m.bregion[~m.fconn[g].y]];
+    const T ci = m.carry_init;
+    const v3<T> uf = (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) ? uvw_carry_from(m, a, g)
+                                                                               : v3<T>{ci, ci, ci};
+    st3(cuvw, k, uf);
 }
 
 // visFace carried into SYMP face f (set by interior, INLET and OUTLET faces)
@@ -1170,7 +1217,8 @@ __device__ __forceinline__ UvwCell<T> uvw_cell(const DevMesh<T> &m, const UvwArg
 template <class T>
 __device__ __forceinline__ void uvw_interior(const DevMesh<T> &m, const UvwArgs<T> &a, int f, const V4<T> &s, T mf,
                                              T rl,
-                                             const UvwCell<T> &P, const UvwCell<T> &N, bool owner, T t[9]) {
+                                             const UvwCell<T> &P, const UvwCell<T> &N, bool owner, const v3<T> &uf,
+                                             T t[9]) {
     const T sx = s.x, sy = s.y, sz = s.z;
     const T facn = s.w;
     const T facp = 1.0 - facn;
@@ -1180,7 +1228,6 @@ __device__ __forceinline__ void uvw_interior(const DevMesh<T> &m, const UvwArgs<
     const T visac = N.vis * facn + P.vis * facp;
     const v3<T> xpn = sub3(N.cc, P.cc);
     const T visFace = visac * rl;
-    const v3<T> uf = uvw_carry(m, a, f);
     const T fuce = mf * uf.x, fvce = mf * uf.y, fwce = mf * uf.z;
     const T fude1 = (du.x + du.x) * sx + (du.y + dv.x) * sy + (du.z + dw.x) * sz;
     const T fvde1 = (du.y + dv.x) * sx + (dv.y + dv.y) * sy + (dv.z + dw.y) * sz;
@@ -1322,7 +1369,8 @@ __device__ __forceinline__ bool uvw_terms(const DevMesh<T> &m, const UvwArgs<T> 
     const V4<T> s = m.fgeoS[f];
     if (cn.y >= 0) {
         const bool in = (en.x & 1) != 0;
-        uvw_interior(m, a, f, s, a.massFlux[f], m.frl[f], uvw_cell(m, a, cn.x), uvw_cell(m, a, cn.y), !in, t);
+        uvw_interior(m, a, f, s, a.massFlux[f], m.frl[f], uvw_cell(m, a, cn.x), uvw_cell(m, a, cn.y), !in,
+                     uvw_carry(m, a, f), t);
         if (in)
             for (int k = 0; k < 9; k++) t[k] = -t[k];
         t[9] = nzero<T>();
@@ -1449,7 +1497,9 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
     bool owned;
     V4<T> s{};
     T mf = 0, rl = 0;
-    // slot sl's face and, for an interior face, its cell outside the tile into column TC + sl
+    v3<T> uf{m.carry_init, m.carry_init, m.carry_init};
+    // slot sl's face and, for an interior face, its carried uFace/vFace/wFace
+    // and its cell outside the tile into column TC + sl
     auto load = [&](int sl) {
         f = slot_conn(m, ts, sl, cn, owned);
         s = m.fgeoS[f];
@@ -1457,6 +1507,11 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
         if (cn.y >= 0) {
             mf = a.massFlux[f];
             rl = m.frl[f];
+            if (a.cuvw) {
+                const int g = m.carry_src[f];
+                const T ci = m.carry_init;
+                uf = g < 0 ? v3<T>{ci, ci, ci} : ld3(a.cuvw, g - m.Fi);
+            }
             out = outside_cell(cn, c0, nc);
             if (out >= 0) {
                 const UvwCell<T> o = uvw_cell(m, a, out);
@@ -1488,7 +1543,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
         T tt[10];
         bool fl = false;
         if (cn.y >= 0) {
-            uvw_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, tt);
+            uvw_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, uf, tt);
             tt[9] = nzero<T>();
         } else {
             fl = uvw_boundary(m, a, f, s, cell(cn.x), ~cn.y, tt);
@@ -2622,6 +2677,7 @@ int scalar_args(Mesh<T> &M, ScalarArgs<T, TS> &a, const TS *Phi, TS *PhiB, const
     a.ks = 1;
     a.write_state = 1;
     a.src = nullptr;
+    a.cval = nullptr;
     if (M.d.typed) {
         a.src = scalar_carry_src<T>(M, a.inletAssigns != 0, a.wallActive != 0);
         if (!a.src) return CUPCFD_E_ERROR;
@@ -2650,6 +2706,12 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
         const int w = tile_width(d);
+        if (lpc == kTile && a.src && d.F > d.Fi) {  // carried PhiFace per source face, read in phase A
+            T *cval = (T *)M->ws_get(53, (size_t)(d.F - d.Fi) * sizeof(T));
+            if (!cval) return CUPCFD_E_ERROR;
+            hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(d.F - d.Fi)), dim3(kBlock), 0, st, d, a, cval);
+            a.cval = cval;
+        }
         if (lpc == kTile && w == kTileCells + kTileSlots1)
             hipLaunchKernelGGL((k_scalar_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
                                gr.nblk);
@@ -2795,6 +2857,12 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
         const int w = tile_width(d);
+        if (lpc == kTile && d.typed && d.F > d.Fi) {  // carried uFace/vFace/wFace per source face, read in phase A
+            T *cuvw = (T *)M->ws_get(54, 3 * (size_t)(d.F - d.Fi) * sizeof(T));
+            if (!cuvw) return CUPCFD_E_ERROR;
+            hipLaunchKernelGGL(k_uvw_carry_vals<T>, dim3(blocks(d.F - d.Fi)), dim3(kBlock), 0, st, d, a, cuvw);
+            a.cuvw = cuvw;
+        }
         if (lpc == kTile && w == kTileCells + kTileSlots1)
             hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
                                gr.nblk);
