@@ -747,6 +747,10 @@ template <class T, class TS = T> struct ScalarArgs {
     T *rface;
     const int *src;  // PhiFace source per face (typed meshes), else null
     const T *cval;   // PhiFace each source face leaves (k_scalar_carry_vals), for the tile schedule; else null
+    // boundary face g's terms (Su x3, Au) and whether it updates Au, at
+    // bterm[4 (g - Fi)] and bflag[g - Fi] (k_scalar_bnd_terms), for the tile schedule
+    const T *bterm;
+    const uint8_t *bflag;
     // stride of the per-scalar arrays (phi, PhiB, Au, Su, dPhidx): 1 for one
     // scalar, K for scalar-innermost batches; write_state: this lane writes
     // the mesh's boundary Q/H/T
@@ -1032,14 +1036,38 @@ __global__ __launch_bounds__(LPC * 32) void k_scalar_vec(DevMesh<T> m, ScalarArg
     }
 }
 
+// The boundary faces of the scalar loop evaluated ahead of the tiles (they
+// depend on no other face's result): boundary face g = Fi + k leaves its terms
+// in bterm[4k .. 4k+3] and its Au flag in bflag[k], and writes PhiB and the
+// mesh's Q/H/T as the face loop does. This keeps the wall-function code out of
+// the tile kernel, whose registers then allow 7 waves per SIMD instead of 6.
+// Meshes with no typed region need no such pass: there every boundary face
+// adds -0.0 and writes nothing.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scalar_bnd_terms(DevMesh<T> m, ScalarArgs<T> a, T *bterm,
+                                                              uint8_t *bflag) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.F - m.Fi) return;
+    const int g = m.Fi + k;
+    const int2 cn = m.fconn[g];
+    T t[4] = {nzero<T>(), nzero<T>(), nzero<T>(), nzero<T>()};
+    bool fl = false;
+    if (m.rtype[m.bregion[~cn.y]] != CUPCFD_RTYPE_DEFAULT)  // a DEFAULT face adds -0.0 and writes nothing
+        fl = scalar_boundary(m, a, g, m.fgeoS[g], cn.x, scalar_cell(m, a, cn.x), ~cn.y, t);
+    *(V4<T> *)(bterm + 4 * (size_t)k) = V4<T>{t[0], t[1], t[2], t[3]};
+    bflag[k] = fl ? 1 : 0;
+}
+
 template <class T, int WC>
 __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, ScalarArgs<T> a, int per, int nblk) {
     extern __shared__ __align__(16) unsigned char smem[];
     // phi, VisEff, dPhidx xyz, centre xyz: NF rows of W columns, columns
     // [0, TC) the tile's cells, column TC + sl the cell of slot sl's face
     // outside the tile. Phase B overwrites rows 0..3 of column TC + sl with the
-    // slot's terms once the thread of slot sl has read that cell. W is WC when
-    // the mesh's tiles fit it (a compile-time row stride), else TC + tslots.
+    // slot's terms once the thread of slot sl has read that cell; a boundary
+    // slot's column gets its terms, evaluated ahead by k_scalar_bnd_terms, in
+    // phase A. W is WC when the mesh's tiles fit it (a compile-time row
+    // stride), else TC + tslots.
     constexpr int TC = kTileCells, NF = 8;
     const int S = m.tslots, W = WC ? WC : TC + S;
     T *lx = (T *)smem;                         // [NF][W]
@@ -1066,23 +1094,34 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
     // cell outside the tile into column TC + sl
     auto load = [&](int sl) {
         f = slot_conn(m, ts, sl, cn, owned);
-        s = m.fgeoS[f];
         out = -1;
-        if (cn.y >= 0) {
-            mf = a.massFlux[f];
-            rl = m.frl[f];
-            if (a.cval) {
-                const int g = a.src[f];
-                pf = g < 0 ? m.carry_init : a.cval[g - m.Fi];
+        if (cn.y < 0) {  // a boundary face: its terms, evaluated ahead, into column TC + sl
+            T *q = lx + TC + sl;
+            if (a.bterm) {
+                const int k = f - m.Fi;
+                const V4<T> bt = *(const V4<T> *)(a.bterm + 4 * (size_t)k);
+                q[0] = bt.x, q[W] = bt.y, q[2 * W] = bt.z, q[3 * W] = bt.w;
+                lf[sl] = a.bflag[k];
+            } else {  // no typed region: every boundary face adds -0.0 and leaves Au alone
+                q[0] = q[W] = q[2 * W] = q[3 * W] = nzero<T>();
+                lf[sl] = 0;
             }
-            out = outside_cell(cn, c0, nc);
-            if (out >= 0) {
-                const ScalarCell<T> o = scalar_cell(m, a, out);
-                T *q = lx + TC + sl;
-                q[0] = o.phi, q[W] = o.vis;
-                q[2 * W] = o.g.x, q[3 * W] = o.g.y, q[4 * W] = o.g.z;
-                q[5 * W] = o.cc.x, q[6 * W] = o.cc.y, q[7 * W] = o.cc.z;
-            }
+            return;
+        }
+        s = m.fgeoS[f];
+        mf = a.massFlux[f];
+        rl = m.frl[f];
+        if (a.cval) {
+            const int g = a.src[f];
+            pf = g < 0 ? m.carry_init : a.cval[g - m.Fi];
+        }
+        out = outside_cell(cn, c0, nc);
+        if (out >= 0) {
+            const ScalarCell<T> o = scalar_cell(m, a, out);
+            T *q = lx + TC + sl;
+            q[0] = o.phi, q[W] = o.vis;
+            q[2 * W] = o.g.x, q[3 * W] = o.g.y, q[4 * W] = o.g.z;
+            q[5 * W] = o.cc.x, q[6 * W] = o.cc.y, q[7 * W] = o.cc.z;
         }
     };
     if (tid < ts.nslot) load(tid);
@@ -1095,17 +1134,13 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
             return ScalarCell<T>{q[0], q[W], v3<T>{q[2 * W], q[3 * W], q[4 * W]},
                                  v3<T>{q[5 * W], q[6 * W], q[7 * W]}};
         };
-        T tt[4];
-        bool fl = false;
-        if (cn.y >= 0) {
-            scalar_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, pf, tt);
-            tt[3] = nzero<T>();
-        } else {
-            fl = scalar_boundary(m, a, f, s, cn.x, cell(cn.x), ~cn.y, tt);
-        }
+        if (cn.y < 0) continue;  // boundary slots were filled in by load()
+        T tt[3];
+        scalar_interior(m, a, f, s, mf, rl, cell(cn.x), cell(cn.y), owned, pf, tt);
         T *q = lx + TC + sl;
-        for (int j = 0; j < 4; j++) q[j * W] = tt[j];
-        lf[sl] = fl ? 1 : 0;
+        for (int j = 0; j < 3; j++) q[j * W] = tt[j];
+        q[3 * W] = nzero<T>();
+        lf[sl] = 0;
     }
     __syncthreads();
     // C: Su, then Au for cells with an Au-updating face
@@ -2678,6 +2713,8 @@ int scalar_args(Mesh<T> &M, ScalarArgs<T, TS> &a, const TS *Phi, TS *PhiB, const
     a.write_state = 1;
     a.src = nullptr;
     a.cval = nullptr;
+    a.bterm = nullptr;
+    a.bflag = nullptr;
     if (M.d.typed) {
         a.src = scalar_carry_src<T>(M, a.inletAssigns != 0, a.wallActive != 0);
         if (!a.src) return CUPCFD_E_ERROR;
@@ -2706,11 +2743,21 @@ int dev_scalar(cupcfd_hip_mesh *h, const T *Phi, T *PhiB, const T *VisEff, T *Au
         const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
         hipStream_t st = (hipStream_t)stream;
         const int w = tile_width(d);
-        if (lpc == kTile && a.src && d.F > d.Fi) {  // carried PhiFace per source face, read in phase A
-            T *cval = (T *)M->ws_get(53, (size_t)(d.F - d.Fi) * sizeof(T));
-            if (!cval) return CUPCFD_E_ERROR;
-            hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(d.F - d.Fi)), dim3(kBlock), 0, st, d, a, cval);
-            a.cval = cval;
+        if (lpc == kTile && d.typed && d.F > d.Fi) {
+            const int nb = d.F - d.Fi;
+            // the boundary faces' terms, PhiB and mesh state ahead of the tiles
+            T *bterm = (T *)M->ws_get(55, 4 * (size_t)nb * sizeof(T));
+            uint8_t *bflag = (uint8_t *)M->ws_get(56, (size_t)nb);
+            if (!bterm || !bflag) return CUPCFD_E_ERROR;
+            hipLaunchKernelGGL(k_scalar_bnd_terms<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, bterm, bflag);
+            a.bterm = bterm;
+            a.bflag = bflag;
+            if (a.src) {  // carried PhiFace per source face, read in phase A
+                T *cval = (T *)M->ws_get(53, (size_t)nb * sizeof(T));
+                if (!cval) return CUPCFD_E_ERROR;
+                hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, cval);
+                a.cval = cval;
+            }
         }
         if (lpc == kTile && w == kTileCells + kTileSlots1)
             hipLaunchKernelGGL((k_scalar_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
