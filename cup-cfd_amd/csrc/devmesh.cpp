@@ -18,6 +18,8 @@ template <class T> Mesh<T>::~Mesh() {
         if (sc_src[k]) (void)hipFree(sc_src[k]);
     if (rbnd) (void)hipFree(rbnd);
     if (rbnd_off) (void)hipFree(rbnd_off);
+    if (cell_order_d) (void)hipFree(cell_order_d);
+    if (face_order_d) (void)hipFree(face_order_d);
 }
 
 template <class T> void *Mesh<T>::ws_get(int slot, size_t bytes) {
@@ -523,6 +525,23 @@ template <class T> bool region_boundaries(Mesh<T> &M) {
 
 template bool region_boundaries<double>(Mesh<double> &);
 template bool region_boundaries<float>(Mesh<float> &);
+
+template <class T> const int *device_order(Mesh<T> &M, bool faces) {
+    int *&d = faces ? M.face_order_d : M.cell_order_d;
+    if (d) return d;
+    const std::vector<int32_t> &h = faces ? M.face_order : M.cell_order;
+    void *p = nullptr;
+    if (hipMalloc(&p, (h.empty() ? 1 : h.size()) * sizeof(int)) != hipSuccess) return nullptr;
+    if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    d = (int *)p;
+    return d;
+}
+
+template const int *device_order<double>(Mesh<double> &, bool);
+template const int *device_order<float>(Mesh<float> &, bool);
 
 }  // namespace cupcfd_hip
 

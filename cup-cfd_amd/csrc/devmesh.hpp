@@ -120,6 +120,9 @@ template <class T> struct Mesh {
     // rbnd[rbnd_off[r] .. rbnd_off[r + 1])), built on first use by the
     // per-region ordered sums
     int *rbnd = nullptr, *rbnd_off = nullptr;
+    // device copies of cell_order / face_order for the drop-in staging,
+    // uploaded on first use
+    int *cell_order_d = nullptr, *face_order_d = nullptr;
     // order-preserving face colouring, built on first use of the colour-group
     // schedule (-1: not built): colour(f) = 1 + the largest colour of the face
     // before f in the rows of f's cells, so colours rise along every row and no
@@ -138,6 +141,13 @@ template <class T> struct Mesh {
 template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns);
 // M.rbnd / M.rbnd_off, built on first call; false if they cannot be allocated
 template <class T> bool region_boundaries(Mesh<T> &M);
+// Device copy of M.cell_order (faces = false) or M.face_order, uploaded on first call; null on failure
+template <class T> const int *device_order(Mesh<T> &M, bool faces);
+// Drop-in staging on the device (null stream, asynchronous): dst[k*ncomp + j]
+// = raw[order[k]*stride + j] (mesh order -> internal), and raw[order[k]*ncomp
+// + j] = src[k*ncomp + j] (internal -> mesh order), for k < n, j < ncomp
+template <class T> int stage_gather(T *dst, const T *raw, const int *order, int n, int ncomp, int64_t stride);
+template <class T> int stage_scatter(T *raw, const T *src, const int *order, int n, int ncomp);
 int sync_state_any(struct ::cupcfd_hip_mesh *h, bool push);
 
 // Counts and pointers a MiniSoA view must carry before either repack reads it.

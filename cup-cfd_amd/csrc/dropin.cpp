@@ -20,11 +20,14 @@ template <> Mesh<double> *mesh_of<double>(cupcfd_hip_mesh *h) { return h && h->p
 template <> Mesh<float> *mesh_of<float>(cupcfd_hip_mesh *h) { return h && h->precision == 32 ? h->m32 : nullptr; }
 
 // Staging of one drop-in call. Device buffers come from the mesh's workspace
-// pool (slots 0..31); host packing buffers are reused per slot.
+// pool (slots 0..31 for the kernels' arrays, 64..127 for the raw copies of
+// the caller's arrays). The host makes contiguous copies of the caller's
+// arrays, in the mesh's numbering; the permutation to and from the internal
+// numbering runs on the device (stage_gather / stage_scatter).
 template <class T> struct Stage {
     using V3 = typename V3Of<T>::type;
     Mesh<T> &M;
-    int slot = 0;
+    int slot = 0, raw_slot = 0;
     int rc = CUPCFD_E_SUCCESS;
     std::vector<T> h;
     explicit Stage(Mesh<T> &m) : M(m) {}
@@ -51,6 +54,49 @@ template <class T> struct Stage {
         }
         return true;
     }
+    T *raw(size_t n) {
+        if (rc != CUPCFD_E_SUCCESS) return nullptr;
+        if (raw_slot > 63) {
+            rc = CUPCFD_E_ERROR;
+            return nullptr;
+        }
+        T *p = (T *)M.ws_get(64 + raw_slot++, (n ? n : 1) * sizeof(T));
+        if (!p) rc = CUPCFD_E_ERROR;
+        return p;
+    }
+    const int *order(bool faces) {
+        if (rc != CUPCFD_E_SUCCESS) return nullptr;
+        const int *o = device_order(M, faces);
+        if (!o) rc = CUPCFD_E_ERROR;
+        return o;
+    }
+    void copy_in(void *d, const void *src, size_t bytes) {
+        if (rc == CUPCFD_E_SUCCESS && bytes && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = CUPCFD_E_ERROR;
+    }
+    void copy_out(void *dst, const void *d, size_t bytes) {
+        if (rc == CUPCFD_E_SUCCESS && bytes && hipMemcpy(dst, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = CUPCFD_E_ERROR;
+    }
+    void ok(int r) {
+        if (rc == CUPCFD_E_SUCCESS && r != CUPCFD_E_SUCCESS) rc = r;
+    }
+    // n items of ncomp values, mesh order (item i at src + i * stride values) -> internal order on the device
+    T *permuted(const T *src, int n, int ncomp, int64_t stride, bool faces) {
+        const int *o = order(faces);
+        const size_t span = n ? (size_t)(n - 1) * stride + ncomp : 0;  // values from src that are read
+        T *r = raw(span), *d = dev((size_t)n * ncomp);
+        copy_in(r, src, span * sizeof(T));
+        if (rc == CUPCFD_E_SUCCESS) ok(stage_gather<T>(d, r, o, n, ncomp, stride));
+        return d;
+    }
+    // n items of ncomp values, internal order on the device -> packed mesh order at dst
+    void unpermuted(const T *d, T *dst, int n, int ncomp, bool faces) {
+        const int *o = order(faces);
+        T *r = raw((size_t)n * ncomp);
+        if (rc == CUPCFD_E_SUCCESS) ok(stage_scatter<T>(r, d, o, n, ncomp));
+        copy_out(dst, r, (size_t)n * ncomp * sizeof(T));
+    }
     T *upload(size_t n) {
         T *d = dev(n);
         if (d && n && hipMemcpy(d, h.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) rc = CUPCFD_E_ERROR;
@@ -69,14 +115,15 @@ template <class T> struct Stage {
     T *cells(const T *src, int n) {  // ncomp 1, mesh order -> internal
         const int C = M.d.C;
         if (!check(src, n, C)) return nullptr;
-        h.resize(C);
-        for (int k = 0; k < C; k++) h[k] = src[M.cell_order[k]];
-        return upload(C);
+        return permuted(src, C, 1, 1, false);
     }
     T *cells3(const V3 &v, int n) {
         const int C = M.d.C;
         if (!check(v.base, n, C)) return nullptr;
-        h.resize(3 * (size_t)C);
+        const int64_t stride = v.stride ? v.stride : (int64_t)(3 * sizeof(T));
+        if (stride % (int64_t)sizeof(T) == 0)
+            return permuted((const T *)v.base, C, 3, stride / (int64_t)sizeof(T), false);
+        h.resize(3 * (size_t)C);  // a stride that is not a whole number of values: packed on the host
         for (int k = 0; k < C; k++)
             for (int j = 0; j < 3; j++) h[3 * (size_t)k + j] = *at3(v, M.cell_order[k], j);
         return upload(3 * (size_t)C);
@@ -84,10 +131,7 @@ template <class T> struct Stage {
     T *faces(const T *src, int n, int ncomp) {
         const int F = M.d.F;
         if (!check(src, n, (int64_t)F * ncomp)) return nullptr;
-        h.resize((size_t)F * ncomp);
-        for (int k = 0; k < F; k++)
-            for (int j = 0; j < ncomp; j++) h[(size_t)k * ncomp + j] = src[(size_t)M.face_order[k] * ncomp + j];
-        return upload((size_t)F * ncomp);
+        return permuted(src, F, ncomp, ncomp, true);
     }
     T *plain(const T *src, int n, int need) {  // boundary / region arrays
         if (!check(src, n, need)) return nullptr;
@@ -99,26 +143,23 @@ template <class T> struct Stage {
         return upload(1);
     }
     // --- outputs ------------------------------------------------------------
-    void cells_out(const T *d, T *dst) {
-        const int C = M.d.C;
-        download(d, C);
-        if (rc == CUPCFD_E_SUCCESS)
-            for (int k = 0; k < C; k++) dst[M.cell_order[k]] = h[k];
-    }
+    void cells_out(const T *d, T *dst) { unpermuted(d, dst, M.d.C, 1, false); }
     void cells3_out(const T *d, const V3 &v) {
         const int C = M.d.C;
-        download(d, 3 * (size_t)C);
+        const int64_t stride = v.stride ? v.stride : (int64_t)(3 * sizeof(T));
+        if (stride == (int64_t)(3 * sizeof(T))) {  // packed: straight into the caller's array
+            unpermuted(d, (T *)v.base, C, 3, false);
+            return;
+        }
+        // strided (e.g. vector objects with a vptr): packed on the host, then
+        // only the three components of each element are written
+        h.resize(3 * (size_t)C);
+        unpermuted(d, h.data(), C, 3, false);
         if (rc == CUPCFD_E_SUCCESS)
             for (int k = 0; k < C; k++)
-                for (int j = 0; j < 3; j++) *at3(v, M.cell_order[k], j) = h[3 * (size_t)k + j];
+                for (int j = 0; j < 3; j++) *at3(v, k, j) = h[3 * (size_t)k + j];
     }
-    void faces_out(const T *d, T *dst, int ncomp) {
-        const int F = M.d.F;
-        download(d, (size_t)F * ncomp);
-        if (rc == CUPCFD_E_SUCCESS)
-            for (int k = 0; k < F; k++)
-                for (int j = 0; j < ncomp; j++) dst[(size_t)M.face_order[k] * ncomp + j] = h[(size_t)k * ncomp + j];
-    }
+    void faces_out(const T *d, T *dst, int ncomp) { unpermuted(d, dst, M.d.F, ncomp, true); }
     void plain_out(const T *d, T *dst, int n) {
         download(d, n);
         if (rc == CUPCFD_E_SUCCESS && n) std::memcpy(dst, h.data(), n * sizeof(T));
@@ -396,7 +437,7 @@ using namespace cupcfd_hip;
             st.plain_out(d_wB, wBoundary, B);                                                                      \
             rc = st.finish();                                                                                      \
         }                                                                                                          \
-        if (rc == CUPCFD_E_SUCCESS && wall) rc = cupcfd_hip_mesh_pull_state(mesh);                                       \
+        if (rc == CUPCFD_E_SUCCESS && wall) rc = cupcfd_hip_mesh_pull_state(mesh);                                 \
         return rc;                                                                                                 \
     }                                                                                                              \
     int cupcfd_fvm_flux_uvw_region_loop1_##S(cupcfd_hip_mesh *mesh) {                                              \

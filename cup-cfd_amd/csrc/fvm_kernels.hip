@@ -2482,6 +2482,44 @@ template <class T> __global__ __launch_bounds__(kBlock) void k_fill_timing(T *x,
     x[i] = (T)(0.5 + (double)(h & 0xffffu) / 65536.0);
 }
 
+// drop-in staging: permutations between the mesh's numbering and the internal
+// one, run on the device so the host only makes contiguous copies
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stage_gather(T *__restrict__ dst, const T *__restrict__ raw,
+                                                         const int *__restrict__ order, int64_t total, int ncomp,
+                                                         int64_t stride) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int64_t k = i / ncomp;
+    dst[i] = raw[(int64_t)order[k] * stride + (i - k * ncomp)];
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stage_scatter(T *__restrict__ raw, const T *__restrict__ src,
+                                                          const int *__restrict__ order, int64_t total, int ncomp) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int64_t k = i / ncomp;
+    raw[(int64_t)order[k] * ncomp + (i - k * ncomp)] = src[i];
+}
+template <class T> int stage_gather(T *dst, const T *raw, const int *order, int n, int ncomp, int64_t stride) {
+    const int64_t total = (int64_t)n * ncomp;
+    if (total > 0)
+        hipLaunchKernelGGL(k_stage_gather<T>, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           nullptr, dst, raw, order, total, ncomp, stride);
+    return hipGetLastError() == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+}
+template <class T> int stage_scatter(T *raw, const T *src, const int *order, int n, int ncomp) {
+    const int64_t total = (int64_t)n * ncomp;
+    if (total > 0)
+        hipLaunchKernelGGL(k_stage_scatter<T>, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           nullptr, raw, src, order, total, ncomp);
+    return hipGetLastError() == hipSuccess ? CUPCFD_E_SUCCESS : CUPCFD_E_ERROR;
+}
+template int stage_gather<double>(double *, const double *, const int *, int, int, int64_t);
+template int stage_gather<float>(float *, const float *, const int *, int, int, int64_t);
+template int stage_scatter<double>(double *, const double *, const int *, int, int);
+template int stage_scatter<float>(float *, const float *, const int *, int, int);
+
 }  // namespace cupcfd_hip
 
 // ===========================================================================
