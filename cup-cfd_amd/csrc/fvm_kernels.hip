@@ -437,9 +437,10 @@ __global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, Grad
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int TC = kTileCells, NF = SWEEP0 ? 1 : 7;  // phi (+ dPhidxo xyz, centre xyz)
     const int S = m.tslots;
-    T *lc = (T *)smem;                        // [NF][TC]
-    T *lt = lc + NF * TC;                     // [3][S]
-    uint16_t *le = (uint16_t *)(lt + 3 * S);  // [tnnz]
+    T *lc = (T *)smem;                         // [NF][TC]
+    T *lt = lc + NF * TC;                      // [3][S]
+    T *lo = lt + 3 * S;                        // [TC][3]: the tile's gradients, packed for the stores
+    uint16_t *le = (uint16_t *)(lo + 3 * TC);  // [tnnz]
     const int t = xcd_block(blockIdx.x, per);
     if (t >= nblk) return;
     const int tid = threadIdx.x;
@@ -497,11 +498,19 @@ __global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, Grad
         for (int q = 0; q < 3; q++) lt[q * S + sl] = tt[q];
     }
     __syncthreads();
-    // C
+    // C: fold, then store the tile's 3 * nc consecutive values with
+    // consecutive threads (the folds are component-major)
     if (fold) {
         T acc = 0;
         for (int e = e0; e < e0 + n; e++) acc = acc + lt[k * S + (le[e - eb] & 0x7fff)];
-        gradient_store(a, c0 + cl, k, acc, vol);
+        const T fact = 1.0 / vol;
+        lo[cl * 3 + k] = acc * fact;
+    }
+    __syncthreads();
+    for (int i = tid; i < 3 * nc; i += kTileBlock) {
+        const T r = lo[i];
+        a.out1[3 * (size_t)c0 + i] = r;
+        if (a.out2) a.out2[3 * (size_t)c0 + i] = r;
     }
 }
 
@@ -2626,7 +2635,8 @@ int dev_gradient(cupcfd_hip_mesh *h, int nGradient, const T *phi, const T *phiB,
         if (!t[0] || !t[1]) return CUPCFD_E_ERROR;
     }
     size_t lds;
-    const int lpc = gather_schedule(*M, kGatherGradient, 7, 3, false, lds);
+    // tile LDS: 7 cell fields and the 3 output components per cell, 3 term rows of tslots
+    const int lpc = gather_schedule(*M, kGatherGradient, 10, 3, false, lds);
     const Grid gr = lpc == kTile ? xcd_grid(d.ntiles, 1) : xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
     const int nthr = lpc == kTile ? kTileBlock : (lpc ? lpc * kCellsPerBlock : kBlock);
     T *acc = nullptr;
