@@ -2458,28 +2458,47 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
 }
 
 // one thread per face, adding its K scalars' terms in order after the RFace term
-template <class T, class TS>
+// The mass-flux cotangent of the batched adjoint, G lanes per face (G = 4,
+// 16 or 64, the smallest that holds K; K > 64 takes chunks of 64): lane j of
+// a face's group evaluates scalar base + j's term, reading the per-scalar rows
+// coalesced, and puts it in LDS; every lane of the group then adds the terms
+// in scalar order, each read at a group-uniform LDS address. So MassFluxBar
+// is the RFace term followed by the K scalars' terms, added in order, as K
+// single-scalar calls accumulate it.
+template <class T, class TS, int G>
 __global__ __launch_bounds__(kBlock) void k_adj_faces_batched(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K) {
-    const int f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= m.F) return;
-    const int2 cn = m.fconn[f];
-    if (cn.y < 0) {
-        a.mfbar[f] = 0.0;
-        return;
-    }
-    const T mf = a.s.massFlux[f];
+    constexpr int FPB = kBlock / G;  // faces per block
+    __shared__ T lds[kBlock];
+    const int grp = threadIdx.x / G, j = threadIdx.x % G;
+    const int f = blockIdx.x * FPB + grp;
+    const bool valid = f < m.F;
+    const int2 cn = valid ? m.fconn[f] : make_int2(0, -1);
+    const bool interior = valid && cn.y >= 0;
+    const T mf = interior ? a.s.massFlux[f] : (T)0.0;
     const T PhiFace = m.carry_init;
-    T acc = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
-    const TS *sp = a.subar + (size_t)cn.x * K, *sn = a.subar + (size_t)cn.y * K;
-    const TS *pu = mf < 0 ? a.s.phi + (size_t)cn.y * K : a.s.phi + (size_t)cn.x * K;  // the upwind cell's phi
+    T acc = 0.0;
+    if (interior) acc = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
     const bool up = mf < 0 || mf > 0;
-    for (int kk = 0; kk < K; kk++) {
-        const T D = (T)sp[kk] - (T)sn[kk];
-        const T Dg = D * a.s.gammaBlend;
-        const T dfci = up ? (T)pu[kk] : (T)0.0;
-        acc = acc + (-(Dg * (PhiFace - dfci)));
+    const TS *sp = a.subar + (size_t)cn.x * K, *sn = a.subar + (size_t)max(cn.y, 0) * K;
+    const TS *pu = a.s.phi + (size_t)(mf < 0 ? max(cn.y, 0) : cn.x) * K;  // the upwind cell's phi
+    T *lg = lds + grp * G;
+    for (int base = 0; base < K; base += G) {
+        const int kk = base + j;
+        T t = 0.0;
+        if (interior && kk < K) {
+            const T D = (T)sp[kk] - (T)sn[kk];
+            const T Dg = D * a.s.gammaBlend;
+            const T dfci = up ? (T)pu[kk] : (T)0.0;
+            t = -(Dg * (PhiFace - dfci));
+        }
+        lg[j] = t;
+        __syncthreads();
+        const int cnt = min(G, K - base);
+        if (interior)
+            for (int i = 0; i < cnt; i++) acc = acc + lg[i];
+        __syncthreads();
     }
-    a.mfbar[f] = acc;
+    if (valid && j == 0) a.mfbar[f] = interior ? acc : (T)0.0;
 }
 
 // values in [0.5, 1.5) from a hash of the index: finite, non-zero inputs for timing runs
@@ -3077,8 +3096,13 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
 #undef ADJ_BATCHED_ROWS
 #undef ADJ_BATCHED_LAUNCH
     }
-    if (d.F)
-        hipLaunchKernelGGL((k_adj_faces_batched<T, TS>), dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, K);
+    if (d.F) {
+        const int G = batch_group(K);
+        const dim3 grid((unsigned)((d.F + kBlock / G - 1) / (kBlock / G)));
+        if (G == 4) hipLaunchKernelGGL((k_adj_faces_batched<T, TS, 4>), grid, dim3(kBlock), 0, st, d, a, K);
+        else if (G == 16) hipLaunchKernelGGL((k_adj_faces_batched<T, TS, 16>), grid, dim3(kBlock), 0, st, d, a, K);
+        else hipLaunchKernelGGL((k_adj_faces_batched<T, TS, 64>), grid, dim3(kBlock), 0, st, d, a, K);
+    }
     return launch_ok();
 }
 
