@@ -1922,8 +1922,8 @@ __device__ __forceinline__ void gradient_batched_cell(const M &m, const BatchGra
     for (int j = 0; j < 3; j++) {
         const T r = acc[j] * fact;
         const size_t o = ((size_t)c * 3 + j) * K + kk;
-        a.out1[o] = (TO)r;
-        if (a.out2) a.out2[o] = (TO)r;
+        st_stream(a.out1 + o, (TO)r);
+        if (a.out2) st_stream(a.out2 + o, (TO)r);
     }
 }
 
@@ -2406,8 +2406,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
         }
     }
     const T fact = 1.0 / m.cvol[c];
-    for (int j = 0; j < 3; j++) a.h[((size_t)c * 3 + j) * K + kk] = acc[j] * fact;
-    a.dir[(size_t)c * K + kk] = acc[3];
+    for (int j = 0; j < 3; j++) st_stream(a.h + ((size_t)c * 3 + j) * K + kk, acc[j] * fact);
+    st_stream(a.dir + (size_t)c * K + kk, acc[3]);
 }
 
 template <class T, class TS, int G, int RL>
@@ -2454,7 +2454,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
             term(en, ld3s(a.h + kk, ip, K), ld3s(a.h + kk, nn, K));
         }
     }
-    a.phibar[(size_t)c * K + kk] = (TS)acc;
+    st_stream(a.phibar + (size_t)c * K + kk, (TS)acc);
 }
 
 // one thread per face, adding its K scalars' terms in order after the RFace term
