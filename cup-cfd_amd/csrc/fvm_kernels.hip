@@ -2156,6 +2156,7 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched(DevMesh<T> m, ScalarA
 //                         dir_c = sum_f (D_f*GammaBlend) * (c == P ? max(mf,0) : min(mf,0))
 //   pass B (per cell c):  phibar_c = dir_c + sum_f fac * dot(h_P + h_N, S_f)
 //   faces:                mfbar_f = -(D_f*GammaBlend*(PhiFace - dfci)) + rterm
+//                         (written in pass A by the face's cell1 entry)
 // with fac = facp for c == P, facn for c == N, and the sums over each cell's
 // faces in row order (ascending mesh face id). Boundary faces add nothing.
 // ===========================================================================
@@ -2170,6 +2171,7 @@ template <class T>
 __device__ __forceinline__ void adj_a_terms(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en, T t[4]) {
     if (en.y < 0) {
         t[0] = t[1] = t[2] = t[3] = nzero<T>();
+        a.mfbar[en.x >> 1] = 0.0;  // a boundary face has no mass-flux cotangent here
         return;
     }
     const int f = en.x >> 1;
@@ -2191,6 +2193,12 @@ __device__ __forceinline__ void adj_a_terms(const DevMesh<T> &m, const AdjArgs<T
     t[2] = tt * (Visac * s.z - VisFace * Xpn.z);
     const T mf = a.s.massFlux[f];
     t[3] = Dg * (in ? (T)fmin((double)mf, 0.0) : (T)fmax((double)mf, 0.0));
+    if (!in) {  // the face's cell1 entry (one per face) also writes its mass-flux cotangent
+        const T dfci = mf < 0 ? a.s.phi[nn] : (mf > 0 ? a.s.phi[ip] : (T)0.0);
+        const T rterm = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
+        const T PhiFace = m.carry_init;
+        a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
+    }
 }
 
 // Pass-B term of CSR entry en of cell c
@@ -2290,24 +2298,6 @@ __global__ __launch_bounds__(LPC * 32) void k_adj_b_vec(DevMesh<T> m, AdjArgs<T>
     T acc = a.dir[c];
     for (int i = 0; i < n; i++) acc = acc + lds[cl * P + i];
     a.phibar[c] = acc;
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_faces(DevMesh<T> m, AdjArgs<T> a) {
-    const int f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= m.F) return;
-    const int2 cn = m.fconn[f];
-    if (cn.y < 0) {
-        a.mfbar[f] = 0.0;
-        return;
-    }
-    const T mf = a.s.massFlux[f];
-    const T D = a.subar[cn.x] - a.subar[cn.y];
-    const T Dg = D * a.s.gammaBlend;
-    const T dfci = mf < 0 ? a.s.phi[cn.y] : (mf > 0 ? a.s.phi[cn.x] : (T)0.0);
-    const T rterm = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
-    const T PhiFace = m.carry_init;
-    a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
 }
 
 // ---------------------------------------------------------------------------
@@ -2457,7 +2447,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
     st_stream(a.phibar + (size_t)c * K + kk, (TS)acc);
 }
 
-// one thread per face, adding its K scalars' terms in order after the RFace term
 // The mass-flux cotangent of the batched adjoint, G lanes per face (G = 4,
 // 16 or 64, the smallest that holds K; K > 64 takes chunks of 64): lane j of
 // a face's group evaluates scalar base + j's term, reading the per-scalar rows
@@ -3044,7 +3033,7 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
             hipLaunchKernelGGL(k_adj_b<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
         }
     }
-    if (d.F) hipLaunchKernelGGL(k_adj_faces<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a);
+
     return launch_ok();
 }
 
