@@ -498,19 +498,25 @@ __global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, Grad
         for (int q = 0; q < 3; q++) lt[q * S + sl] = tt[q];
     }
     __syncthreads();
-    // C: fold, then store the tile's 3 * nc consecutive values with
+    // C: fold, then (FP64) store the tile's 3 * nc consecutive values with
     // consecutive threads (the folds are component-major)
-    if (fold) {
+    if constexpr (sizeof(T) == 8) {
+        if (fold) {
+            T acc = 0;
+            for (int e = e0; e < e0 + n; e++) acc = acc + lt[k * S + (le[e - eb] & 0x7fff)];
+            const T fact = 1.0 / vol;
+            lo[cl * 3 + k] = acc * fact;
+        }
+        __syncthreads();
+        for (int i = tid; i < 3 * nc; i += kTileBlock) {
+            const T r = lo[i];
+            a.out1[3 * (size_t)c0 + i] = r;
+            if (a.out2) a.out2[3 * (size_t)c0 + i] = r;
+        }
+    } else if (fold) {
         T acc = 0;
         for (int e = e0; e < e0 + n; e++) acc = acc + lt[k * S + (le[e - eb] & 0x7fff)];
-        const T fact = 1.0 / vol;
-        lo[cl * 3 + k] = acc * fact;
-    }
-    __syncthreads();
-    for (int i = tid; i < 3 * nc; i += kTileBlock) {
-        const T r = lo[i];
-        a.out1[3 * (size_t)c0 + i] = r;
-        if (a.out2) a.out2[3 * (size_t)c0 + i] = r;
+        gradient_store(a, c0 + cl, k, acc, vol);
     }
 }
 
