@@ -1816,15 +1816,42 @@ template <class T, class TI, class TO> struct BatchGradArgs {
 
 // The fields of a device mesh the batched gradient reads (held in registers
 // by the mesh-set kernel, which switches meshes between work units)
+// Read-only device memory seen through the constant address space, so that
+// wave-uniform loads become scalar loads even where the compiler cannot show
+// that a work unit's stores do not alias the next unit's geometry. operator[]
+// loads by value (4- and 8-byte words; a V4 as four values).
+template <class X> struct RoArray {
+    const __attribute__((address_space(4))) X *p;
+    __device__ __forceinline__ X operator[](int64_t i) const { return p[i]; }
+};
+template <> struct RoArray<int2> {
+    const __attribute__((address_space(4))) long long *p;
+    __device__ __forceinline__ int2 operator[](int64_t i) const {
+        const long long v = p[i];
+        return make_int2((int)v, (int)(v >> 32));
+    }
+};
+template <class T> struct RoArray<V4<T>> {
+    const __attribute__((address_space(4))) T *p;
+    __device__ __forceinline__ V4<T> operator[](int64_t i) const {
+        return V4<T>{p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]};
+    }
+};
+template <class X, class Q> __device__ __forceinline__ X ro(const Q *q) {
+    X a;
+    a.p = (decltype(a.p))q;
+    return a;
+}
 template <class T> struct GradGeom {
     int C, rowlen;
-    const int *row;
-    const int2 *ent;
-    const V4<T> *fgeoS, *fgeoX, *cgeo;
-    const T *cvol;
+    RoArray<int> row;
+    RoArray<int2> ent;
+    RoArray<V4<T>> fgeoS, fgeoX, cgeo;
+    RoArray<T> cvol;
     __device__ __forceinline__ explicit GradGeom(const DevMesh<T> &m)
-        : C(m.C), rowlen(m.rowlen), row(m.row), ent(m.ent), fgeoS(m.fgeoS), fgeoX(m.fgeoX), cgeo(m.cgeo),
-          cvol(m.cvol) {}
+        : C(m.C), rowlen(m.rowlen), row(ro<RoArray<int>>(m.row)), ent(ro<RoArray<int2>>(m.ent)),
+          fgeoS(ro<RoArray<V4<T>>>(m.fgeoS)), fgeoX(ro<RoArray<V4<T>>>(m.fgeoX)), cgeo(ro<RoArray<V4<T>>>(m.cgeo)),
+          cvol(ro<RoArray<T>>(m.cvol)) {}
 };
 
 // a value every lane of the wave holds, moved to scalar registers
@@ -1835,15 +1862,22 @@ template <class P> __device__ __forceinline__ P *uni(P *p) {
     const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
     return (P *)((hi << 32) | lo);
 }
+template <class A> __device__ __forceinline__ A uni_ro(A a) {
+    const uint64_t v = (uint64_t)a.p;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    a.p = (decltype(a.p))((hi << 32) | lo);
+    return a;
+}
 template <class T> __device__ __forceinline__ GradGeom<T> uni(GradGeom<T> g) {
     g.C = uni(g.C);
     g.rowlen = uni(g.rowlen);
-    g.row = uni(g.row);
-    g.ent = uni(g.ent);
-    g.fgeoS = uni(g.fgeoS);
-    g.fgeoX = uni(g.fgeoX);
-    g.cgeo = uni(g.cgeo);
-    g.cvol = uni(g.cvol);
+    g.row = uni_ro(g.row);
+    g.ent = uni_ro(g.ent);
+    g.fgeoS = uni_ro(g.fgeoS);
+    g.fgeoX = uni_ro(g.fgeoX);
+    g.cgeo = uni_ro(g.cgeo);
+    g.cvol = uni_ro(g.cvol);
     return g;
 }
 template <class T, class TI, class TO>
