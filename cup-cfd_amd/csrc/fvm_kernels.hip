@@ -1975,6 +1975,75 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, Batch
     gradient_batched_cell<T, TI, TO, SWEEP0, RL>(m, a, K, batch_uniform<G>(L.c), L.k);
 }
 
+// Batched gradient, first sweep, for lane groups narrower than a wave (G =
+// 16: a wave spans 4 cells, so its lanes cannot share scalar loads) on meshes
+// whose rows all hold RL entries. The block's rows, face vectors and cell
+// volumes are loaded once into LDS, one lane per entry, instead of by all G
+// lanes of each cell: the rows before the first barrier, the face vectors and
+// volumes while every lane's phi gathers are in flight, before the second.
+// Terms and their order are gradient_batched_cell's.
+template <class T, class TI, class TO, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_gradient_batched_staged(DevMesh<T> m, BatchGradArgs<T, TI, TO> a, int K,
+                                                                    int per, int nblk) {
+    constexpr int NC = kBlock / G, NP = NC * RL;
+    __shared__ int2 sen[NP];
+    __shared__ V4<T> ss[NP];
+    __shared__ T svol[NC];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cb = lb * NC, tid = threadIdx.x;
+    for (int p = tid; p < NP; p += kBlock) {
+        const int c = cb + p / RL;
+        sen[p] = c < m.C ? m.ent[(size_t)c * RL + p % RL] : make_int2(0, 0);
+    }
+    __syncthreads();
+    const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
+    const bool live = c < m.C && kk < K;
+    T phiC = 0, ov[RL];
+    if (live) {
+        const TI *phi = a.phi + kk, *phiB = a.phiB + kk;
+        phiC = (T)phi[(size_t)c * K];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            const int2 en = sen[cl * RL + i];
+            ov[i] = en.y >= 0 ? (T)phi[(size_t)en.y * K] : (T)phiB[(size_t)(~en.y) * K];
+        }
+    }
+    for (int p = tid; p < NP; p += kBlock)
+        if (cb + p / RL < m.C) ss[p] = m.fgeoS[sen[p].x >> 1];
+    for (int p = tid; p < NC; p += kBlock)
+        if (cb + p < m.C) svol[p] = m.cvol[cb + p];
+    __syncthreads();
+    if (!live) return;
+    const GradCell<T> own{phiC, {}, {}};
+    T acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        const int2 en = sen[cl * RL + i];
+        const V4<T> s = ss[cl * RL + i];
+        T t[3];
+        if (en.y >= 0) {
+            const GradCell<T> o{ov[i], {}, {}};
+            const bool in = (en.x & 1) != 0;
+            gradient_interior<T, true>(s, s, in ? o : own, in ? own : o, t);
+        } else {
+            t[0] = ov[i] * s.x;
+            t[1] = ov[i] * s.y;
+            t[2] = ov[i] * s.z;
+        }
+        acc[0] = acc[0] + t[0];
+        acc[1] = acc[1] + t[1];
+        acc[2] = acc[2] + t[2];
+    }
+    const T fact = 1.0 / svol[cl];
+    for (int j = 0; j < 3; j++) {
+        const T r = acc[j] * fact;
+        const size_t o = ((size_t)c * 3 + j) * K + kk;
+        st_stream(a.out1 + o, (TO)r);
+        if (a.out2) st_stream(a.out2 + o, (TO)r);
+    }
+}
+
 // A set of meshes for one persistent launch: their device descriptors and,
 // per lane-group width, the prefix of work units (blocks of kBlock / G cells)
 template <class T> struct MeshSetArgs {
@@ -2097,6 +2166,86 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_plain(DevMesh<T> m, S
         const V4<T> s = m.fgeoS[f];
         const ScalarFaceCoef<T> q = coef[f];
         const bool in = (en[i].x & 1) != 0;
+        const T phiP = in ? ophi[i] : phiC, phiN = in ? phiC : ophi[i];
+        const v3<T> gP = in ? og[i] : gC, gN = in ? gC : og[i];
+        const v3<T> S{s.x, s.y, s.z};
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const v3<T> g = lerp3(gN, facn, gP, facp);
+        const T fde1 = q.visac * dot3(g, S);
+        const T fci = q.fmn * (double)phiN + q.fmx * (double)phiP;
+        const T fdi = q.visface * dot3(g, v3<T>{q.xpx, q.xpy, q.xpz});
+        const T blend = a.gammaBlend * (q.fce - fci);
+        T t0 = -blend, t1 = fde1, t2 = -fdi;
+        if (in) {
+            t0 = -t0;
+            t1 = -t1;
+            t2 = -t2;
+        }
+        su = ((su + t0) + t1) + t2;
+    }
+    sup[(size_t)c * K] = (TS)su;
+}
+
+// k_scalar_batched_plain for 16-lane groups (a wave spans 4 cells, so its
+// lanes cannot share scalar loads): the block's rows, and the face vectors and
+// coefficients of their interior entries, are loaded once into LDS, one lane
+// per entry, instead of by all G lanes of each cell. The rows come before the
+// first barrier; the face data while every lane's phi and gradient gathers are
+// in flight, before the second. Terms and their order are
+// k_scalar_batched_plain's.
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_scalar_batched_staged(DevMesh<T> m, ScalarArgs<T, TS> a,
+                                                                  const ScalarFaceCoef<T> *coef, int K, int per,
+                                                                  int nblk) {
+    constexpr int NC = kBlock / G, NP = NC * RL;
+    __shared__ int2 sen[NP];
+    __shared__ V4<T> ss[NP];
+    __shared__ ScalarFaceCoef<T> sq[NP];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cb = lb * NC, tid = threadIdx.x;
+    for (int p = tid; p < NP; p += kBlock) {
+        const int c = cb + p / RL;
+        sen[p] = c < m.C ? m.ent[(size_t)c * RL + p % RL] : make_int2(0, -1);
+    }
+    __syncthreads();
+    const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
+    const bool live = c < m.C && kk < K;
+    const TS *phi = a.phi + kk, *dphi = a.dPhidx + kk;
+    TS *sup = a.su + kk;
+    T phiC = 0, su = 0, ophi[RL];
+    v3<T> gC{}, og[RL];
+    if (live) {
+        phiC = (T)phi[(size_t)c * K];
+        gC = ld3s_as<T>(dphi, c, K);
+        su = (T)sup[(size_t)c * K];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            const int2 en = sen[cl * RL + i];
+            if (en.y >= 0) {
+                ophi[i] = (T)phi[(size_t)en.y * K];
+                og[i] = ld3s_as<T>(dphi, en.y, K);
+            }
+        }
+    }
+    for (int p = tid; p < NP; p += kBlock) {
+        const int2 en = sen[p];
+        if (en.y >= 0) {
+            const int f = en.x >> 1;
+            ss[p] = m.fgeoS[f];
+            sq[p] = coef[f];
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        const int2 en = sen[cl * RL + i];
+        if (en.y < 0) continue;
+        const V4<T> s = ss[cl * RL + i];
+        const ScalarFaceCoef<T> q = sq[cl * RL + i];
+        const bool in = (en.x & 1) != 0;
         const T phiP = in ? ophi[i] : phiC, phiN = in ? phiC : ophi[i];
         const v3<T> gP = in ? og[i] : gC, gN = in ? gC : og[i];
         const v3<T> S{s.x, s.y, s.z};
@@ -2887,9 +3036,17 @@ void launch_gradient_batched(const DevMesh<T> &d, const BatchGradArgs<T, TI, TO>
         hipLaunchKernelGGL((k_gradient_batched<T, TI, TO, false, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per, \
                            gr.nblk);
     // the preloaded rows pay off when the row's connectivity is wave-uniform
-    // (G = 64); narrower groups walk the row entry by entry
+    // (G = 64); 16-lane groups share each row through LDS on a first sweep
+    // over uniform rows, and otherwise walk the row entry by entry
     if (G == 4) {
         CUPCFD_GB(4, 0)
+    } else if (G == 16 && s0 && (rl == 4 || rl == 6)) {
+        if (rl == 4)
+            hipLaunchKernelGGL((k_gradient_batched_staged<T, TI, TO, 16, 4>), grid, kBlock, 0, s, d, ga, K, gr.per,
+                               gr.nblk);
+        else
+            hipLaunchKernelGGL((k_gradient_batched_staged<T, TI, TO, 16, 6>), grid, kBlock, 0, s, d, ga, K, gr.per,
+                               gr.nblk);
     } else if (G == 16) {
         CUPCFD_GB(16, 0)
     } else if (rl == 6) {
@@ -2978,6 +3135,12 @@ int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, TS *PhiB, const
         else hipLaunchKernelGGL((k_scalar_batched<T, TS, GG>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk); \
     } while (0)
     if (G == 4) SCALAR_BATCHED_LAUNCH(4);
+    else if (G == 16 && RL == 6)
+        hipLaunchKernelGGL((k_scalar_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, coef, K, gr.per,
+                           gr.nblk);
+    else if (G == 16 && RL == 4)
+        hipLaunchKernelGGL((k_scalar_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per,
+                           gr.nblk);
     else if (G == 16) SCALAR_BATCHED_LAUNCH(16);
     else SCALAR_BATCHED_LAUNCH(64);
 #undef SCALAR_BATCHED_LAUNCH
