@@ -2636,6 +2636,125 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
     st_stream(a.phibar + (size_t)c * K + kk, (TS)acc);
 }
 
+// Passes A and B for 16-lane groups on meshes whose rows all hold RL entries
+// (a wave spans 4 cells, so its lanes cannot share scalar loads): the block's
+// rows and the per-face values their interior entries use are loaded once
+// into LDS, one lane per entry, instead of by all G lanes of each cell. The
+// rows come before the first barrier; the face values while every lane's
+// per-scalar gathers are in flight, before the second. Terms and their order
+// are k_adj_a_batched's and k_adj_b_batched's.
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
+                                                                 int nblk) {
+    constexpr int NC = kBlock / G, NP = NC * RL;
+    __shared__ int2 sen[NP];
+    __shared__ AdjFaceCoef<T> sq[NP];
+    __shared__ T sfn[NP];
+    __shared__ T svol[NC];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cb = lb * NC, tid = threadIdx.x;
+    for (int p = tid; p < NP; p += kBlock) {
+        const int c = cb + p / RL;
+        sen[p] = c < m.C ? m.ent[(size_t)c * RL + p % RL] : make_int2(0, -1);
+    }
+    __syncthreads();
+    const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
+    const bool live = c < m.C && kk < K;
+    T sc = 0, so[RL];
+    if (live) {
+        sc = (T)a.subar[(size_t)c * K + kk];
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            const int2 en = sen[cl * RL + i];
+            if (en.y >= 0) so[i] = (T)a.subar[(size_t)en.y * K + kk];
+        }
+    }
+    for (int p = tid; p < NP; p += kBlock) {
+        const int2 en = sen[p];
+        if (en.y >= 0) {
+            const int f = en.x >> 1;
+            sq[p] = a.coef[f];
+            sfn[p] = m.fgeoS[f].w;
+        }
+    }
+    for (int p = tid; p < NC; p += kBlock)
+        if (cb + p < m.C) svol[p] = m.cvol[cb + p];
+    __syncthreads();
+    if (!live) return;
+    T acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        const int2 en = sen[cl * RL + i];
+        if (en.y < 0) continue;
+        const bool in = (en.x & 1) != 0;
+        const AdjFaceCoef<T> q = sq[cl * RL + i];
+        const T facn = sfn[cl * RL + i];
+        const T facp = 1.0 - facn;
+        const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
+        const T Dg = D * a.s.gammaBlend;
+        const T fac = in ? facn : facp;
+        const T tt = D * fac;
+        acc[0] = acc[0] + tt * q.wx;
+        acc[1] = acc[1] + tt * q.wy;
+        acc[2] = acc[2] + tt * q.wz;
+        acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+    }
+    const T fact = 1.0 / svol[cl];
+    for (int j = 0; j < 3; j++) st_stream(a.h + ((size_t)c * 3 + j) * K + kk, acc[j] * fact);
+    st_stream(a.dir + (size_t)c * K + kk, acc[3]);
+}
+
+template <class T, class TS, int G, int RL>
+__global__ __launch_bounds__(kBlock) void k_adj_b_batched_staged(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
+                                                                 int nblk) {
+    constexpr int NC = kBlock / G, NP = NC * RL;
+    __shared__ int2 sen[NP];
+    __shared__ V4<T> ss[NP];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int cb = lb * NC, tid = threadIdx.x;
+    for (int p = tid; p < NP; p += kBlock) {
+        const int c = cb + p / RL;
+        sen[p] = c < m.C ? m.ent[(size_t)c * RL + p % RL] : make_int2(0, -1);
+    }
+    __syncthreads();
+    const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
+    const bool live = c < m.C && kk < K;
+    T acc = 0;
+    v3<T> hc{}, ho[RL];
+    if (live) {
+        acc = a.dir[(size_t)c * K + kk];
+        hc = ld3s(a.h + kk, c, K);
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            const int2 en = sen[cl * RL + i];
+            if (en.y >= 0) ho[i] = ld3s(a.h + kk, en.y, K);
+        }
+    }
+    for (int p = tid; p < NP; p += kBlock) {
+        const int2 en = sen[p];
+        if (en.y >= 0) ss[p] = m.fgeoS[en.x >> 1];
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < RL; i++) {
+        const int2 en = sen[cl * RL + i];
+        if (en.y < 0) continue;
+        const bool in = (en.x & 1) != 0;
+        const v3<T> hp = in ? ho[i] : hc, hn = in ? hc : ho[i];
+        const V4<T> s = ss[cl * RL + i];
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const T fac = in ? facn : facp;
+        const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
+        const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
+        acc = acc + fac * pF;
+    }
+    st_stream(a.phibar + (size_t)c * K + kk, (TS)acc);
+}
+
 // The mass-flux cotangent of the batched adjoint, G lanes per face (G = 4,
 // 16 or 64, the smallest that holds K; K > 64 takes chunks of 64): lane j of
 // a face's group evaluates scalar base + j's term, reading the per-scalar rows
@@ -3283,7 +3402,13 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
         else ADJ_BATCHED_LAUNCH(GG, 0);                                                                           \
     } while (0)
         if (G == 4) ADJ_BATCHED_ROWS(4);
-        else if (G == 16) ADJ_BATCHED_ROWS(16);
+        else if (G == 16 && RL == 6) {
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 16 && RL == 4) {
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 16) ADJ_BATCHED_ROWS(16);
         else ADJ_BATCHED_ROWS(64);
 #undef ADJ_BATCHED_ROWS
 #undef ADJ_BATCHED_LAUNCH
