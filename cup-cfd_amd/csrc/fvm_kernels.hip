@@ -2636,6 +2636,93 @@ __global__ __launch_bounds__(kBlock) void k_adj_b_batched(DevMesh<T> m, AdjBatch
     st_stream(a.phibar + (size_t)c * K + kk, (TS)acc);
 }
 
+// Pass A for 64-lane groups on uniform rows when all K <= 64 scalars of a cell
+// sit in one group, with the mass-flux cotangent folded in (no separate
+// faces pass): each face's cell1 entry evaluates the face's K terms
+// -(D*GammaBlend*(PhiFace - dfci)) from the SuBar difference pass A already
+// forms and the upwind cell's phi, and puts them in LDS; after one barrier a
+// lane per (cell, entry) adds the RFace term and then the K terms in scalar
+// order, as k_adj_faces_batched does. A face whose cell1 entry is a boundary
+// entry gets 0.
+template <class T, class TS, int RL>
+__global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
+                                                             int nblk) {
+    constexpr int G = 64, NC = kBlock / G, NS = NC * RL, LS = G + 1;  // LS: row stride, odd against bank conflicts
+    __shared__ T lt[NS * LS];
+    const int lb = xcd_block(blockIdx.x, per);
+    if (lb >= nblk) return;
+    const int tid = threadIdx.x, cl = tid / G, kk = tid % G;
+    const int c = __builtin_amdgcn_readfirstlane(lb * NC + cl);
+    if (c < m.C) {
+        const bool live = kk < K;
+        const T PhiFace = m.carry_init;
+        T sc = 0, pc = 0, so[RL], po[RL];
+        int2 en[RL];
+#pragma unroll
+        for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
+        if (live) {
+            sc = (T)a.subar[(size_t)c * K + kk];
+            pc = (T)a.s.phi[(size_t)c * K + kk];
+#pragma unroll
+            for (int i = 0; i < RL; i++) {
+                if (en[i].y < 0) continue;
+                so[i] = (T)a.subar[(size_t)en[i].y * K + kk];
+                if (!(en[i].x & 1)) po[i] = (T)a.s.phi[(size_t)en[i].y * K + kk];
+            }
+        }
+        T acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < RL; i++) {
+            T t = 0.0;
+            if (en[i].y >= 0 && live) {
+                const int f = en[i].x >> 1;
+                const bool in = (en[i].x & 1) != 0;
+                const AdjFaceCoef<T> q = a.coef[f];
+                const T facn = m.fgeoS[f].w;
+                const T facp = 1.0 - facn;
+                const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
+                const T Dg = D * a.s.gammaBlend;
+                const T fac = in ? facn : facp;
+                const T tt = D * fac;
+                acc[0] = acc[0] + tt * q.wx;
+                acc[1] = acc[1] + tt * q.wy;
+                acc[2] = acc[2] + tt * q.wz;
+                acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+                if (!in) {
+                    const T mf = a.s.massFlux[f];
+                    const bool up = mf < 0 || mf > 0;
+                    const T dfci = up ? (mf < 0 ? po[i] : pc) : (T)0.0;
+                    t = -(Dg * (PhiFace - dfci));
+                }
+            }
+            lt[(cl * RL + i) * LS + kk] = t;
+        }
+        if (live) {
+            const T fact = 1.0 / m.cvol[c];
+            for (int j = 0; j < 3; j++) st_stream(a.h + ((size_t)c * 3 + j) * K + kk, acc[j] * fact);
+            st_stream(a.dir + (size_t)c * K + kk, acc[3]);
+        }
+    }
+    __syncthreads();
+    if (tid < NS) {
+        const int c2 = lb * NC + tid / RL;
+        if (c2 < m.C) {
+            const int2 e = m.ent[c2 * RL + tid % RL];
+            if (!(e.x & 1)) {
+                const int f = e.x >> 1;
+                T s = 0.0;
+                if (e.y >= 0) {
+                    const T mf = a.s.massFlux[f];
+                    s = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
+                    const T *l = lt + tid * LS;
+                    for (int k = 0; k < K; k++) s = s + l[k];
+                }
+                a.mfbar[f] = s;
+            }
+        }
+    }
+}
+
 // Passes A and B for 16-lane groups on meshes whose rows all hold RL entries
 // (a wave spans 4 cells, so its lanes cannot share scalar loads): the block's
 // rows and the per-face values their interior entries use are loaded once
@@ -3401,7 +3488,13 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
         else if (RL == 4) ADJ_BATCHED_LAUNCH(GG, 4);                                                              \
         else ADJ_BATCHED_LAUNCH(GG, 0);                                                                           \
     } while (0)
-        if (G == 4) ADJ_BATCHED_ROWS(4);
+        if (G == 64 && K <= 64 && RL == 6) {
+            hipLaunchKernelGGL((k_adj_a_batched_mf<T, TS, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched<T, TS, 64, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 64 && K <= 64 && RL == 4) {
+            hipLaunchKernelGGL((k_adj_a_batched_mf<T, TS, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched<T, TS, 64, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 4) ADJ_BATCHED_ROWS(4);
         else if (G == 16 && RL == 6) {
             hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
@@ -3413,7 +3506,9 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
 #undef ADJ_BATCHED_ROWS
 #undef ADJ_BATCHED_LAUNCH
     }
-    if (d.F) {
+    // the mass-flux cotangent, unless pass A has written it (64-lane groups, one chunk, uniform rows)
+    const bool mf_in_a = d.C && batch_group(K) == 64 && K <= 64 && (d.rowlen == 6 || d.rowlen == 4);
+    if (d.F && !mf_in_a) {
         const int G = batch_group(K);
         const dim3 grid((unsigned)((d.F + kBlock / G - 1) / (kBlock / G)));
         if (G == 4) hipLaunchKernelGGL((k_adj_faces_batched<T, TS, 4>), grid, dim3(kBlock), 0, st, d, a, K);

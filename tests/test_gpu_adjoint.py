@@ -193,8 +193,8 @@ def test_march_checkpointed_reverse():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("K", [1, 3, 16, 70])
-@pytest.mark.parametrize("name", ["perm12", "tets"])
+@pytest.mark.parametrize("K", [1, 3, 16, 40, 64, 70])
+@pytest.mark.parametrize("name", ["ref5", "perm12", "tets"])
 def test_batched_adjoint_matches_single(name, K):
     """PhiBar_k equals the single-scalar adjoint's bit for bit; MassFluxBar is the RFace term followed by the
     K scalars' terms added in order (each term taken from a single-scalar call with a zero RFace cotangent)."""
@@ -222,7 +222,7 @@ def test_batched_adjoint_matches_single(name, K):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("K", [3, 16, 70])
+@pytest.mark.parametrize("K", [3, 16, 64, 70])
 @pytest.mark.parametrize("name", ["perm12", "tets"])
 def test_batched_adjoint_f32_storage_is_rounded_f64(name, K):
     """FP32 storage of phi, SuBar and PhiBar: PhiBar is the FP64 batched adjoint on the widened inputs, rounded
