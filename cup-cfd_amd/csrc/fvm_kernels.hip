@@ -2730,14 +2730,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBa
 // rows come before the first barrier; the face values while every lane's
 // per-scalar gathers are in flight, before the second. Terms and their order
 // are k_adj_a_batched's and k_adj_b_batched's.
-template <class T, class TS, int G, int RL>
+// MF: K <= G, so each face's K mass-flux terms are formed here and added by a
+// lane per (cell, entry) after a third barrier, as in k_adj_a_batched_mf.
+template <class T, class TS, int G, int RL, bool MF>
 __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, AdjBatchArgs<T, TS> a, int K, int per,
                                                                  int nblk) {
-    constexpr int NC = kBlock / G, NP = NC * RL;
+    constexpr int NC = kBlock / G, NP = NC * RL, LS = G + 1;
     __shared__ int2 sen[NP];
     __shared__ AdjFaceCoef<T> sq[NP];
     __shared__ T sfn[NP];
     __shared__ T svol[NC];
+    __shared__ T lt[MF ? NP * LS : 1];
     const int lb = xcd_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int cb = lb * NC, tid = threadIdx.x;
@@ -2748,13 +2751,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
     __syncthreads();
     const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
     const bool live = c < m.C && kk < K;
-    T sc = 0, so[RL];
+    T sc = 0, pc = 0, so[RL], po[RL];
     if (live) {
         sc = (T)a.subar[(size_t)c * K + kk];
+        if (MF) pc = (T)a.s.phi[(size_t)c * K + kk];
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             const int2 en = sen[cl * RL + i];
-            if (en.y >= 0) so[i] = (T)a.subar[(size_t)en.y * K + kk];
+            if (en.y >= 0) {
+                so[i] = (T)a.subar[(size_t)en.y * K + kk];
+                if (MF && !(en.x & 1)) po[i] = (T)a.s.phi[(size_t)en.y * K + kk];
+            }
         }
     }
     for (int p = tid; p < NP; p += kBlock) {
@@ -2768,28 +2775,51 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
     for (int p = tid; p < NC; p += kBlock)
         if (cb + p < m.C) svol[p] = m.cvol[cb + p];
     __syncthreads();
-    if (!live) return;
-    T acc[4] = {0, 0, 0, 0};
+    if (live) {
+        const T PhiFace = m.carry_init;
+        T acc[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < RL; i++) {
-        const int2 en = sen[cl * RL + i];
-        if (en.y < 0) continue;
-        const bool in = (en.x & 1) != 0;
-        const AdjFaceCoef<T> q = sq[cl * RL + i];
-        const T facn = sfn[cl * RL + i];
-        const T facp = 1.0 - facn;
-        const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
-        const T Dg = D * a.s.gammaBlend;
-        const T fac = in ? facn : facp;
-        const T tt = D * fac;
-        acc[0] = acc[0] + tt * q.wx;
-        acc[1] = acc[1] + tt * q.wy;
-        acc[2] = acc[2] + tt * q.wz;
-        acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+        for (int i = 0; i < RL; i++) {
+            const int2 en = sen[cl * RL + i];
+            if (en.y < 0) continue;
+            const bool in = (en.x & 1) != 0;
+            const AdjFaceCoef<T> q = sq[cl * RL + i];
+            const T facn = sfn[cl * RL + i];
+            const T facp = 1.0 - facn;
+            const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
+            const T Dg = D * a.s.gammaBlend;
+            const T fac = in ? facn : facp;
+            const T tt = D * fac;
+            acc[0] = acc[0] + tt * q.wx;
+            acc[1] = acc[1] + tt * q.wy;
+            acc[2] = acc[2] + tt * q.wz;
+            acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
+            if (MF && !in) {
+                const T mf = a.s.massFlux[en.x >> 1];
+                const bool up = mf < 0 || mf > 0;
+                const T dfci = up ? (mf < 0 ? po[i] : pc) : (T)0.0;
+                lt[(cl * RL + i) * LS + tid % G] = -(Dg * (PhiFace - dfci));
+            }
+        }
+        const T fact = 1.0 / svol[cl];
+        for (int j = 0; j < 3; j++) st_stream(a.h + ((size_t)c * 3 + j) * K + kk, acc[j] * fact);
+        st_stream(a.dir + (size_t)c * K + kk, acc[3]);
     }
-    const T fact = 1.0 / svol[cl];
-    for (int j = 0; j < 3; j++) st_stream(a.h + ((size_t)c * 3 + j) * K + kk, acc[j] * fact);
-    st_stream(a.dir + (size_t)c * K + kk, acc[3]);
+    if (!MF) return;
+    __syncthreads();
+    for (int p = tid; p < NP; p += kBlock) {
+        const int2 e = sen[p];
+        if (cb + p / RL >= m.C || (e.x & 1)) continue;
+        const int f = e.x >> 1;
+        T s = 0.0;
+        if (e.y >= 0) {
+            const T mf = a.s.massFlux[f];
+            s = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
+            const T *l = lt + p * LS;
+            for (int k = 0; k < K; k++) s = s + l[k];
+        }
+        a.mfbar[f] = s;
+    }
 }
 
 template <class T, class TS, int G, int RL>
@@ -3496,18 +3526,28 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
             hipLaunchKernelGGL((k_adj_b_batched<T, TS, 64, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 4) ADJ_BATCHED_ROWS(4);
         else if (G == 16 && RL == 6) {
-            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            if (K <= 16)
+                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                                   gr.nblk);
+            else
+                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6, false>), grid, kBlock, 0, st, d, a, K, gr.per,
+                                   gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 16 && RL == 4) {
-            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+            if (K <= 16)
+                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                                   gr.nblk);
+            else
+                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4, false>), grid, kBlock, 0, st, d, a, K, gr.per,
+                                   gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 16) ADJ_BATCHED_ROWS(16);
         else ADJ_BATCHED_ROWS(64);
 #undef ADJ_BATCHED_ROWS
 #undef ADJ_BATCHED_LAUNCH
     }
-    // the mass-flux cotangent, unless pass A has written it (64-lane groups, one chunk, uniform rows)
-    const bool mf_in_a = d.C && batch_group(K) == 64 && K <= 64 && (d.rowlen == 6 || d.rowlen == 4);
+    // the mass-flux cotangent, unless pass A has written it (16- or 64-lane groups, one chunk, uniform rows)
+    const bool mf_in_a = d.C && (batch_group(K) == 64 || batch_group(K) == 16) && K <= 64 && (d.rowlen == 6 || d.rowlen == 4);
     if (d.F && !mf_in_a) {
         const int G = batch_group(K);
         const dim3 grid((unsigned)((d.F + kBlock / G - 1) / (kBlock / G)));
