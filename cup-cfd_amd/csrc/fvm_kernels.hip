@@ -1975,8 +1975,8 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched(DevMesh<T> m, Batch
     gradient_batched_cell<T, TI, TO, SWEEP0, RL>(m, a, K, batch_uniform<G>(L.c), L.k);
 }
 
-// Batched gradient, first sweep, for lane groups narrower than a wave (G =
-// 16: a wave spans 4 cells, so its lanes cannot share scalar loads) on meshes
+// Batched gradient, first sweep, for lane groups narrower than a wave (G = 4
+// or 16: a wave spans 16 or 4 cells, so its lanes cannot share scalar loads) on meshes
 // whose rows all hold RL entries. The block's rows, face vectors and cell
 // volumes are loaded once into LDS, one lane per entry, instead of by all G
 // lanes of each cell: the rows before the first barrier, the face vectors and
@@ -2187,8 +2187,8 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_plain(DevMesh<T> m, S
     sup[(size_t)c * K] = (TS)su;
 }
 
-// k_scalar_batched_plain for 16-lane groups (a wave spans 4 cells, so its
-// lanes cannot share scalar loads): the block's rows, and the face vectors and
+// k_scalar_batched_plain for 4- and 16-lane groups (a wave spans 16 or 4
+// cells, so its lanes cannot share scalar loads): the block's rows, and the face vectors and
 // coefficients of their interior entries, are loaded once into LDS, one lane
 // per entry, instead of by all G lanes of each cell. The rows come before the
 // first barrier; the face data while every lane's phi and gradient gathers are
@@ -2723,8 +2723,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBa
     }
 }
 
-// Passes A and B for 16-lane groups on meshes whose rows all hold RL entries
-// (a wave spans 4 cells, so its lanes cannot share scalar loads): the block's
+// Passes A and B for 4- and 16-lane groups on meshes whose rows all hold RL
+// entries (a wave spans 16 or 4 cells, so its lanes cannot share scalar loads): the block's
 // rows and the per-face values their interior entries use are loaded once
 // into LDS, one lane per entry, instead of by all G lanes of each cell. The
 // rows come before the first barrier; the face values while every lane's
@@ -3272,9 +3272,16 @@ void launch_gradient_batched(const DevMesh<T> &d, const BatchGradArgs<T, TI, TO>
         hipLaunchKernelGGL((k_gradient_batched<T, TI, TO, false, GG, RR>), grid, kBlock, 0, s, d, ga, K, gr.per, \
                            gr.nblk);
     // the preloaded rows pay off when the row's connectivity is wave-uniform
-    // (G = 64); 16-lane groups share each row through LDS on a first sweep
-    // over uniform rows, and otherwise walk the row entry by entry
-    if (G == 4) {
+    // (G = 64); 4- and 16-lane groups share each row through LDS on a first
+    // sweep over uniform rows, and otherwise walk the row entry by entry
+    if (G == 4 && s0 && (rl == 4 || rl == 6)) {
+        if (rl == 4)
+            hipLaunchKernelGGL((k_gradient_batched_staged<T, TI, TO, 4, 4>), grid, kBlock, 0, s, d, ga, K, gr.per,
+                               gr.nblk);
+        else
+            hipLaunchKernelGGL((k_gradient_batched_staged<T, TI, TO, 4, 6>), grid, kBlock, 0, s, d, ga, K, gr.per,
+                               gr.nblk);
+    } else if (G == 4) {
         CUPCFD_GB(4, 0)
     } else if (G == 16 && s0 && (rl == 4 || rl == 6)) {
         if (rl == 4)
@@ -3370,7 +3377,13 @@ int dev_scalar_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, TS *PhiB, const
                                gr.nblk);                                                                             \
         else hipLaunchKernelGGL((k_scalar_batched<T, TS, GG>), grid, kBlock, 0, st, d, a, coef, K, gr.per, gr.nblk); \
     } while (0)
-    if (G == 4) SCALAR_BATCHED_LAUNCH(4);
+    if (G == 4 && RL == 6)
+        hipLaunchKernelGGL((k_scalar_batched_staged<T, TS, 4, 6>), grid, kBlock, 0, st, d, a, coef, K, gr.per,
+                           gr.nblk);
+    else if (G == 4 && RL == 4)
+        hipLaunchKernelGGL((k_scalar_batched_staged<T, TS, 4, 4>), grid, kBlock, 0, st, d, a, coef, K, gr.per,
+                           gr.nblk);
+    else if (G == 4) SCALAR_BATCHED_LAUNCH(4);
     else if (G == 16 && RL == 6)
         hipLaunchKernelGGL((k_scalar_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, coef, K, gr.per,
                            gr.nblk);
@@ -3524,6 +3537,14 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
         } else if (G == 64 && K <= 64 && RL == 4) {
             hipLaunchKernelGGL((k_adj_a_batched_mf<T, TS, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched<T, TS, 64, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 4 && RL == 6) {
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 4, 6, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                               gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 4, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
+        } else if (G == 4 && RL == 4) {
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 4, 4, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                               gr.nblk);
+            hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 4, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 4) ADJ_BATCHED_ROWS(4);
         else if (G == 16 && RL == 6) {
             if (K <= 16)
@@ -3546,8 +3567,8 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
 #undef ADJ_BATCHED_ROWS
 #undef ADJ_BATCHED_LAUNCH
     }
-    // the mass-flux cotangent, unless pass A has written it (16- or 64-lane groups, one chunk, uniform rows)
-    const bool mf_in_a = d.C && (batch_group(K) == 64 || batch_group(K) == 16) && K <= 64 && (d.rowlen == 6 || d.rowlen == 4);
+    // the mass-flux cotangent, unless pass A has written it (one chunk of scalars, uniform rows)
+    const bool mf_in_a = d.C && K <= 64 && (d.rowlen == 6 || d.rowlen == 4);
     if (d.F && !mf_in_a) {
         const int G = batch_group(K);
         const dim3 grid((unsigned)((d.F + kBlock / G - 1) / (kBlock / G)));
