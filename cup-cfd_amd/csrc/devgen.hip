@@ -23,6 +23,7 @@
 
 #include "devmesh.hpp"
 #include "host_mesh.hpp"
+#include "sg_geom.hpp"
 
 namespace cupcfd_hip {
 
@@ -61,18 +62,19 @@ struct Grid {
     int64_t nYZ, nXZ, nBYZ, nBXZ;
 };
 
-// cell centre as getCellCenter computes it, volume in closed form
-// (MeshStructGenSource.cpp:330-340)
-__global__ void k_sg_cells(Grid g, int C, double *cc, double *vol) {
+// cell centre as getCellCenter computes it (MeshStructGenSource.cpp:330-340),
+// volume from the reference's five-tetrahedron split around that centre
+__global__ void k_sg_cells(Grid g, sg::HexPlan plan, int C, double *cc, double *vol) {
+    const double half[3] = {g.dd[0] / 2, g.dd[1] / 2, g.dd[2] / 2};
     GRID_STRIDE(c, C) {
         const int ijk[3] = {(int)(c % g.nx), (int)((c / g.nx) % g.ny), (int)(c / ((int64_t)g.nx * g.ny))};
-        double ext[3];
+        sg::P3 ctr;
         for (int k = 0; k < 3; k++) {
             const double l = g.mn[k] + (ijk[k] * g.dd[k]), h = g.mn[k] + (ijk[k] + 1) * g.dd[k];
-            cc[3 * c + k] = l + ((h - l) / 2.0);
-            ext[k] = h - l;
+            ctr.c[k] = l + ((h - l) / 2.0);
+            cc[3 * c + k] = ctr.c[k];
         }
-        vol[c] = ext[0] * ext[1] * ext[2];
+        vol[c] = sg::hex_volume(plan, ctr, half);
     }
 }
 
@@ -141,24 +143,32 @@ __global__ void k_sg_faces(Grid g, int F, int *c1, int *c2, int *fb, double *lam
             bregion[bl] = g.regions ? reg : 0;
             bdist[bl] = g.regions ? g.dd[axis] / 2.0 : 0.0;
         }
-        const int cc3[3] = {a, b, c};
-        double ext[3], mid[3];
-        for (int k = 0; k < 3; k++) {
-            if (k == axis) {
-                mid[k] = g.mn[k] + n * g.dd[k];
-                ext[k] = 1.0;
-            } else {
-                const double l = g.mn[k] + (cc3[k] * g.dd[k]), h = g.mn[k] + (cc3[k] + 1) * g.dd[k];
-                mid[k] = l + ((h - l) / 2.0);
-                ext[k] = h - l;
-            }
+        // geometry from the face's four vertices in getFaceVerticesLabelsCSR's
+        // order (MeshStructGenSource.cpp:1010-1085)
+        int node[3] = {a, b, c};
+        node[axis] = n;
+        const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
+        sg::P3 v[4];
+        for (int i = 0; i < 4; i++) {
+            int q[3] = {node[0], node[1], node[2]};
+            q[u] += (i == 1 || i == 2) ? 1 : 0;
+            q[w] += (i >= 2) ? 1 : 0;
+            for (int k = 0; k < 3; k++) v[i].c[k] = g.mn[k] + (g.dd[k] * q[k]);
         }
-        const double ar = ext[0] * ext[1] * ext[2];
-        area[f] = ar;
-        const double sgn = (n == 0) ? -1.0 : 1.0;
+        double ar;
+        sg::P3 mid;
+        sg::quad_area_centroid(v, &ar, &mid);
+        // cell1's centre, as k_sg_cells computes it
+        sg::P3 c1c;
         for (int k = 0; k < 3; k++) {
-            ctr[3 * (int64_t)f + k] = mid[k];
-            nrm[3 * (int64_t)f + k] = (k == axis) ? sgn * ar : 0.0;
+            const double l = g.mn[k] + (lo[k] * g.dd[k]), h = g.mn[k] + (lo[k] + 1) * g.dd[k];
+            c1c.c[k] = l + ((h - l) / 2.0);
+        }
+        const sg::P3 S = sg::quad_normal(v, c1c);
+        area[f] = ar;
+        for (int k = 0; k < 3; k++) {
+            ctr[3 * (int64_t)f + k] = mid.c[k];
+            nrm[3 * (int64_t)f + k] = S.c[k];
         }
     }
 }
@@ -517,7 +527,8 @@ int gen_structgen(const cupcfd_structgen_params &p, Pool &P, GenMesh &G) {
     if (F64 >= (int64_t)1 << 30) return CUPCFD_E_CONFIG_INVALID_VALUE;
     P.bufs.min_chunk = std::max<size_t>(size_t(1) << 20, 256 * (size_t)F64);
     G.alloc(P, (int)C64, (int)F64, (int)B64);
-    P.launch(G.C, k_sg_cells, g, G.C, G.cc, G.vol);
+    const double half[3] = {g.dd[0] / 2, g.dd[1] / 2, g.dd[2] / 2};
+    P.launch(G.C, k_sg_cells, g, sg::hex_plan(half), G.C, G.cc, G.vol);
     P.launch(G.F, k_sg_faces, g, G.F, G.c1, G.c2, G.fb, G.lam, G.area, G.nrm, G.ctr, G.bface, G.bregion, G.bdist);
     if (p.face_extras)
         P.launch(G.F, k_face_extras, G.F, (uint64_t)(p.seed ^ 0xFACEE47A5ull),

@@ -10,6 +10,7 @@
 #include <new>
 
 #include "host_mesh.hpp"
+#include "sg_geom.hpp"
 
 namespace cupcfd_hip {
 
@@ -138,7 +139,10 @@ int build_structgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m) {
     resize_mesh(m, C, F, B);
     add_regions(m, p.regions != 0);
 
-    // cells: centre as getCellCenter computes it, volume in closed form
+    // cells: centre as getCellCenter computes it, volume from the reference's
+    // five-tetrahedron split of the hexahedron around that centre
+    const double half[3] = {g.dx / 2, g.dy / 2, g.dz / 2};
+    const sg::HexPlan plan = sg::hex_plan(half);
     for (int z = 0; z < nz; z++)
         for (int y = 0; y < ny; y++)
             for (int x = 0; x < nx; x++) {
@@ -146,10 +150,9 @@ int build_structgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m) {
                 double xl = g.lo(g.xmin, g.dx, x), xh = g.hi(g.xmin, g.dx, x);
                 double yl = g.lo(g.ymin, g.dy, y), yh = g.hi(g.ymin, g.dy, y);
                 double zl = g.lo(g.zmin, g.dz, z), zh = g.hi(g.zmin, g.dz, z);
-                m.cellCenter[3 * (size_t)c + 0] = xl + ((xh - xl) / 2.0);
-                m.cellCenter[3 * (size_t)c + 1] = yl + ((yh - yl) / 2.0);
-                m.cellCenter[3 * (size_t)c + 2] = zl + ((zh - zl) / 2.0);
-                m.cellVolume[c] = (xh - xl) * (yh - yl) * (zh - zl);
+                sg::P3 ctr{{xl + ((xh - xl) / 2.0), yl + ((yh - yl) / 2.0), zl + ((zh - zl) / 2.0)}};
+                for (int k = 0; k < 3; k++) m.cellCenter[3 * (size_t)c + k] = ctr.c[k];
+                m.cellVolume[c] = sg::hex_volume(plan, ctr, half);
             }
 
     // faces, in label order: YZ (x-normal), XZ (y-normal), XY (z-normal)
@@ -213,28 +216,31 @@ int build_structgen(const cupcfd_structgen_params &p, cupcfd_host_mesh &m) {
             double d = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
             m.bndDistance[bl] = p.regions ? d / 2.0 : 0.0;
         }
-        // geometry: the face spans its cell's extent on the two other axes and
-        // sits at the node coordinate on its own axis
-        double mn[3] = {g.xmin, g.ymin, g.zmin}, dd[3] = {g.dx, g.dy, g.dz};
-        int cc[3] = {a, b, c};
-        double ext[3], ctr[3];
-        for (int k = 0; k < 3; k++) {
-            if (k == axis) {
-                ctr[k] = mn[k] + n * dd[k];
-                ext[k] = 1.0;
-            } else {
-                double l = g.lo(mn[k], dd[k], cc[k]), h = g.hi(mn[k], dd[k], cc[k]);
-                ctr[k] = l + ((h - l) / 2.0);
-                ext[k] = h - l;
-            }
+        // geometry from the face's four vertices in getFaceVerticesLabelsCSR's
+        // order (MeshStructGenSource.cpp:1010-1085): node (a, b, c) with the
+        // face's own axis at its node coordinate n
+        const double mn[3] = {g.xmin, g.ymin, g.zmin}, dd[3] = {g.dx, g.dy, g.dz};
+        int node[3] = {a, b, c};
+        node[axis] = n;
+        const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;  // the two in-plane axes, low then high
+        const int step[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        sg::P3 v[4];
+        for (int i = 0; i < 4; i++) {
+            int q[3] = {node[0], node[1], node[2]};
+            q[u] += step[i][0];
+            q[w] += step[i][1];
+            for (int k = 0; k < 3; k++) v[i].c[k] = mn[k] + (dd[k] * q[k]);
         }
-        double area = ext[0] * ext[1] * ext[2];
+        double area;
+        sg::P3 ctr;
+        sg::quad_area_centroid(v, &area, &ctr);
+        const sg::P3 c1c{{m.cellCenter[3 * (size_t)m.faceCell1[f]], m.cellCenter[3 * (size_t)m.faceCell1[f] + 1],
+                          m.cellCenter[3 * (size_t)m.faceCell1[f] + 2]}};
+        const sg::P3 S = sg::quad_normal(v, c1c);
         m.faceArea[f] = area;
-        // outward from cell1: +axis except on the low boundary plane
-        double sgn = (n == 0) ? -1.0 : 1.0;
         for (int k = 0; k < 3; k++) {
-            m.faceCenter[3 * (size_t)f + k] = ctr[k];
-            m.faceNorm[3 * (size_t)f + k] = (k == axis) ? sgn * area : 0.0;
+            m.faceCenter[3 * (size_t)f + k] = ctr.c[k];
+            m.faceNorm[3 * (size_t)f + k] = S.c[k];
         }
     }
     if (p.face_extras) fill_face_extras(m, p.seed, std::min(g.dx, std::min(g.dy, g.dz)));
