@@ -316,6 +316,80 @@ class HostMesh:
             self._h = None
 
 
+ARRAY_KEYS = ("faceCell1", "faceCell2", "faceBnd", "faceLambda", "faceRLencos", "faceArea", "faceNorm",
+              "faceCenter", "faceXpac", "faceXnac", "cellCenter", "cellVolume", "bndFace", "bndRegion",
+              "bndDistance", "bndYPlus", "bndUPlus", "bndQ", "bndH", "bndT", "bndShear", "regType", "regStd",
+              "regFlux", "regAdiab", "regYLog", "regELog", "regDensity", "regTurbKE", "regTurbDiss", "regSplvl",
+              "regDen", "regR", "regT", "regForceTangent", "regUVW")
+_INT_KEYS = {"faceCell1", "faceCell2", "faceBnd", "bndFace", "bndRegion", "regType"}
+_U8_KEYS = {"regStd", "regFlux", "regAdiab"}
+
+
+class ArrayMesh:
+    """A MiniSoA mesh given as numpy arrays (the keys of ``HostMesh.a``; 3-vectors as (n, 3)), in the mesh's
+    own numbering: a mesh read from a file, or one a test edits. Offers the same ``a``, counts, views and state
+    accessors as HostMesh, so DeviceMesh, the oracle and the tests take either. The arrays are copied; the FP32
+    view holds the FP64 values rounded once."""
+
+    STATE = HostMesh.STATE
+
+    def __init__(self, arrays):
+        a = {}
+        for k in ARRAY_KEYS:
+            v = np.asarray(arrays[k])
+            dt = np.int32 if k in _INT_KEYS else (np.uint8 if k in _U8_KEYS else np.float64)
+            v = np.ascontiguousarray(v, dtype=dt).copy()
+            if k in ("faceNorm", "faceCenter", "faceXpac", "faceXnac", "cellCenter", "bndShear", "regForceTangent",
+                     "regUVW"):
+                v = v.reshape(-1, 3)
+            a[k] = v
+        self.a = a
+        self.nCells = a["cellVolume"].size
+        self.nFaces = a["faceCell1"].size
+        self.nBoundaries = a["bndFace"].size
+        self.nRegions = a["regType"].size
+        self.params = {}
+        self.view64 = self._view(MeshViewF64, V3F64, a)
+        self._a32 = None
+        self._view32 = None
+
+    def _view(self, View, V3, a):
+        v = View()
+        v.nCells, v.nFaces, v.nBoundaries, v.nRegions = self.nCells, self.nFaces, self.nBoundaries, self.nRegions
+        p = lambda k: a[k].ctypes.data if a[k].size else None  # noqa: E731
+        q = lambda k: V3(a[k].ctypes.data if a[k].size else None, 0)  # noqa: E731
+        v.faceCell1ID, v.faceCell2ID, v.faceBoundaryID = p("faceCell1"), p("faceCell2"), p("faceBnd")
+        v.faceLambda, v.faceRLencos, v.faceArea = p("faceLambda"), p("faceRLencos"), p("faceArea")
+        v.faceNorm, v.faceCenter, v.faceXpac, v.faceXnac = q("faceNorm"), q("faceCenter"), q("faceXpac"), q("faceXnac")
+        v.cellCenter, v.cellVolume = q("cellCenter"), p("cellVolume")
+        v.boundaryFaceID, v.boundaryRegionID, v.boundaryDistance = p("bndFace"), p("bndRegion"), p("bndDistance")
+        v.boundaryYPlus, v.boundaryUPlus, v.boundaryQ = p("bndYPlus"), p("bndUPlus"), p("bndQ")
+        v.boundaryH, v.boundaryT, v.boundaryShear = p("bndH"), p("bndT"), q("bndShear")
+        v.regionType, v.regionStd, v.regionFlux, v.regionAdiab = p("regType"), p("regStd"), p("regFlux"), p("regAdiab")
+        v.regionYLog, v.regionELog, v.regionDensity = p("regYLog"), p("regELog"), p("regDensity")
+        v.regionTurbKE, v.regionTurbDiss, v.regionSplvl = p("regTurbKE"), p("regTurbDiss"), p("regSplvl")
+        v.regionDen, v.regionR, v.regionT = p("regDen"), p("regR"), p("regT")
+        v.regionForceTangent, v.regionUVW = q("regForceTangent"), q("regUVW")
+        return v
+
+    def view32(self):
+        if self._view32 is None:
+            self._a32 = {k: (v.astype(np.float32) if v.dtype == np.float64 else v) for k, v in self.a.items()}
+            self._view32 = self._view(MeshViewF32, V3F32, self._a32)
+        return self._view32
+
+    def state(self, precision=64):
+        src = self.a if precision == 64 else (self.view32() and self._a32)
+        return {k: src[k].copy() for k in self.STATE}
+
+    def set_state(self, st, precision=64):
+        if precision != 64:
+            self.view32()
+        dst = self.a if precision == 64 else self._a32
+        for k, v in st.items():
+            dst[k][...] = v
+
+
 # ---------------------------------------------------------------------------
 # Device mesh and kernels
 # ---------------------------------------------------------------------------
