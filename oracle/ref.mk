@@ -12,7 +12,8 @@
 # functions (include/utility/interface/component/ArrayDrivers.ipp:342,351)
 # into warnings, as the reference CI's GCC flags do (CMakeLists.txt:78); it
 # does not change generated arithmetic. MPI is MPICH from /opt/conda (mpi.h,
-# libmpi.so.12), run as a singleton rank.
+# libmpi.so.12); the harness runs under mpiexec -n 2 (a one-rank mesh
+# finalize fails in the reference, see ref_harness.cpp).
 #
 #   make -f oracle/ref.mk            # build oracle/_ref/cupcfd_ref_harness
 REF ?= /root/reference

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only.
 //
 // Runs CUP-CFD's own fvm kernels (include/fvm/implementation/component/*.ipp),
-// compiled from the reference sources by oracle/Makefile's `ref` target, on a
+// compiled from the reference sources by oracle/ref.mk, on a
 // mesh and inputs read from a case file, and writes every output back. It is
 // the strongest check of the C restatement in oracle_kernels.inc: both run on
 // the same mesh arrays and inputs, and the restatement must give the same bits.
