@@ -3449,7 +3449,7 @@ template <class T>
 int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const T *MF, const T *SuBar,
                        const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
                        T Sigma_k, int VarED, T Sigma_e, T Sigma_s, T Schmidt, T GammaBlend, T *PhiBar, T *MFBar,
-                       void *stream) {
+                       void *stream, T *scratch_h = nullptr, T *scratch_dir = nullptr) {
     Mesh<T> *M = get<T>(h);
     if (!M) return CUPCFD_E_NULL_PTR;
     const DevMesh<T> &d = M->d;
@@ -3465,8 +3465,9 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
     a.mfbar = MFBar;
     hipStream_t st = (hipStream_t)stream;
     if (d.C) {
-        a.h = (T *)M->ws_get(47, 3 * (size_t)d.C * sizeof(T));
-        a.dir = (T *)M->ws_get(48, (size_t)d.C * sizeof(T));
+        // h and dir are written for every cell by pass A before pass B reads them
+        a.h = scratch_h ? scratch_h : (T *)M->ws_get(47, 3 * (size_t)d.C * sizeof(T));
+        a.dir = scratch_dir ? scratch_dir : (T *)M->ws_get(48, (size_t)d.C * sizeof(T));
         if (!a.h || !a.dir) return CUPCFD_E_ERROR;
         const int lpc = adj_lanes(*M);
         const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
@@ -3816,6 +3817,37 @@ int permute(cupcfd_hip_mesh *h, bool cells, const void *src, void *dst, int ncom
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the allocation-free adjoint entry point
+// (cupcfd_dev_scalar_adjoint_noalloc_f64): the pass-A fields h (3 per cell)
+// and dir (1 per cell), 32 bytes per cell, for meshes of up to 2,097,152
+// cells (a 128^3 grid). Each GPU the library's code object is loaded on holds
+// its own copy until the process ends. Host code uses only the per-device
+// address hipGetSymbolAddress returns, never the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// scratch's address on the current device, looked up once per device
+int scratch_on_current_device(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return CUPCFD_E_ERROR;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CUPCFD_E_ERROR;
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CUPCFD_E_SUCCESS;
+}
+}  // namespace
+
 extern "C" {
 
 int cupcfd_dev_cells_to_internal(cupcfd_hip_mesh *mesh, const void *src, void *dst, int32_t ncomp,
@@ -4046,6 +4078,24 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
                                       Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
                                       PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_dev_scalar_adjoint_noalloc_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
+                                          const double *MassFlux, const double *SuBar, const double *RFaceBar,
+                                          int32_t SolveTurb, double vislam, int32_t ivar, int32_t VarT,
+                                          double Sigma_T, double Prandtl, int32_t VarTE, double Sigma_k, int32_t VarED,
+                                          double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
+                                          double *PhiBar, double *MassFluxBar, void *stream) {
+    Mesh<double> *M = get<double>(mesh);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const size_t C = (size_t)M->d.C;
+    if (4 * C * sizeof(double) > kNoAllocScratchBytes) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    double *sp = nullptr;
+    const int rc = scratch_on_current_device(&sp);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
+                                      Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
+                                      PhiBar, MassFluxBar, stream, sp, sp + 3 * C);
 }
 
 int cupcfd_dev_scalar_adjoint_batched_f64(cupcfd_hip_mesh *mesh, int32_t K, const double *PhiCell,

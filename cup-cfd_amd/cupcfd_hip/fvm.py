@@ -252,14 +252,17 @@ class resident:
                  T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"]), cls._s(stream))
 
     @classmethod
-    def scalar_adjoint(cls, mesh, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None):
+    def scalar_adjoint(cls, mesh, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None, noalloc=False):
         """Adjoint of gradient (nGradient = 1) + scalar face loop w.r.t. (phi, mf): writes phibar (C) and
-        mfbar (F) from the cotangents subar (C) and rbar (2F). FP64 meshes whose regions are all DEFAULT."""
+        mfbar (F) from the cotangents subar (C) and rbar (2F). FP64 meshes whose regions are all DEFAULT.
+        ``noalloc`` calls cupcfd_dev_scalar_adjoint_noalloc_f64 (fixed library scratch, no device
+        allocation, meshes of up to 2,097,152 cells; same bits)."""
         from . import lib, check
         t = cls._t
         p = params
         D = ct.c_double
-        check("cupcfd_dev_scalar_adjoint_f64", lib().cupcfd_dev_scalar_adjoint_f64(
+        fn = "cupcfd_dev_scalar_adjoint_noalloc_f64" if noalloc else "cupcfd_dev_scalar_adjoint_f64"
+        check(fn, getattr(lib(), fn)(
             mesh.handle, t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
             _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
             _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),

@@ -485,6 +485,20 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
     double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
     double *PhiBar, double *MassFluxBar, void *stream);
+/* cupcfd_dev_scalar_adjoint_f64 with no device allocation: no counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its pass-A
+ * scratch (32 bytes per cell) lives in a fixed 64 MiB device array of the
+ * library, one per GPU, held until the process ends; meshes of more than
+ * 2,097,152 cells return CUPCFD_E_CONFIG_INVALID_VALUE and launch nothing.
+ * Results are bit for bit those of cupcfd_dev_scalar_adjoint_f64. Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+int cupcfd_dev_scalar_adjoint_noalloc_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,
+    const double *MassFlux, const double *SuBar, const double *RFaceBar, int32_t SolveTurb,
+    double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
+    double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
+    double *PhiBar, double *MassFluxBar, void *stream);
 
 /* Library version string and the gfx target it was built for. */
 const char *cupcfd_hip_version(void);
