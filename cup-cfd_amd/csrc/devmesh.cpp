@@ -14,6 +14,7 @@ namespace cupcfd_hip {
 template <class T> Mesh<T>::~Mesh() {
     for (void *p : allocs) (void)hipFree(p);
     for (void *p : ws) (void)hipFree(p);
+    for (void *p : ws_retired) (void)hipFree(p);
     for (int k = 0; k < 4; k++)
         if (sc_src[k]) (void)hipFree(sc_src[k]);
     if (rbnd) (void)hipFree(rbnd);
@@ -28,10 +29,10 @@ template <class T> void *Mesh<T>::ws_get(int slot, size_t bytes) {
         ws_size.resize(slot + 1, 0);
     }
     if (ws_size[slot] < bytes) {
-        if (ws[slot]) (void)hipFree(ws[slot]);
-        ws[slot] = nullptr;
-        ws_size[slot] = 0;
-        if (hipMalloc(&ws[slot], bytes) != hipSuccess) return nullptr;
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        if (ws[slot]) ws_retired.push_back(ws[slot]);
+        ws[slot] = p;
         ws_size[slot] = bytes;
     }
     return ws[slot];

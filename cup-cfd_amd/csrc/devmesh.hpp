@@ -131,9 +131,13 @@ template <class T> struct Mesh {
     int ncolours = -1;
     const int *colour_of = nullptr, *colour_faces = nullptr;
     std::vector<int> colour_off;
-    // workspace pool for drop-in staging and multi-sweep scratch
+    // workspace pool for drop-in staging and multi-sweep scratch. A slot that
+    // must grow gets a new buffer; the old one is retired, not freed, until
+    // the mesh is destroyed, so a graph captured with the old buffer never
+    // replays into freed memory.
     std::vector<void *> ws;
     std::vector<size_t> ws_size;
+    std::vector<void *> ws_retired;
     void *ws_get(int slot, size_t bytes);
     ~Mesh();
 };

@@ -3548,20 +3548,16 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 4, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 4) ADJ_BATCHED_ROWS(4);
         else if (G == 16 && RL == 6) {
-            if (K <= 16)
-                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6, true>), grid, kBlock, 0, st, d, a, K, gr.per,
-                                   gr.nblk);
-            else
-                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6, false>), grid, kBlock, 0, st, d, a, K, gr.per,
-                                   gr.nblk);
+            // G == 16 means 4 < K <= 16 (batch_group): one chunk of scalars, so pass A
+            // writes the mass-flux cotangent (mf_in_a below)
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 6, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                               gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 6>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 16 && RL == 4) {
-            if (K <= 16)
-                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4, true>), grid, kBlock, 0, st, d, a, K, gr.per,
-                                   gr.nblk);
-            else
-                hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4, false>), grid, kBlock, 0, st, d, a, K, gr.per,
-                                   gr.nblk);
+            // G == 16 means 4 < K <= 16 (batch_group): one chunk of scalars, so pass A
+            // writes the mass-flux cotangent (mf_in_a below)
+            hipLaunchKernelGGL((k_adj_a_batched_staged<T, TS, 16, 4, true>), grid, kBlock, 0, st, d, a, K, gr.per,
+                               gr.nblk);
             hipLaunchKernelGGL((k_adj_b_batched_staged<T, TS, 16, 4>), grid, kBlock, 0, st, d, a, K, gr.per, gr.nblk);
         } else if (G == 16) ADJ_BATCHED_ROWS(16);
         else ADJ_BATCHED_ROWS(64);

@@ -176,7 +176,11 @@ def calculateViscosityDolfynCellLoop2(mesh, visLam, visEffCell, visEffBoundary):
 class resident:
     """cupcfd_dev_* over torch device tensors (internal order, packed xyz vectors).
 
-    ``stream`` is a torch.cuda.Stream or None (the current stream)."""
+    ``stream`` is a torch.cuda.Stream or None (the current stream). Every tensor is checked before the call:
+    on the GPU, contiguous, of the mesh's precision (float32 for the per-scalar arrays of the _f32s calls)
+    and at least as long as the mesh needs (cells, 3 x cells, faces, 2 x faces, boundaries or regions, times
+    K for the batched per-scalar arrays); a violation raises CupCfdError (E_ARRAY_SIZE_MISMATCH for a short
+    tensor, E_CONFIG_INVALID_VALUE otherwise) and launches nothing."""
 
     @staticmethod
     def _s(stream):
@@ -189,6 +193,45 @@ class resident:
         return None if t is None else ct.c_void_p(t.data_ptr())
 
     @staticmethod
+    def _chk(t, n, dtype, name, gpu=True):
+        """ctypes pointer of tensor t after checking that it is a contiguous CUDA tensor of ``dtype`` with at
+        least n elements (None passes through: the C ABI decides whether the argument may be absent)."""
+        if t is None:
+            return None
+        import torch
+        from . import CupCfdError, E_ARRAY_SIZE_MISMATCH
+        bad = 90  # E_CONFIG_INVALID_VALUE
+        if not isinstance(t, torch.Tensor):
+            raise CupCfdError(f"{name}: expected a torch tensor, got {type(t).__name__}", bad)
+        if t.dtype != dtype:
+            raise CupCfdError(f"{name}: dtype {t.dtype}, expected {dtype}", bad)
+        if not t.is_contiguous():
+            raise CupCfdError(f"{name}: tensor is not contiguous", bad)
+        if t.numel() < n:
+            raise CupCfdError(f"{name}: {t.numel()} elements, the mesh needs {n}", E_ARRAY_SIZE_MISMATCH)
+        if gpu and not t.is_cuda:
+            raise CupCfdError(f"{name}: tensor is not on the GPU", bad)
+        return ct.c_void_p(t.data_ptr())
+
+    @classmethod
+    def _args(cls, mesh, spec, K=1, f32=()):
+        """Checked pointers for [(tensor, kind, name), ...]; names in f32 are float32 per-scalar arrays times K,
+        kinds ending in 'K' are per-scalar (times K)."""
+        import torch
+        size = {"c": mesh.nCells, "c3": 3 * mesh.nCells, "f": mesh.nFaces, "f2": 2 * mesh.nFaces,
+                "b": mesh.nBoundaries, "r": mesh.nRegions, "1": 1}
+        native = torch.float64 if mesh.precision == 64 else torch.float32
+        out = []
+        for t, kind, name in spec:  # layout and sizes of every argument first, then the device
+            n = size[kind.rstrip("K")] * (K if kind.endswith("K") else 1)
+            out.append(cls._chk(t, n, torch.float32 if name in f32 else native, name, gpu=False))
+        for t, _, name in spec:
+            if t is not None and not t.is_cuda:
+                from . import CupCfdError
+                raise CupCfdError(f"{name}: tensor is not on the GPU", 90)
+        return out
+
+    @staticmethod
     def call(mesh, name, *args):
         from . import lib, check
         fn = f"cupcfd_dev_{name}_{mesh.sfx}"
@@ -196,60 +239,86 @@ class resident:
 
     @classmethod
     def gradient(cls, mesh, nGradient, phi, phiB, g, go, stream=None):
-        cls.call(mesh, "gradient_phi_gauss_dolfyn", mesh.handle, _I(nGradient), cls._t(phi), cls._t(phiB),
-                 cls._t(g), cls._t(go), cls._s(stream))
+        a = cls._args(mesh, [(phi, "c", "phi"), (phiB, "b", "phiB"), (g, "c3", "dPhidx"), (go, "c3", "dPhidxo")])
+        cls.call(mesh, "gradient_phi_gauss_dolfyn", mesh.handle, _I(nGradient), *a, cls._s(stream))
 
     @classmethod
     def mass(cls, mesh, dudx, dvdx, dwdx, dpdx, den, denB, u, v, w, mf, p, ar, su, rface, small=1e-18,
              counters=None, flags=(0, 0, 1, 0), teC=None, teB=None, edC=None, edB=None, visC=None, visB=None,
              tC=None, tB=None, stream=None):
         cnt = counters if counters is not None else np.zeros(4, np.int32)
-        t = cls._t
+        assert isinstance(cnt, np.ndarray) and cnt.dtype == np.int32 and cnt.size >= 4 and cnt.flags["C_CONTIGUOUS"]
+        a = cls._args(mesh, [(dudx, "c3", "dudx"), (dvdx, "c3", "dvdx"), (dwdx, "c3", "dwdx"), (dpdx, "c3", "dpdx"),
+                             (den, "c", "den"), (denB, "b", "denB"), (u, "c", "u"), (v, "c", "v"), (w, "c", "w"),
+                             (mf, "f", "massFlux"), (p, "c", "p"), (ar, "c", "ar"), (su, "c", "su"),
+                             (rface, "f2", "rface")])
+        e = cls._args(mesh, [(teC, "c", "teCell"), (teB, "b", "teBoundary"), (edC, "c", "edCell"),
+                             (edB, "b", "edBoundary"), (visC, "c", "viseffCell"), (visB, "b", "viseffBoundary"),
+                             (tC, "c", "tCell"), (tB, "b", "tBoundary")])
         base = cnt.ctypes.data
-        cls.call(mesh, "flux_mass_face_loop", mesh.handle, t(dudx), t(dvdx), t(dwdx), t(dpdx), t(den), t(denB),
-                 t(u), t(v), t(w), t(mf), t(p), t(ar), t(su), t(rface), mesh.T(small), ct.c_void_p(base),
+        cls.call(mesh, "flux_mass_face_loop", mesh.handle, *a, mesh.T(small), ct.c_void_p(base),
                  ct.c_void_p(base + 4), ct.c_void_p(base + 8), ct.c_void_p(base + 12), *[_I(x) for x in flags],
-                 t(teC), t(teB), t(edC), t(edB), t(visC), t(visB), t(tC), t(tB), cls._s(stream))
+                 *e, cls._s(stream))
         return cnt
+
+    @classmethod
+    def _scalar_spec(cls, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx, K=None):
+        k = "K" if K is not None else ""
+        return [(phi, "c" + k, "phi"), (phiB, "b" + k, "phiB"), (visEff, "c", "visEff"), (au, "c" + k, "au"),
+                (su, "c" + k, "su"), (den, "c", "den"), (mf, "f", "massFlux"), (te, "c", "te"), (cpB, "b", "cpB"),
+                (visEffB, "b", "visEffB"), (rface, "f2", "rface"), (dphidx, "c3" + k, "dphidx")]
+
+    @staticmethod
+    def _scalar_params(T, p):
+        return [_I(p["SolveTurb"]), _I(p["SolveEnthalpy"]), T(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]),
+                T(p["Sigma_T"]), T(p["Prandtl"]), _I(p["VarTE"]), T(p["Sigma_k"]), _I(p["VarED"]), T(p["Sigma_e"]),
+                T(p["Sigma_s"]), T(p["Schmidt"]), T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"])]
 
     @classmethod
     def scalar(cls, mesh, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx, params,
                stream=None):
-        t = cls._t
-        T = mesh.T
-        p = params
-        cls.call(mesh, "flux_scalar_face_loop", mesh.handle, t(phi), t(phiB), t(visEff), t(au), t(su), t(den),
-                 t(mf), t(te), t(cpB), t(visEffB), t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]),
-                 T(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]), T(p["Sigma_T"]), T(p["Prandtl"]), _I(p["VarTE"]),
-                 T(p["Sigma_k"]), _I(p["VarED"]), T(p["Sigma_e"]), T(p["Sigma_s"]), T(p["Schmidt"]),
-                 T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"]), cls._s(stream))
+        a = cls._args(mesh, cls._scalar_spec(phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx))
+        cls.call(mesh, "flux_scalar_face_loop", mesh.handle, *a, *cls._scalar_params(mesh.T, params), cls._s(stream))
 
     @classmethod
     def uvw(cls, mesh, gammaBlend, small, u, v, w, uB, vB, wB, visEff, visEffB, mf, dudx, dvdx, dwdx, rface, su, sv,
             sw, au, av, aw, stream=None):
-        t = cls._t
-        cls.call(mesh, "flux_uvw_face_loop1", mesh.handle, mesh.T(gammaBlend), mesh.T(small), t(u), t(v), t(w),
-                 t(uB), t(vB), t(wB), t(visEff), t(visEffB), t(mf), t(dudx), t(dvdx), t(dwdx), t(rface), t(su),
-                 t(sv), t(sw), t(au), t(av), t(aw), cls._s(stream))
+        a = cls._args(mesh, [(u, "c", "u"), (v, "c", "v"), (w, "c", "w"), (uB, "b", "uB"), (vB, "b", "vB"),
+                             (wB, "b", "wB"), (visEff, "c", "visEff"), (visEffB, "b", "visEffB"),
+                             (mf, "f", "massFlux"), (dudx, "c3", "dudx"), (dvdx, "c3", "dvdx"), (dwdx, "c3", "dwdx"),
+                             (rface, "f2", "rface"), (su, "c", "su"), (sv, "c", "sv"), (sw, "c", "sw"),
+                             (au, "c", "au"), (av, "c", "av"), (aw, "c", "aw")])
+        cls.call(mesh, "flux_uvw_face_loop1", mesh.handle, mesh.T(gammaBlend), mesh.T(small), *a, cls._s(stream))
 
     @classmethod
     def gradient_batched(cls, mesh, nGradient, K, phi, phiB, g, go, stream=None):
         """K scalars: phi (C, K), phiB (B, K), g and go (C, 3, K), scalar-innermost."""
-        cls.call(mesh, "gradient_batched", mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB),
-                 cls._t(g), cls._t(go), cls._s(stream))
+        a = cls._args(mesh, [(phi, "cK", "phi"), (phiB, "bK", "phiB"), (g, "c3K", "dPhidx"), (go, "c3K", "dPhidxo")],
+                      K=K)
+        cls.call(mesh, "gradient_batched", mesh.handle, _I(nGradient), _I(K), *a, cls._s(stream))
 
     @classmethod
     def scalar_batched(cls, mesh, K, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx, params,
                        stream=None):
         """K scalars: phi, au, su (C, K), phiB (B, K), dphidx (C, 3, K); the rest shared."""
-        t = cls._t
-        T = mesh.T
-        p = params
-        cls.call(mesh, "flux_scalar_batched", mesh.handle, _I(K), t(phi), t(phiB), t(visEff), t(au), t(su), t(den),
-                 t(mf), t(te), t(cpB), t(visEffB), t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]),
-                 T(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]), T(p["Sigma_T"]), T(p["Prandtl"]), _I(p["VarTE"]),
-                 T(p["Sigma_k"]), _I(p["VarED"]), T(p["Sigma_e"]), T(p["Sigma_s"]), T(p["Schmidt"]),
-                 T(p["GammaBlend"]), T(p["Small"]), T(p["Large"]), T(p["TMCmu"]), cls._s(stream))
+        a = cls._args(mesh, cls._scalar_spec(phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx,
+                                             K=K), K=K)
+        cls.call(mesh, "flux_scalar_batched", mesh.handle, _I(K), *a, *cls._scalar_params(mesh.T, params),
+                 cls._s(stream))
+
+    @staticmethod
+    def _adjoint_params(p):
+        D = ct.c_double
+        return [_I(p["SolveTurb"]), D(p["vislam"]), _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]),
+                _I(p["VarTE"]), D(p["Sigma_k"]), _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]),
+                D(p["GammaBlend"])]
+
+    @classmethod
+    def _adjoint_args(cls, mesh, phi, visEff, mf, subar, rbar, phibar, mfbar, K=None, f32=()):
+        k = "K" if K is not None else ""
+        return cls._args(mesh, [(phi, "c" + k, "phi"), (visEff, "c", "visEff"), (mf, "f", "massFlux"),
+                                (subar, "c" + k, "subar"), (rbar, "f2", "rbar"), (phibar, "c" + k, "phibar"),
+                                (mfbar, "f", "mfbar")], K=K or 1, f32=f32)
 
     @classmethod
     def scalar_adjoint(cls, mesh, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None, noalloc=False):
@@ -258,15 +327,9 @@ class resident:
         ``noalloc`` calls cupcfd_dev_scalar_adjoint_noalloc_f64 (fixed library scratch, no device
         allocation, meshes of up to 2,097,152 cells; same bits)."""
         from . import lib, check
-        t = cls._t
-        p = params
-        D = ct.c_double
+        a = cls._adjoint_args(mesh, phi, visEff, mf, subar, rbar, phibar, mfbar)
         fn = "cupcfd_dev_scalar_adjoint_noalloc_f64" if noalloc else "cupcfd_dev_scalar_adjoint_f64"
-        check(fn, getattr(lib(), fn)(
-            mesh.handle, t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
-            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
-            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
-            t(mfbar), cls._s(stream)))
+        check(fn, getattr(lib(), fn)(mesh.handle, *a[:5], *cls._adjoint_params(params), *a[5:], cls._s(stream)))
 
     @classmethod
     def gradient_batched_f32s(cls, mesh, nGradient, K, phi, phiB, g, go, stream=None):
@@ -274,8 +337,10 @@ class resident:
         geometry and arithmetic, the final gradient rounded to float."""
         from . import lib, check
         assert mesh.precision == 64
+        a = cls._args(mesh, [(phi, "cK", "phi"), (phiB, "bK", "phiB"), (g, "c3K", "dPhidx"), (go, "c3K", "dPhidxo")],
+                      K=K, f32=("phi", "phiB", "dPhidx", "dPhidxo"))
         check("cupcfd_dev_gradient_batched_f32s", lib().cupcfd_dev_gradient_batched_f32s(
-            mesh.handle, _I(nGradient), _I(K), cls._t(phi), cls._t(phiB), cls._t(g), cls._t(go), cls._s(stream)))
+            mesh.handle, _I(nGradient), _I(K), *a, cls._s(stream)))
 
     @classmethod
     def scalar_batched_f32s(cls, mesh, K, phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx,
@@ -284,90 +349,82 @@ class resident:
         for scalar_batched); the shared arrays and all arithmetic float64, Au/Su/PhiB rounded to float once."""
         from . import lib, check
         assert mesh.precision == 64
-        t = cls._t
-        p = params
-        D = ct.c_double
+        a = cls._args(mesh, cls._scalar_spec(phi, phiB, visEff, au, su, den, mf, te, cpB, visEffB, rface, dphidx,
+                                             K=K), K=K, f32=("phi", "phiB", "au", "su", "dphidx"))
         check("cupcfd_dev_flux_scalar_batched_f32s", lib().cupcfd_dev_flux_scalar_batched_f32s(
-            mesh.handle, _I(K), t(phi), t(phiB), t(visEff), t(au), t(su), t(den), t(mf), t(te), t(cpB), t(visEffB),
-            t(rface), t(dphidx), _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]), D(p["vislam"]), _I(p["ivar"]),
-            _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]), _I(p["VarED"]),
-            D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), D(p["Small"]), D(p["Large"]),
-            D(p["TMCmu"]), cls._s(stream)))
+            mesh.handle, _I(K), *a, *cls._scalar_params(ct.c_double, params), cls._s(stream)))
 
     @classmethod
     def gradient_batched_set(cls, meshset, K, phi_ptrs, phiB_ptrs, g_ptrs, go_ptrs=None, stream=None):
         """Batched gradient (nGradient = 1) on every mesh of a DeviceMeshSet in one persistent launch.
         *_ptrs are int64 device tensors holding each member's device array address (data_ptr())."""
+        import torch
         from . import lib, check
+        n = len(meshset.meshes)  # the pointer tables hold one address per member mesh
+        ptrs = [cls._chk(t, n, torch.int64, name)
+                for t, name in ((phi_ptrs, "phi_ptrs"), (phiB_ptrs, "phiB_ptrs"), (g_ptrs, "g_ptrs"),
+                                (go_ptrs, "go_ptrs"))]
         fn = f"cupcfd_dev_gradient_batched_set_{meshset.sfx}"
-        check(fn, getattr(lib(), fn)(meshset.handle, _I(K), cls._t(phi_ptrs), cls._t(phiB_ptrs), cls._t(g_ptrs),
-                                     cls._t(go_ptrs), cls._s(stream)))
+        check(fn, getattr(lib(), fn)(meshset.handle, _I(K), *ptrs, cls._s(stream)))
 
     @classmethod
     def scalar_adjoint_batched(cls, mesh, K, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None):
         """Batched adjoint: phi, subar, phibar (C, K) scalar-innermost; visEff, mf, rbar shared; mfbar (F) sums
         the K scalars' terms after the RFace term."""
         from . import lib, check
-        t = cls._t
-        p = params
-        D = ct.c_double
+        a = cls._adjoint_args(mesh, phi, visEff, mf, subar, rbar, phibar, mfbar, K=K)
         check("cupcfd_dev_scalar_adjoint_batched_f64", lib().cupcfd_dev_scalar_adjoint_batched_f64(
-            mesh.handle, _I(K), t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
-            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
-            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
-            t(mfbar), cls._s(stream)))
+            mesh.handle, _I(K), *a[:5], *cls._adjoint_params(params), *a[5:], cls._s(stream)))
 
     @classmethod
     def scalar_adjoint_batched_f32s(cls, mesh, K, phi, visEff, mf, subar, rbar, params, phibar, mfbar, stream=None):
         """Batched adjoint with float32 storage of phi, subar and phibar (the rest float64), on a 64-bit mesh."""
         from . import lib, check
         assert mesh.precision == 64
-        t = cls._t
-        p = params
-        D = ct.c_double
+        a = cls._adjoint_args(mesh, phi, visEff, mf, subar, rbar, phibar, mfbar, K=K, f32=("phi", "subar", "phibar"))
         check("cupcfd_dev_scalar_adjoint_batched_f32s", lib().cupcfd_dev_scalar_adjoint_batched_f32s(
-            mesh.handle, _I(K), t(phi), t(visEff), t(mf), t(subar), t(rbar), _I(p["SolveTurb"]), D(p["vislam"]),
-            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
-            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), t(phibar),
-            t(mfbar), cls._s(stream)))
+            mesh.handle, _I(K), *a[:5], *cls._adjoint_params(params), *a[5:], cls._s(stream)))
 
     # the boundary, region and viscosity loops; scalar outputs (flowin, flowout, ratearea, ...) are one-value
     # device tensors
     @classmethod
     def mass_bl1(cls, mesh, mf, flowin, stream=None):
         """MassKernels.h:90-94"""
-        cls.call(mesh, "flux_mass_boundary_loop1", mesh.handle, cls._t(mf), cls._t(flowin), cls._s(stream))
+        a = cls._args(mesh, [(mf, "f", "massFlux"), (flowin, "1", "flowin")])
+        cls.call(mesh, "flux_mass_boundary_loop1", mesh.handle, *a, cls._s(stream))
 
     @classmethod
     def mass_bl2(cls, mesh, mf, flowRegion, flowout, stream=None):
         """MassKernels.h:103-108"""
-        t = cls._t
-        cls.call(mesh, "flux_mass_boundary_loop2", mesh.handle, t(mf), t(flowRegion), t(flowout), cls._s(stream))
+        a = cls._args(mesh, [(mf, "f", "massFlux"), (flowRegion, "r", "flowRegion"), (flowout, "1", "flowout")])
+        cls.call(mesh, "flux_mass_boundary_loop2", mesh.handle, *a, cls._s(stream))
 
     @classmethod
     def mass_bl3(cls, mesh, flowin, ratearea, stream=None):
         """MassKernels.h:117-120 (flowin: a host value)"""
-        cls.call(mesh, "flux_mass_boundary_loop3", mesh.handle, mesh.T(flowin), cls._t(ratearea), cls._s(stream))
+        a = cls._args(mesh, [(ratearea, "1", "ratearea")])
+        cls.call(mesh, "flux_mass_boundary_loop3", mesh.handle, mesh.T(flowin), *a, cls._s(stream))
 
     @classmethod
     def mass_bl4(cls, mesh, mf, uB, vB, wB, denB, ratearea, flowout, stream=None):
         """MassKernels.h:129-138 (ratearea: a host value)"""
-        t = cls._t
-        cls.call(mesh, "flux_mass_boundary_loop4", mesh.handle, t(mf), t(uB), t(vB), t(wB), t(denB),
-                 mesh.T(ratearea), t(flowout), cls._s(stream))
+        a = cls._args(mesh, [(mf, "f", "massFlux"), (uB, "b", "uB"), (vB, "b", "vB"), (wB, "b", "wB"),
+                             (denB, "b", "denB"), (flowout, "1", "flowout")])
+        cls.call(mesh, "flux_mass_boundary_loop4", mesh.handle, *a[:5], mesh.T(ratearea), a[5], cls._s(stream))
 
     @classmethod
     def mass_bl5(cls, mesh, mf, su, uB, vB, wB, fact, solveU, solveV, solveW, flowFact, flowout2, stream=None):
         """MassKernels.h:147-157"""
-        t = cls._t
-        cls.call(mesh, "flux_mass_boundary_loop5", mesh.handle, t(mf), t(su), t(uB), t(vB), t(wB), mesh.T(fact),
-                 _I(solveU), _I(solveV), _I(solveW), t(flowFact), t(flowout2), cls._s(stream))
+        a = cls._args(mesh, [(mf, "f", "massFlux"), (su, "c", "su"), (uB, "b", "uB"), (vB, "b", "vB"),
+                             (wB, "b", "wB"), (flowFact, "r", "flowFact"), (flowout2, "1", "flowout2")])
+        cls.call(mesh, "flux_mass_boundary_loop5", mesh.handle, *a[:5], mesh.T(fact), _I(solveU), _I(solveV),
+                 _I(solveW), *a[5:], cls._s(stream))
 
     @classmethod
     def mass_region(cls, mesh, flowFact, flowRegion, flowIn, stream=None):
         """MassKernels.h:166-171 (flowIn: a host value)"""
-        cls.call(mesh, "flux_mass_region_loop", mesh.handle, cls._t(flowFact), cls._t(flowRegion), mesh.T(flowIn),
-                 cls._s(stream))
+        a = cls._args(mesh, [(flowFact, "r", "flowFact"), (flowRegion, "r", "flowRegion")])
+        cls.call(mesh, "flux_mass_region_loop", mesh.handle, *a, mesh.T(flowIn), cls._s(stream))
 
     @classmethod
     def uvw_region1(cls, mesh, stream=None):
@@ -382,9 +439,10 @@ class resident:
     @classmethod
     def visc_cell1(cls, mesh, small, cmu, visURF, visLam, te, ed, den, visEff, stream=None):
         """ViscosityKernels.h:38-48"""
-        t, T = cls._t, mesh.T
-        cls.call(mesh, "viscosity_cell_loop1", mesh.handle, T(small), T(cmu), T(visURF), T(visLam), t(te), t(ed),
-                 t(den), t(visEff), cls._s(stream))
+        T = mesh.T
+        a = cls._args(mesh, [(te, "c", "TE"), (ed, "c", "ED"), (den, "c", "den"), (visEff, "c", "visEff")])
+        cls.call(mesh, "viscosity_cell_loop1", mesh.handle, T(small), T(cmu), T(visURF), T(visLam), *a,
+                 cls._s(stream))
 
     @classmethod
     def visc_region(cls, mesh, kappa, stream=None):
@@ -396,12 +454,13 @@ class resident:
     @classmethod
     def visc_boundary(cls, mesh, tmcmu, large, small, kappa, visLam, te, den, visEff, visEffB, stream=None):
         """ViscosityKernels.h:53-60"""
-        t, T = cls._t, mesh.T
+        T = mesh.T
+        a = cls._args(mesh, [(te, "c", "TE"), (den, "c", "den"), (visEff, "c", "visEff"), (visEffB, "b", "visEffB")])
         cls.call(mesh, "viscosity_boundary_loop", mesh.handle, T(tmcmu), T(large), T(small), T(kappa), T(visLam),
-                 t(te), t(den), t(visEff), t(visEffB), cls._s(stream))
+                 *a, cls._s(stream))
 
     @classmethod
     def visc_cell2(cls, mesh, visLam, visEff, visEffB, stream=None):
         """ViscosityKernels.h:62-67"""
-        cls.call(mesh, "viscosity_cell_loop2", mesh.handle, mesh.T(visLam), cls._t(visEff), cls._t(visEffB),
-                 cls._s(stream))
+        a = cls._args(mesh, [(visEff, "c", "visEff"), (visEffB, "b", "visEffB")])
+        cls.call(mesh, "viscosity_cell_loop2", mesh.handle, mesh.T(visLam), *a, cls._s(stream))

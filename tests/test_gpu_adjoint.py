@@ -245,3 +245,47 @@ def test_batched_adjoint_f32_storage_is_rounded_f64(name, K):
     (pb32, mb32), (pb64, mb64) = out
     assert torch.equal(pb32, pb64.to(torch.float32))
     assert torch.equal(mb32, mb64)
+
+
+@pytest.mark.gpu
+def test_graph_survives_workspace_growth():
+    """A batched adjoint captured in a hipGraph at K = 3 keeps replaying correctly after an eager call at
+    K = 64 has grown the mesh's scratch: the outgrown buffers stay allocated until the mesh is destroyed."""
+    c = Case(nx=9, ny=8, nz=7, seed=21, permute_cells=True, permute_faces=True)
+    rng = np.random.default_rng(123)
+    mf, vis = c.faces(c.f.massFlux), c.cells(c.f.visEff)
+    rb = c.faces(rng.normal(size=2 * c.F), 2)
+
+    def inputs(K):
+        return c.cells(rng.uniform(-1, 1, (c.C, K))), c.cells(rng.normal(size=(c.C, K)))
+
+    phi3, sb3 = inputs(3)
+    pb3 = torch.empty((c.C, 3), dtype=torch.float64, device="cuda")
+    mb3 = torch.empty(c.F, dtype=torch.float64, device="cuda")
+    R.scalar_adjoint_batched(c.dev, 3, phi3, vis, mf, sb3, rb, P, pb3, mb3)  # sizes the K = 3 scratch
+    torch.cuda.synchronize()
+    want_p, want_m = pb3.clone(), mb3.clone()
+    stream = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(graph, stream=stream):
+            R.scalar_adjoint_batched(c.dev, 3, phi3, vis, mf, sb3, rb, P, pb3, mb3, stream=stream)
+    torch.cuda.synchronize()
+    phi64, sb64 = inputs(64)
+    pb64 = torch.empty((c.C, 64), dtype=torch.float64, device="cuda")
+    mb64 = torch.empty(c.F, dtype=torch.float64, device="cuda")
+    R.scalar_adjoint_batched(c.dev, 64, phi64, vis, mf, sb64, rb, P, pb64, mb64)  # grows the scratch
+    torch.cuda.synchronize()
+    eager64 = (pb64.clone(), mb64.clone())
+    for _ in range(3):
+        pb3.fill_(0.0)
+        mb3.fill_(0.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(pb3, want_p) and torch.equal(mb3, want_m)
+    # the K = 64 results were not disturbed by the replays, and a fresh K = 64 call still agrees
+    R.scalar_adjoint_batched(c.dev, 64, phi64, vis, mf, sb64, rb, P, pb64, mb64)
+    torch.cuda.synchronize()
+    assert torch.equal(pb64, eager64[0]) and torch.equal(mb64, eager64[1])
+    del graph
