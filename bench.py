@@ -205,6 +205,122 @@ def cpu_baseline_all_cores(n, budget_s):
     }
 
 
+def oracle_outputs(host, f, kernels=("gradient", "mass", "scalar", "uvw")):
+    """The oracle's outputs of each kernel on fresh copies of ``f`` (mesh numbering), FP64."""
+    import pyoracle as po
+    from kernel_runs import run
+
+    m = po.OracleMesh(host, 64)
+    return {k: run(k, po, m, f.copy(), lambda: {})[1] for k in kernels}
+
+
+def parity_vs_oracle(dev, f, oracle):
+    """Each kernel once through the drop-in C ABI on fresh copies of ``f`` against the oracle's outputs on the
+    same inputs: per output, bit identity and the max-norm relative difference."""
+    from cupcfd_hip import fvm
+    from kernel_runs import compare, run
+
+    res, all_bits = {}, True
+    for k, o in oracle.items():
+        _, g = run(k, fvm, dev, f.copy(), lambda: {})
+        res[k] = {}
+        for name, want in o.items():
+            a, b = np.ascontiguousarray(g[name]), np.ascontiguousarray(want)
+            bits = a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+            res[k][name] = {"bit_identical": bool(bits), "max_rel": compare(a, b)[1]}
+            all_bits = all_bits and bits
+    return {"all_bit_identical": bool(all_bits), "kernels": res}
+
+
+def bench_config0(torch, iters=100):
+    """The reference's CPU benchmark case: StructGen 32^3 over [0,1]^3 (one DEFAULT region), FP64,
+    ``iters`` iterations of GradientPhiGaussDolfyn + FluxMassDolfynFaceLoop. Timed on the GPU resident
+    (HIP events around all iterations), through the drop-in C ABI (host arrays, synchronous: the
+    reference's calling convention) and on one host core with the oracle."""
+    import cupcfd_hip as ch
+    import pyoracle as po
+    from cupcfd_hip import fvm
+    from cupcfd_hip import workload as W
+    from cupcfd_hip.fvm import resident as R
+
+    host = ch.HostMesh(32)
+    dev = ch.DeviceMesh(host, 64, reorder=True)
+    f = W.Fields(host, seed=0)
+    C, F = host.nCells, host.nFaces
+    kernels, d = make_sweep(torch, host, dev, f, arrays=True)
+    k = dict(kernels)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        k["gradient"]()
+        k["mass"]()
+
+    quiesce()
+    step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(iters):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_gpu = e0.elapsed_time(e1) / 1e3
+    # drop-in: host arrays, one synchronous call each
+    cnt = np.zeros(4, np.int32)
+    c = [cnt[i:i + 1] for i in range(4)]
+    g, go = np.zeros((C, 3)), np.zeros((C, 3))
+    mf, rface, su = f.massFlux.copy(), np.zeros(2 * F), f.su.copy()
+    q = W.MASS
+
+    def dropin(api, mesh):
+        api.GradientPhiGaussDolfyn(mesh, 1, f.phi, f.phiB, g, go)
+        api.FluxMassDolfynFaceLoop(mesh, f.dudx, f.dvdx, f.dwdx, f.dpdx, f.den, f.denB, f.u, f.v, f.w, mf, f.p, f.ar,
+                                   su, rface, q["small"], c[0], c[1], c[2], c[3], 1, 1, 1, 1, f.te, f.teB, f.ed,
+                                   f.edB, f.visEff, f.visEffB, f.t, f.tB)
+
+    quiesce()
+    dropin(fvm, dev)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        dropin(fvm, dev)
+    t_drop = time.perf_counter() - t0
+    om = po.OracleMesh(host, 64)
+    dropin(po, om)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        dropin(po, om)
+    t_cpu = time.perf_counter() - t0
+    rate = lambda t: C * iters / t / 1e9  # noqa: E731
+    return {"workload": f"StructGen 32^3 over [0,1]^3, one DEFAULT region, FP64, {iters} x (gradient + mass flux)",
+            "cells": C, "iterations": iters,
+            "gpu_resident_ms": t_gpu * 1e3, "gpu_resident_gcell_iters_per_s": rate(t_gpu),
+            "gpu_dropin_ms": t_drop * 1e3, "gpu_dropin_gcell_iters_per_s": rate(t_drop),
+            "cpu_oracle_1core_ms": t_cpu * 1e3, "cpu_oracle_1core_gcell_iters_per_s": rate(t_cpu),
+            "note": "32^3 fits the 256 MB Infinity Cache: a latency point, not an HBM point"}
+
+
+def bench_norms(torch, dev, fields, steps):
+    """Residual norms (max |x|, sqrt(sum x^2), FP64) of the sweep's source terms, timed over ``steps``
+    calls; algorithmic bytes = one read of each field."""
+    from cupcfd_hip.fvm import resident as R
+
+    stream = torch.cuda.current_stream()
+    out = torch.zeros((len(fields), 2), dtype=torch.float64, device="cuda")
+    R.residual_norms(dev, fields, out, stream)
+    torch.cuda.synchronize()
+    quiesce()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        R.residual_norms(dev, fields, out, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / steps
+    nbytes = sum(x.numel() * x.element_size() for x in fields)
+    return {"fields": ["scalar Su", "UVW su", "UVW sv", "UVW sw"], "us": t * 1e6, "algorithmic_bytes": nbytes,
+            "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS, "values": out.cpu().numpy().tolist()}
+
+
 def batched_fields(host, K, seed):
     """K scalar fields phi_k = sum_{m=1..8} a_m sin(k_m . x + theta_m) + 0.05 U(-1,1) at cell centres (and
     boundary face centres), with seeded wave vectors, phases and amplitudes; returned scalar-innermost."""
@@ -288,10 +404,28 @@ SWEEP_OUTPUTS = [("gradient", "dPhidx", "g"), ("mass", "massFlux", "mf_out"), ("
                  ("uvw", "su", "su"), ("uvw", "sv", "sv"), ("uvw", "sw", "sw"), ("uvw", "rFace", "rface3")]
 
 
-def bench_f32(torch, host, f, steps, warmup, schedule):
+# kernel_runs output name of each SWEEP_OUTPUTS entry (mesh numbering)
+ORACLE_NAME = {("mass", "RFace"): "rface"}
+
+
+def oracle_internal(dev, host, kern, name, a):
+    """An oracle output (mesh numbering) of the sweep in the device mesh's internal order, flattened."""
+    co, fo, F = dev.cell_order, dev.face_order, host.nFaces
+    a = np.asarray(a)
+    if kern == "mass" and name == "massFlux":
+        return a[fo].ravel()
+    if name in ("RFace", "rFace"):
+        return a.reshape(F, 2)[fo].ravel()
+    if name == "PhiB":
+        return a.ravel()
+    return a[co].ravel()
+
+
+def bench_f32(torch, host, f, steps, warmup, schedule, oracle=None):
     """The same sweep in FP32 (T = float throughout: the reference's float instantiation, FP32 mesh and
     fields): per-kernel time and algorithmic-byte fraction of the HBM peak at s = 4, and the max-norm relative
-    error of each output against the FP64 sweep of the same inputs after one sweep from the same state."""
+    error of each output after one sweep against the FP64 oracle on the same inputs (``oracle``, outputs of
+    oracle_outputs), or, without it, against the FP64 sweep on the GPU."""
     import cupcfd_hip as ch
 
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
@@ -313,9 +447,14 @@ def bench_f32(torch, host, f, steps, warmup, schedule):
     torch.cuda.synchronize()
     err = {}
     for kern, name, key in SWEEP_OUTPUTS:
-        ref = a64[key]
+        if oracle is not None:
+            o = oracle[kern][ORACLE_NAME.get((kern, name), name)]
+            ref = torch.from_numpy(np.ascontiguousarray(oracle_internal(d32, host, kern, name, o))).cuda()
+        else:
+            ref = a64[key]
         scale = ref.abs().max().item()
-        err[f"{kern}.{name}"] = ((a32[key].double() - ref).abs().max().item() / scale) if scale > 0 else 0.0
+        err[f"{kern}.{name}"] = ((a32[key].double().ravel() - ref.ravel()).abs().max().item() / scale) \
+            if scale > 0 else 0.0
     del k64, a64, d64
     per, step_s, _ = time_sweep(torch, k32, steps, warmup)
     ab = algorithmic_bytes(C, F, Fi, B, 4)
@@ -327,7 +466,8 @@ def bench_f32(torch, host, f, steps, warmup, schedule):
     return {"workload": f"the same sweep in FP32 (float mesh, fields and arithmetic), StructGen "
                         f"{host.params['nx']}^3 relabelled", "gcell_sweeps_per_s": C / step_s / 1e9,
             "ms_per_step": step_s * 1e3, "gather_schedules": dict(zip(("gradient", "scalar", "uvw"), sched)),
-            "kernels": kern, "rel_err_vs_f64": err}
+            "kernels": kern, "rel_err_vs_f64": err,
+            "rel_err_reference": "FP64 oracle on the same inputs" if oracle is not None else "FP64 GPU sweep"}
 
 
 def time_sweep(torch, kernels, steps, warmup):
@@ -465,10 +605,11 @@ def bench_all_kernels(torch, n, steps, warmup, schedule):
             "repetition_us": sum(v["us"] for v in out.values()), "kernels": out}
 
 
-def bench_batched(torch, host, dev, K, steps, warmup, label):
+def bench_batched(torch, host, dev, K, steps, warmup, label, oracle_sample=None):
     """K scalars through gradient + scalar flux on ``host`` (FP64): one batched call of each against K
     single-scalar calls on K separate arrays. The scalars share the mass flux of one FluxMassDolfynFaceLoop
-    call. Returns per-scalar cell throughputs and their ratio."""
+    call. Returns per-scalar cell throughputs and their ratio. The FP32-storage errors are measured against
+    the FP64 oracle on the scalars of ``oracle_sample`` (default 0, K/2, K-1)."""
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
@@ -580,6 +721,43 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
                           sh["cpB"], sh["visEffB"], rface, g32, W.SCALAR, stream)
     torch.cuda.synchronize()
     err_s32 = {n: ((s32[n].double() - s64[n]).abs().max() / s64[n].abs().max()).item() for n in ("su", "au", "pB")}
+    # the same FP32-storage errors against the FP64 oracle on a sample of scalars (mesh numbering)
+    import pyoracle as po
+    sample = sorted(set(oracle_sample if oracle_sample is not None else (0, K // 2, K - 1)))
+    om = po.OracleMesh(host, 64)
+    mf_mesh = np.empty(F)
+    mf_mesh[fo] = mf.cpu().numpy()
+    sb_h, rb_h = sbar.cpu().numpy(), rbar.cpu().numpy()
+    rb_mesh = np.empty((F, 2))
+    rb_mesh[fo] = rb_h.reshape(F, 2)
+    g32h, su32h, au32h, pB32h = g32.cpu().numpy(), s32["su"].cpu().numpy(), s32["au"].cpu().numpy(), \
+        s32["pB"].cpu().numpy()
+    pbar32h = pbar32.cpu().numpy()
+    oerr = {"gradient": 0.0, "Su": 0.0, "Au": 0.0, "PhiB": 0.0, "PhiBar": 0.0}
+
+    def rel(a, b):
+        scale = float(np.abs(b).max())
+        return float(np.abs(np.asarray(a, np.float64) - b).max()) / scale if scale > 0 else 0.0
+
+    p = W.SCALAR
+    for k in sample:
+        gk, gok = np.zeros((C, 3)), np.zeros((C, 3))
+        po.GradientPhiGaussDolfyn(om, 1, phi[:, k].copy(), phiB[:, k].copy(), gk, gok)
+        oerr["gradient"] = max(oerr["gradient"], rel(g32h[:, :, k], gk[co]))
+        suk, auk, pbk = np.zeros(C), np.ones(C), phiB[:, k].copy()
+        po.FluxScalarDolfynFaceLoop(om, phi[:, k].copy(), pbk, f.visEff.copy(), auk, suk, f.den.copy(), mf_mesh.copy(),
+                                    f.te.copy(), f.cpB.copy(), f.visEffB.copy(), np.zeros(2 * F), gk, p["SolveTurb"],
+                                    p["SolveEnthalpy"], p["vislam"], p["ivar"], p["VarT"], p["Sigma_T"], p["Prandtl"],
+                                    p["VarTE"], p["Sigma_k"], p["VarED"], p["Sigma_e"], p["Sigma_s"], p["Schmidt"],
+                                    p["GammaBlend"], p["Small"], p["Large"], p["TMCmu"])
+        oerr["Su"] = max(oerr["Su"], rel(su32h[:, k], suk[co]))
+        oerr["Au"] = max(oerr["Au"], rel(au32h[:, k], auk[co]))
+        oerr["PhiB"] = max(oerr["PhiB"], rel(pB32h[:, k], pbk))
+        sb_mesh = np.empty(C)
+        sb_mesh[co] = sb_h[:, k]
+        _, pbar_k, _ = po.scalar_adjoint(om, phi[:, k].copy(), f.visEff.copy(), mf_mesh.copy(), sb_mesh,
+                                         rb_mesh.ravel().copy(), p)
+        oerr["PhiBar"] = max(oerr["PhiBar"], rel(pbar32h[:, k], pbar_k[co]))
     return {
         "workload": f"gradient + scalar flux of K={K} scalars, {label}, FP64, shared mass flux",
         "cells": C, "K": K,
@@ -592,6 +770,11 @@ def bench_batched(torch, host, dev, K, steps, warmup, label):
         "f32_storage_scalar_rel_err": {"Su": err_s32["su"], "Au": err_s32["au"], "PhiB": err_s32["pB"]},
         "batched_adjoint_ms": tadj * 1e3, "batched_forward_adjoint_ratio": (tb + tadj) / tb,
         "batched_adjoint_f32_storage_ms": tadj32 * 1e3, "f32_storage_adjoint_rel_err": err_adj32,
+        "f32_storage_rel_err_vs_oracle": {"scalars": sample, **oerr,
+                                          "note": "FP32-storage chain (gradient, scalar flux, adjoint) against the "
+                                                  "FP64 oracle on the same inputs, max over the sampled scalars; the "
+                                                  "_rel_err fields above are against the FP64 batched GPU results, "
+                                                  "which equal the oracle's bit for bit (tests)"},
     }
 
 
@@ -778,7 +961,8 @@ def bench_tets(torch, n, K, steps, warmup):
                        f"region", "cells": C, "faces": F, "boundaries": B, "mesh_setup_s": t_mesh,
            "mesh_build": mesh_build, "gcell_sweeps_per_s": C / step_s / 1e9, "ms_per_step": step_s * 1e3,
            "kernels": kernel_report(C, F, F - B, B, per)}
-    out["batched"] = bench_batched(torch, host, dev, K, steps=max(2, min(steps, 5)), warmup=1, label=label)
+    out["batched"] = bench_batched(torch, host, dev, K, steps=max(2, min(steps, 5)), warmup=1, label=label,
+                                   oracle_sample=(0, K - 1))
     torch.cuda.empty_cache()
     out["adjoint"] = bench_adjoint(torch, host, dev, steps=max(3, min(steps, 10)), warmup=1, label=label)
     torch.cuda.empty_cache()
@@ -899,7 +1083,7 @@ def main():
     Fi = int((host.a["faceBnd"] < 0).sum())
     f = W.Fields(host, seed=777)
 
-    kernels = make_sweep(torch, host, dev, f)
+    kernels, sweep_arrays = make_sweep(torch, host, dev, f, arrays=True)
     per, step_s, t_wall = time_sweep(torch, kernels, args.steps, args.warmup)
     K = args.steps
     kern = kernel_report(C, F, Fi, B, per)
@@ -940,11 +1124,22 @@ def main():
         "wall_s_timed_region": t_wall,
         "mesh_setup_s": t_mesh,
     }
+    out["residual_norms"] = bench_norms(torch, dev, [sweep_arrays[k] for k in ("Su", "su", "sv", "sw")],
+                                        steps=max(5, min(args.steps, 50)))
+    del kernels, sweep_arrays
+    oracle = None
+    if not args.no_cpu:
+        # the oracle's outputs on the same mesh and inputs, against one drop-in sweep on the GPU
+        oracle = oracle_outputs(host, f)
+        out["parity_vs_oracle"] = parity_vs_oracle(dev, f, oracle)
+        out["parity_vs_oracle"]["workload"] = "each kernel once on fresh copies of the sweep's inputs, same mesh"
     if not args.no_all_kernels:
         out["all_kernels"] = bench_all_kernels(torch, n, steps=max(5, min(args.steps, 20)), warmup=2,
                                                schedule=args.schedule)
     if not args.no_f32:
-        out["f32"] = bench_f32(torch, host, f, steps=max(5, min(args.steps, 20)), warmup=2, schedule=args.schedule)
+        out["f32"] = bench_f32(torch, host, f, steps=max(5, min(args.steps, 20)), warmup=2, schedule=args.schedule,
+                               oracle=oracle)
+    del oracle
     if not args.no_mesh_build:
         out["mesh_build"] = bench_mesh_build(torch, host, dev, t_gen, t_host_repack, f"StructGen {n}^3 relabelled")
     if not args.no_batched:
@@ -962,6 +1157,7 @@ def main():
         torch.cuda.empty_cache()
         out["tets"] = bench_tets(torch, args.tets_n, args.tets_k, steps=max(3, min(args.steps, 10)), warmup=1)
     if not args.no_cpu:
+        out["config0"] = bench_config0(torch)
         out["cpu_baseline"] = cpu_baseline(host, f, args.cpu_budget)
         out["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(n, args.cpu_budget / 2)
         out["cpu_baseline"]["host_cpu"] = host_cpu_model()

@@ -129,6 +129,8 @@ ADJOINT = ["cupcfd_dev_scalar_adjoint_f64", "cupcfd_dev_scalar_adjoint_batched_f
 # FP32-storage entry points on 64-bit meshes
 F32S = ["cupcfd_dev_gradient_batched_f32s", "cupcfd_dev_flux_scalar_batched_f32s",
         "cupcfd_dev_scalar_adjoint_batched_f32s"]
+# residual norms of cell fields
+NORMS = ["cupcfd_dev_residual_norms_f64", "cupcfd_dev_residual_norms_f32"]
 # mesh sets: persistent launches over several meshes
 MESHSET = ["cupcfd_hip_meshset_create", "cupcfd_hip_meshset_destroy", "cupcfd_dev_gradient_batched_set_f64",
            "cupcfd_dev_gradient_batched_set_f32"]
@@ -143,7 +145,7 @@ def all_symbols():
             out.append(f"cupcfd_dev_{k}_{s}")
         for k in BATCHED:
             out.append(f"cupcfd_dev_{k}_{s}")
-    out += ADJOINT + F32S + MESHSET
+    out += ADJOINT + F32S + MESHSET + NORMS
     return out
 
 
@@ -185,7 +187,7 @@ def _declare(L):
                 getattr(L, f"cupcfd_{mode}_{k}_{s}").restype = ct.c_int
         for k in BATCHED:
             getattr(L, f"cupcfd_dev_{k}_{s}").restype = ct.c_int
-    for k in ADJOINT + F32S + MESHSET:
+    for k in ADJOINT + F32S + MESHSET + NORMS:
         getattr(L, k).restype = ct.c_int
     L.cupcfd_hip_meshset_create.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_void_p)]
     L.cupcfd_hip_meshset_destroy.argtypes = [ct.c_void_p]

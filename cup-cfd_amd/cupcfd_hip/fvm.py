@@ -385,6 +385,26 @@ class resident:
         check("cupcfd_dev_scalar_adjoint_batched_f32s", lib().cupcfd_dev_scalar_adjoint_batched_f32s(
             mesh.handle, _I(K), *a[:5], *cls._adjoint_params(params), *a[5:], cls._s(stream)))
 
+    @classmethod
+    def residual_norms(cls, mesh, fields, norms=None, stream=None):
+        """max |x| and sqrt(sum x^2) of each cell field (1..8 tensors of nCells values, internal order; float64,
+        or float32 storage), accumulated in double. Returns the (nfields, 2) float64 device tensor it wrote."""
+        import torch
+        from . import lib, check
+        fields = list(fields)
+        dt = fields[0].dtype if fields else torch.float64
+        if dt not in (torch.float64, torch.float32):
+            from . import CupCfdError
+            raise CupCfdError("residual_norms: fields must be float64 or float32", 90)
+        ptrs = [cls._chk(t, mesh.nCells, dt, f"fields[{i}]") for i, t in enumerate(fields)]
+        if norms is None:
+            norms = torch.empty((len(fields), 2), dtype=torch.float64, device="cuda")
+        out = cls._chk(norms, 2 * len(fields), torch.float64, "norms")
+        fn = "cupcfd_dev_residual_norms_f64" if dt == torch.float64 else "cupcfd_dev_residual_norms_f32"
+        arr = (ct.c_void_p * max(1, len(ptrs)))(*[p.value for p in ptrs])
+        check(fn, getattr(lib(), fn)(mesh.handle, _I(len(ptrs)), arr, out, cls._s(stream)))
+        return norms
+
     # the boundary, region and viscosity loops; scalar outputs (flowin, flowout, ratearea, ...) are one-value
     # device tensors
     @classmethod

@@ -101,8 +101,12 @@ typedef struct cupcfd_host_mesh cupcfd_host_mesh;
  * (src/geometry/mesh/implementation/source/MeshStructGenSource.cpp:33-1259:
  * cell label x + y*nx + z*nx*ny, faces YZ then XZ then XY, boundary labels,
  * cell1 = low-side cell, cell2 = high-side cell or -1, lambda 0.5 / -1) as
- * assembled by UnstructuredMeshInterface::addData on one rank. Geometry is
- * evaluated in closed form for the axis-aligned grid.
+ * assembled by UnstructuredMeshInterface::addData on one rank. Geometry
+ * follows the reference's own computation bit for bit: face areas and
+ * centres from Heron triangles of the face's four vertices
+ * (Polygon3D.cpp:31-55), normals by the clockwise test from cell1's centre
+ * (MeshStructGenSource.cpp:887-944), cell volumes as five tetrahedra of the
+ * hexahedron (Hexahedron.ipp:30-68).
  *   regions = 0: one RTYPE_DEFAULT region, boundary distance 0 (what the
  *                reference builds); regions = 1: six typed regions, x-low
  *                INLET, x-high OUTLET, y-low/y-high WALL, z-low/z-high SYMP,
@@ -499,6 +503,20 @@ int cupcfd_dev_scalar_adjoint_noalloc_f64(cupcfd_hip_mesh *mesh, const double *P
     double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
     double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
     double *PhiBar, double *MassFluxBar, void *stream);
+
+/* Residual norms of cell fields, for a solver's convergence test after a
+ * sweep (no counterpart in the reference, whose kernels leave their residuals
+ * in Su / su / sv / sw for the caller). fields: a host array of nfields (1..8)
+ * device pointers, each to the mesh's nCells values in internal order
+ * (double for _f64; float for _f32, on a 32- or 64-bit mesh). norms: a device
+ * array of 2 * nfields doubles receiving, per field, max |x| and
+ * sqrt(sum x^2), accumulated in double in an order fixed by the launch shape
+ * (the same bits on every call). NaN propagates. Asynchronous on stream; the
+ * first call on a mesh sizes a workspace, later calls allocate nothing. */
+int cupcfd_dev_residual_norms_f64(cupcfd_hip_mesh *mesh, int32_t nfields, const double *const *fields,
+    double *norms, void *stream);
+int cupcfd_dev_residual_norms_f32(cupcfd_hip_mesh *mesh, int32_t nfields, const float *const *fields,
+    double *norms, void *stream);
 
 /* Library version string and the gfx target it was built for. */
 const char *cupcfd_hip_version(void);
