@@ -26,6 +26,8 @@ E_ARRAY_SIZE_MISMATCH = 25
 E_ERROR = 98
 E_NULL_PTR = 99
 E_NOT_IMPLEMENTED = 109
+E_FILE_MISSING = 10
+E_INVALID_FILE_FORMAT = 105
 
 RTYPE_DEFAULT, RTYPE_INLET, RTYPE_OUTLET, RTYPE_SYMP, RTYPE_WALL = range(5)
 MESH_REORDER = 1
@@ -109,7 +111,7 @@ KERNELS = [
 
 HOST_SYMBOLS = [
     "cupcfd_structgen_create", "cupcfd_host_mesh_view_f64", "cupcfd_host_mesh_view_f32",
-    "cupcfd_host_mesh_destroy", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
+    "cupcfd_host_mesh_destroy", "cupcfd_hdf5_mesh_create", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
@@ -151,6 +153,7 @@ def all_symbols():
 
 def _declare(L):
     L.cupcfd_structgen_create.argtypes = [ct.POINTER(StructGenParams), ct.POINTER(ct.c_void_p)]
+    L.cupcfd_hdf5_mesh_create.argtypes = [ct.c_char_p, ct.POINTER(ct.c_void_p)]
     L.cupcfd_host_mesh_view_f64.argtypes = [ct.c_void_p, ct.POINTER(MeshViewF64)]
     L.cupcfd_host_mesh_view_f32.argtypes = [ct.c_void_p, ct.POINTER(MeshViewF32)]
     L.cupcfd_host_mesh_destroy.argtypes = [ct.c_void_p]
@@ -272,6 +275,16 @@ class HostMesh:
         }
 
     STATE = ("bndYPlus", "bndUPlus", "bndQ", "bndH", "bndT", "bndShear", "regYLog", "regForceTangent")
+
+    @classmethod
+    def from_hdf5(cls, path):
+        """A mesh read from one of the reference's HDF5 mesh files (cupcfd_hdf5_mesh_create;
+        MeshHDF5Source, src/geometry/mesh/implementation/source/MeshHDF5Source.cpp)."""
+        h = ct.c_void_p()
+        check("cupcfd_hdf5_mesh_create", lib().cupcfd_hdf5_mesh_create(os.fsencode(path), ct.byref(h)))
+        m = cls(0, _handle=h)
+        m.params = dict(path=os.fspath(path))
+        return m
 
     def view32(self):
         if self._view32 is None:

@@ -2,7 +2,9 @@
 //
 // Reads a cupcfd.json in CUP-CFD's schema (README.md:100-160;
 // examples/benchmarks.json): "DataTypes".FloatDataType selects double or
-// float, "Mesh"."MeshSource"."MeshSourceStructGen" gives the grid, and every
+// float, "Mesh"."MeshSource" is either "MeshSourceStructGen" (the grid) or
+// "MeshSourceFile" {"FileFormat": "HDF5", "FilePath"} (MeshSourceFileConfigJSON.cpp:
+// 60-95; read by cupcfd_hdf5_mesh_create), and every
 // "Benchmarks"[i]."BenchmarkKernels" {"BenchmarkName", "Repetitions"} entry
 // runs the 16 fvm kernels per repetition in BenchmarkKernels::runBenchmark's
 // order (src/benchmarks/implementation/component/BenchmarkKernels.cpp:55-115)
@@ -180,9 +182,15 @@ template <class T, class S> void copy1(std::vector<T> &dst, const S *src, int n)
     for (int i = 0; i < n; i++) dst[i] = (T)src[i];
 }
 
-template <class T> int load_mesh(SoAMesh<T> &m, const cupcfd_structgen_params &p) {
+// where the mesh comes from: a StructGen grid, or an HDF5 mesh file
+struct MeshSpec {
+    cupcfd_structgen_params sg{};
+    std::string file;  // non-empty: read this HDF5 mesh file instead
+};
+
+template <class T> int load_mesh(SoAMesh<T> &m, const MeshSpec &spec) {
     cupcfd_host_mesh *h = nullptr;
-    int rc = cupcfd_structgen_create(&p, &h);
+    int rc = spec.file.empty() ? cupcfd_structgen_create(&spec.sg, &h) : cupcfd_hdf5_mesh_create(spec.file.c_str(), &h);
     if (rc != CUPCFD_E_SUCCESS) return rc;
     cupcfd_mesh_view_f64 v;
     cupcfd_host_mesh_view_f64(h, &v);
@@ -556,7 +564,7 @@ static double num(const Json *o, const char *k, double dflt) {
     return v && v->kind == Json::NUM ? v->num : (v && v->kind == Json::BOOL ? (v->b ? 1.0 : 0.0) : dflt);
 }
 
-template <class T> int run_all(const Json &cfg, const cupcfd_structgen_params &base) {
+template <class T> int run_all(const Json &cfg, const MeshSpec &base) {
     const Json *benches = cfg.get("Benchmarks");
     if (!benches || benches->kind != Json::ARR) {
         std::fprintf(stderr, "cupcfd.json: no Benchmarks array\n");
@@ -578,13 +586,17 @@ template <class T> int run_all(const Json &cfg, const cupcfd_structgen_params &b
             std::fprintf(stderr, "BenchmarkKernels needs BenchmarkName and Repetitions\n");
             return CUPCFD_E_CONFIG_INVALID_VALUE;
         }
-        cupcfd_structgen_params p = base;
+        MeshSpec spec = base;
+        cupcfd_structgen_params &p = spec.sg;
         p.seed = (uint64_t)num(bk, "Seed", (double)p.seed);
-        p.permute_cells = p.permute_faces = (int)num(bk, "Permute", 0);
+        p.permute_cells = p.permute_faces = (int)num(bk, "Permute", 0);  // StructGen grids only
         p.regions = (int)num(bk, "Typed", 0);
         SoAMesh<T> mesh;
-        int rc = load_mesh(mesh, p);
-        if (rc != CUPCFD_E_SUCCESS) return rc;
+        int rc = load_mesh(mesh, spec);
+        if (rc != CUPCFD_E_SUCCESS) {
+            std::fprintf(stderr, "mesh source returned %d\n", rc);
+            return rc;
+        }
         // "MeshBuild": "host" repacks the device mesh on the host, anything
         // else on the GPU; the repack runs here, timed on its own, so the
         // first kernel's time is the kernel's alone
@@ -672,11 +684,21 @@ int main(int argc, char **argv) {
     const Json *mesh = cfg.get("Mesh");
     const Json *src = mesh ? mesh->get("MeshSource") : nullptr;
     const Json *sg = src ? src->get("MeshSourceStructGen") : nullptr;
-    if (!sg) {
-        std::fprintf(stderr, "only Mesh.MeshSource.MeshSourceStructGen is supported\n");
+    const Json *sf = src ? src->get("MeshSourceFile") : nullptr;
+    MeshSpec spec;
+    if (sf) {
+        const Json *fmt = sf->get("FileFormat");
+        const Json *fp = sf->get("FilePath");
+        if (!fmt || fmt->kind != Json::STR || fmt->str != "HDF5" || !fp || fp->kind != Json::STR) {
+            std::fprintf(stderr, "MeshSourceFile needs FileFormat \"HDF5\" and a FilePath\n");
+            return 1;
+        }
+        spec.file = fp->str;
+    } else if (!sg) {
+        std::fprintf(stderr, "Mesh.MeshSource needs MeshSourceStructGen or MeshSourceFile\n");
         return 1;
     }
-    cupcfd_structgen_params p{};
+    cupcfd_structgen_params &p = spec.sg;
     p.nx = (int)num(sg, "CellX", 0);
     p.ny = (int)num(sg, "CellY", 0);
     p.nz = (int)num(sg, "CellZ", 0);
@@ -687,6 +709,6 @@ int main(int argc, char **argv) {
     p.zmin = num(sg, "SpatialZMin", 0);
     p.zmax = num(sg, "SpatialZMax", 1);
     p.seed = 1;
-    int rc = (fdt && fdt->str == "float") ? run_all<float>(cfg, p) : run_all<double>(cfg, p);
+    int rc = (fdt && fdt->str == "float") ? run_all<float>(cfg, spec) : run_all<double>(cfg, spec);
     return rc == CUPCFD_E_SUCCESS ? 0 : 2;
 }

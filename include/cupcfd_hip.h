@@ -42,12 +42,14 @@ extern "C" {
 #endif
 
 /* cupcfd::error::eCodes values used at this boundary (include/error/Error.h) */
+#define CUPCFD_E_FILE_MISSING 10
 #define CUPCFD_E_ARRAY_SIZE_MISMATCH 25
 #define CUPCFD_E_INVALID_INDEX 68
 #define CUPCFD_E_CONFIG_INVALID_VALUE 90
 #define CUPCFD_E_SUCCESS 95
 #define CUPCFD_E_ERROR 98
 #define CUPCFD_E_NULL_PTR 99
+#define CUPCFD_E_INVALID_FILE_FORMAT 105
 #define CUPCFD_E_NOT_IMPLEMENTED 109
 
 /* cupcfd::geometry::mesh::RType (UnstructuredMeshInterface.h:58-65) */
@@ -131,6 +133,21 @@ typedef struct cupcfd_structgen_params {
 } cupcfd_structgen_params;
 
 int cupcfd_structgen_create(const cupcfd_structgen_params *params, cupcfd_host_mesh **out);
+/*
+ * A mesh read from one of the reference's HDF5 mesh files (MeshHDF5Source,
+ * src/geometry/mesh/implementation/source/MeshHDF5Source.cpp: 1-based labels,
+ * /cell, /face, /bnd groups), assembled as UnstructuredMeshInterface::addData
+ * assembles it on one rank: cells in label order, the faces the cells list in
+ * label order, their boundaries in label order, one RTYPE_DEFAULT region per
+ * region label, xpac = xnac = rlencos = 0, real data converted to double.
+ * libhdf5 is loaded at run time (CUPCFD_HDF5_LIBRARY, else libhdf5.so[.103]
+ * on the loader path, else /opt/conda/lib/libhdf5.so.103); without it this
+ * returns CUPCFD_E_NOT_IMPLEMENTED. A path that does not exist gives
+ * CUPCFD_E_FILE_MISSING; a file libhdf5 cannot open, or missing or
+ * mis-shaped datasets, give CUPCFD_E_INVALID_FILE_FORMAT; labels out of
+ * range give CUPCFD_E_INVALID_INDEX.
+ */
+int cupcfd_hdf5_mesh_create(const char *path, cupcfd_host_mesh **out);
 int cupcfd_host_mesh_view_f64(cupcfd_host_mesh *mesh, cupcfd_mesh_view_f64 *out);
 /* float copies of every array, owned by the host mesh */
 int cupcfd_host_mesh_view_f32(cupcfd_host_mesh *mesh, cupcfd_mesh_view_f32 *out);

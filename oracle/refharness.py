@@ -120,13 +120,19 @@ def structgen(nx, ny, nz, bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), precision=64):
             "s.nx": np.array([nx], np.int32), "s.ny": np.array([ny], np.int32), "s.nz": np.array([nz], np.int32)}
     for k, v in zip(("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"), bounds):
         recs["s." + k] = np.array([float(v)])
-    parts = run_case(recs)
+    return _merge(run_case(recs), precision, base=0)
+
+
+def _merge(parts, precision, base):
     for r in parts:
         rc = int(r["rc"][0])
         if rc != 95:
-            raise RuntimeError(f"reference StructGen build returned eCodes {rc}")
+            raise RuntimeError(f"reference mesh build returned eCodes {rc}")
     dt = np.float64 if precision == 64 else np.float32
-    C = nx * ny * nz
+    for r in parts:  # build labels -> 0-based ids
+        for k in ("label.cell", "label.face", "label.bnd"):
+            r[k] = r[k] - base
+    C = max(int(r["label.cell"].max()) + 1 for r in parts)
     F = max(int(r["label.face"].max()) + 1 for r in parts)
     B = max(int(r["label.bnd"].max()) + 1 if r["label.bnd"].size else 0 for r in parts)
     out = {"faceCell1": np.full(F, -2, np.int32), "faceCell2": np.full(F, -2, np.int32),

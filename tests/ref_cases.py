@@ -16,6 +16,8 @@ face, so the goldens leave the scalar and UVW cases out there.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 import cupcfd_hip as ch
@@ -34,6 +36,8 @@ MESHES = {
     # jittered, typed, relabelled tetrahedra (2x2x2 hexes x 6)
     "tets222": (dict(nx=2, ny=2, nz=2, tets=True, jitter=0.2, typed=True, permute_cells=True, permute_faces=True,
                      face_extras=True, seed=5), True),
+    # the reference's HDF5 test mesh (tests/geometry/mesh/data/MeshHDF5.hdf5) read by cupcfd_hdf5_mesh_create
+    "hdf5": (dict(path=os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "MeshHDF5.hdf5")), False),
 }
 
 # kernels whose outputs read no uninitialised carried value on a DEFAULT-only mesh
@@ -73,7 +77,7 @@ def outlet_first(host):
 
 def golden_mesh(name):
     params, first = MESHES[name]
-    host = ch.HostMesh(**params)
+    host = ch.HostMesh.from_hdf5(params["path"]) if "path" in params else ch.HostMesh(**params)
     return outlet_first(host) if first else ch.ArrayMesh(host.a)
 
 

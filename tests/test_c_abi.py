@@ -25,7 +25,7 @@ def declared_symbols():
 def test_header_symbols_exported():
     L = ch.lib()
     declared = declared_symbols()
-    assert len(declared) == len(ch.all_symbols()) == 28 + 2 * 2 * 16 + 2 * len(ch.BATCHED) + len(ch.ADJOINT) + len(ch.F32S) + len(ch.MESHSET) + len(ch.NORMS)
+    assert len(declared) == len(ch.all_symbols()) == 29 + 2 * 2 * 16 + 2 * len(ch.BATCHED) + len(ch.ADJOINT) + len(ch.F32S) + len(ch.MESHSET) + len(ch.NORMS)
     assert declared == set(ch.all_symbols())
     for name in declared:
         assert hasattr(L, name), name
@@ -35,8 +35,9 @@ def test_ecodes_match_reference_enum():
     # cupcfd::error::eCodes ordinals (include/error/Error.h, 0-based enum)
     text = open(HEADER).read()
     vals = dict(re.findall(r"#define CUPCFD_(E_\w+) (\d+)", text))
-    assert vals == {"E_ARRAY_SIZE_MISMATCH": "25", "E_INVALID_INDEX": "68", "E_CONFIG_INVALID_VALUE": "90",
-                    "E_SUCCESS": "95", "E_ERROR": "98", "E_NULL_PTR": "99", "E_NOT_IMPLEMENTED": "109"}
+    assert vals == {"E_FILE_MISSING": "10", "E_ARRAY_SIZE_MISMATCH": "25", "E_INVALID_INDEX": "68",
+                    "E_CONFIG_INVALID_VALUE": "90", "E_SUCCESS": "95", "E_ERROR": "98", "E_NULL_PTR": "99",
+                    "E_INVALID_FILE_FORMAT": "105", "E_NOT_IMPLEMENTED": "109"}
 
 
 def test_version():

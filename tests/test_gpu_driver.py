@@ -92,3 +92,28 @@ def test_driver_mesh_build_host_and_device(tmp_path, precision):
         assert h["mesh_setup_s"] > 0 and d["mesh_setup_s"] > 0
     # the plain and the typed relabelled meshes differ
     assert reps["device"][0]["mesh_digest"] != reps["device"][1]["mesh_digest"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_driver_reads_hdf5_mesh_file(tmp_path, precision):
+    # "MeshSourceFile" {"FileFormat": "HDF5", "FilePath"} in place of the StructGen grid
+    cfg = json.load(open(CFG))
+    cfg["DataTypes"]["FloatDataType"] = precision
+    cfg["Mesh"]["MeshSource"] = {"MeshSourceFile": {"FileFormat": "HDF5",
+                                                    "FilePath": os.path.join(ROOT, "tests", "golden",
+                                                                             "MeshHDF5.hdf5")}}
+    cfg["Benchmarks"] = [b for b in cfg["Benchmarks"] if "BenchmarkKernels" in b][:1]
+    p = tmp_path / "cupcfd.json"
+    p.write_text(json.dumps(cfg))
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    (b,) = json.loads(r.stdout)["benchmarks"]
+    assert (b["LocalCells"], b["LocalFaces"], b["LocalBounds"]) == (4, 20, 17)
+    assert b["failures"] == 0
+    assert sorted(b["kernels"]) == sorted(KERNELS)
+    # a path that does not exist: E_FILE_MISSING (10) reported, non-zero exit
+    cfg["Mesh"]["MeshSource"]["MeshSourceFile"]["FilePath"] = str(tmp_path / "absent.hdf5")
+    p.write_text(json.dumps(cfg))
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "returned 10" in r.stderr
