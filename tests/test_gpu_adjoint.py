@@ -196,8 +196,9 @@ def test_march_checkpointed_reverse():
 @pytest.mark.parametrize("K", [1, 3, 16, 40, 64, 70])
 @pytest.mark.parametrize("name", ["ref5", "perm12", "tets"])
 def test_batched_adjoint_matches_single(name, K):
-    """PhiBar_k equals the single-scalar adjoint's bit for bit; MassFluxBar is the RFace term followed by the
-    K scalars' terms added in order (each term taken from a single-scalar call with a zero RFace cotangent)."""
+    """PhiBar_k equals the single-scalar adjoint's and the oracle adjoint's bit for bit; MassFluxBar is the
+    RFace term followed by the K scalars' terms added in order (each term taken from a single-scalar call, and
+    from an oracle call, with a zero RFace cotangent)."""
     c = case(name)
     rng = np.random.default_rng(40 + K)
     phi = rng.uniform(-1, 1, (c.C, K))
@@ -219,6 +220,17 @@ def test_batched_adjoint_matches_single(name, K):
         assert torch.equal(pb[:, k].contiguous(), p1), k
         want = want + m1.cpu().numpy()
     assert np.array_equal(mb.cpu().numpy(), want)
+    # every scalar against the oracle's adjoint directly (mesh numbering), MassFluxBar folded the same way
+    m = po.OracleMesh(c.host, 64)
+    pb_h, mb_h = pb.cpu().numpy(), mb.cpu().numpy()
+    _, _, mb_o = po.scalar_adjoint(m, phi[:, 0].copy(), c.f.visEff, c.f.massFlux, np.zeros(c.C), rb, P)
+    for k in range(K):
+        rc, pk, mk = po.scalar_adjoint(m, phi[:, k].copy(), c.f.visEff, c.f.massFlux, sb[:, k].copy(),
+                                       np.zeros(2 * c.F), P)
+        assert rc == 95
+        assert np.array_equal(pb_h[:, k].view(np.uint64), pk[c.co].view(np.uint64)), k
+        mb_o = mb_o + mk
+    assert np.array_equal(mb_h.view(np.uint64), mb_o[c.fo].view(np.uint64))
 
 
 @pytest.mark.gpu

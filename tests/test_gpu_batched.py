@@ -237,3 +237,82 @@ def test_gradient_batched_meshset_matches_per_mesh(K, precision):
         torch.cuda.synchronize()
         assert _same(g[i], g1) and _same(go[i], go1), i
     del mset
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("meshname", ["typed_perm", "tets_typed", "tets_plain"])
+@pytest.mark.parametrize("variant", ["scalar", "scalar_wall"])
+def test_batched_every_scalar_matches_oracle(meshname, precision, variant):
+    """Every scalar k of the batched gradient (nGradient 1 and 3) and batched scalar flux against a direct
+    oracle call on scalar k's arrays (mesh numbering, mesh state reset to the same seed each call).
+    Bit-identical, except the WALL-enthalpy variant's log-based wall function (1e-12 / 2e-6)."""
+    import pyoracle as po
+    from kernel_runs import compare
+
+    K = 5
+    host, dev = _mesh(meshname, precision)
+    dt = np.float64 if precision == 64 else np.float32
+    co, fo = dev.cell_order, dev.face_order
+    C, F, B = host.nCells, host.nFaces, host.nBoundaries
+    rng = np.random.default_rng(5000 + precision)
+    per = dict(phi=rng.uniform(-1, 1, (C, K)), phiB=rng.uniform(-1, 1, (B, K)), au=rng.uniform(0.5, 1.5, (C, K)),
+               su=rng.uniform(-1, 1, (C, K)), dphidx=rng.uniform(-1, 1, (C, 3, K)))
+    per = {k: v.astype(dt) for k, v in per.items()}
+    f = W.Fields(host, seed=77).cast(dt)
+    p = dict(W.SCALAR)
+    if variant == "scalar_wall":
+        p["ivar"] = p["VarT"]
+        p["SolveEnthalpy"] = 1
+    st0 = {k: v.astype(dt) for k, v in seed_state(host).items()}
+    om = po.OracleMesh(host, precision)
+    tol = 1e-12 if precision == 64 else 2e-6
+
+    def check(got, want, what):
+        same, rel = compare(got, want)
+        if variant == "scalar":
+            assert np.array_equal(np.ascontiguousarray(got).view(np.uint8),
+                                  np.ascontiguousarray(want).view(np.uint8)), f"{what} (max rel {rel:.3e})"
+        assert rel <= tol, f"{what}: max rel {rel:.3e}"
+
+    # batched gradient against the oracle, every scalar
+    for nG in (1, 3):
+        g = torch.zeros((C, 3, K), dtype=torch.float64 if precision == 64 else torch.float32, device="cuda")
+        go = torch.zeros_like(g)
+        R.gradient_batched(dev, nG, K, _dev(per["phi"][co], dt), _dev(per["phiB"], dt), g, go)
+        g_h, go_h = g.cpu().numpy(), go.cpu().numpy()
+        for k in range(K):
+            g1, go1 = np.zeros((C, 3), dt), np.zeros((C, 3), dt)
+            assert po.GradientPhiGaussDolfyn(om, nG, per["phi"][:, k].copy(), per["phiB"][:, k].copy(), g1,
+                                             go1) == ch.E_SUCCESS
+            assert np.array_equal(g_h[:, :, k], g1[co]), (nG, k)
+            assert np.array_equal(go_h[:, :, k], go1[co]), (nG, k)
+    # batched scalar flux against the oracle, every scalar
+    host.set_state(st0, precision)
+    dev.push_state()
+    bt = dict(phi=_dev(per["phi"][co], dt), phiB=_dev(per["phiB"], dt), au=_dev(per["au"][co], dt),
+              su=_dev(per["su"][co], dt), dphidx=_dev(per["dphidx"][co], dt))
+    sh = dict(visEff=_dev(f.visEff[co], dt), den=_dev(f.den[co], dt), mf=_dev(f.massFlux[fo], dt),
+              te=_dev(f.te[co], dt), cpB=_dev(f.cpB, dt), visEffB=_dev(f.visEffB, dt))
+    rface = torch.zeros(2 * F, dtype=bt["phi"].dtype, device="cuda")
+    R.scalar_batched(dev, K, bt["phi"], bt["phiB"], sh["visEff"], bt["au"], bt["su"], sh["den"], sh["mf"], sh["te"],
+                     sh["cpB"], sh["visEffB"], rface, bt["dphidx"], p)
+    torch.cuda.synchronize()
+    su_h, au_h, pb_h = bt["su"].cpu().numpy(), bt["au"].cpu().numpy(), bt["phiB"].cpu().numpy()
+    rface_h = rface.cpu().numpy().reshape(F, 2)
+    for k in range(K):
+        om.set_state(st0)
+        s1, a1, pb1 = per["su"][:, k].copy(), per["au"][:, k].copy(), per["phiB"][:, k].copy()
+        rf1 = np.zeros(2 * F, dt)
+        rc = po.FluxScalarDolfynFaceLoop(om, per["phi"][:, k].copy(), pb1, f.visEff.copy(), a1, s1, f.den.copy(),
+                                         f.massFlux.copy(), f.te.copy(), f.cpB.copy(), f.visEffB.copy(), rf1,
+                                         np.ascontiguousarray(per["dphidx"][:, :, k]), p["SolveTurb"],
+                                         p["SolveEnthalpy"], p["vislam"], p["ivar"], p["VarT"], p["Sigma_T"],
+                                         p["Prandtl"], p["VarTE"], p["Sigma_k"], p["VarED"], p["Sigma_e"],
+                                         p["Sigma_s"], p["Schmidt"], p["GammaBlend"], p["Small"], p["Large"],
+                                         p["TMCmu"])
+        assert rc == ch.E_SUCCESS
+        check(su_h[:, k], s1[co], f"Su scalar {k}")
+        check(au_h[:, k], a1[co], f"Au scalar {k}")
+        check(pb_h[:, k], pb1, f"PhiB scalar {k}")
+        check(rface_h, rf1.reshape(F, 2)[fo], f"RFace scalar {k}")

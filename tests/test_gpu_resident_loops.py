@@ -152,7 +152,7 @@ _big = {}
 def test_ordered_sums_over_several_chunks(kernel, precision):
     """The boundary loops' ordered sums against the oracle on a typed 64^3 mesh, where every side has 4096
     boundaries: the flow sums run over two LDS chunks of 3072 values, and each WALL region's force tangent over
-    four chunks of 1024. FP64 bit for bit; FP32 within the parity tolerance of tests/test_gpu_parity.py."""
+    four chunks of 1024. Bit for bit in FP64 and FP32."""
     import pyoracle as po
 
     if precision not in _big:
@@ -170,6 +170,4 @@ def test_ordered_sums_over_several_chunks(kernel, precision):
     assert rc_o == ch.E_SUCCESS and rc_g == ch.E_SUCCESS
     for k in want:
         same, rel = compare(got[k], want[k])
-        if precision == 64:
-            assert same, f"{kernel}: {k} not bit-identical (max rel {rel:.3e})"
-        assert rel <= 2e-6, f"{kernel}/{precision}: {k} max rel {rel:.3e}"
+        assert same, f"{kernel}/{precision}: {k} not bit-identical (max rel {rel:.3e})"
