@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcupcfd_hip.so")
+# CUPCFD_HIP_LIBRARY names another build of it (the sanitizer build of cup-cfd_amd/Makefile)
+LIB_PATH = os.environ.get("CUPCFD_HIP_LIBRARY") or os.path.join(os.path.dirname(_HERE), "libcupcfd_hip.so")
 
 E_SUCCESS = 95
 E_INVALID_INDEX = 68

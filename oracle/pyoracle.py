@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle_fvm.so")
+# CUPCFD_ORACLE_LIBRARY names another build of it (the sanitizer build of oracle/Makefile)
+LIB = os.environ.get("CUPCFD_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle_fvm.so")
 
 _lib = None
 
