@@ -116,6 +116,8 @@ OR_DECL(float, f32)
 typedef struct {
     int SolveTurb, ivar, VarT, VarTE, VarED;
     double vislam, Sigma_T, Prandtl, Sigma_k, Sigma_e, Sigma_s, Schmidt, GammaBlend;
+    int SolveEnthalpy;
+    double Small, TMCmu;
 } or_scalar_params;
 
 /* Tangent (sudot, rfacedot) and adjoint (phibar, mfbar) of GradientPhiGaussDolfyn
@@ -128,6 +130,21 @@ int or_scalar_tangent_f64(or_mesh_f64 *m, const double *phi, const double *VisEf
 int or_scalar_adjoint_f64(or_mesh_f64 *m, const double *phi, const double *VisEff, const double *MassFlux,
                           const double *subar, const double *rbar, const or_scalar_params *p, double *phibar,
                           double *mfbar);
+
+/* The same chain with nGradient >= 1 gradient sweeps, on any mesh: boundary
+ * faces, the PhiFace carried from INLET / OUTLET / WALL faces and the
+ * enthalpy wall included (oracle_adjoint.inc). dPhidx is the forward
+ * gradient the scalar loop read; TE, Den, CpB and visEffB its other inputs.
+ * Cotangents are on Su and RFace (PhiBoundary, Au and the mesh state are
+ * not differentiated). */
+int or_scalar_tangent_full_f64(or_mesh_f64 *m, int nGradient, const double *phi, const double *VisEff,
+                               const double *MassFlux, const double *dPhidx, const double *TE, const double *Den,
+                               const double *CpB, const double *visEffB, const double *phidot,
+                               const double *mfdot, const or_scalar_params *p, double *sudot, double *rfacedot);
+int or_scalar_adjoint_full_f64(or_mesh_f64 *m, int nGradient, const double *phi, const double *VisEff,
+                               const double *MassFlux, const double *dPhidx, const double *TE, const double *Den,
+                               const double *CpB, const double *visEffB, const double *subar,
+                               const double *rbar, const or_scalar_params *p, double *phibar, double *mfbar);
 
 #ifdef __cplusplus
 }

@@ -233,7 +233,8 @@ class ScalarParams(ct.Structure):
     _fields_ = [("SolveTurb", ct.c_int), ("ivar", ct.c_int), ("VarT", ct.c_int), ("VarTE", ct.c_int),
                 ("VarED", ct.c_int), ("vislam", ct.c_double), ("Sigma_T", ct.c_double), ("Prandtl", ct.c_double),
                 ("Sigma_k", ct.c_double), ("Sigma_e", ct.c_double), ("Sigma_s", ct.c_double),
-                ("Schmidt", ct.c_double), ("GammaBlend", ct.c_double)]
+                ("Schmidt", ct.c_double), ("GammaBlend", ct.c_double), ("SolveEnthalpy", ct.c_int),
+                ("Small", ct.c_double), ("TMCmu", ct.c_double)]
 
     @classmethod
     def of(cls, p):
@@ -259,4 +260,32 @@ def scalar_adjoint(m, phi, visEff, massFlux, subar, rbar, params):
     P = ScalarParams.of(params)
     rc = _fn(m, "scalar_adjoint")(m.ref, _P(phi), _P(visEff), _P(massFlux), _P(subar), _P(rbar), ct.byref(P),
                                   _P(pb), _P(mb))
+    return rc, pb, mb
+
+
+def scalar_tangent_full(m, nGradient, phi, visEff, massFlux, dPhidx, TE, Den, CpB, visEffB, phidot, mfdot, params):
+    """(rc, sudot, rfacedot): tangent of gradient (nGradient sweeps) + scalar face loop w.r.t. (phi, massFlux),
+    on any mesh; dPhidx is the forward gradient the scalar loop read."""
+    assert m.precision == 64
+    sd = np.zeros(m.nCells)
+    rd = np.zeros(2 * m.nFaces)
+    P = ScalarParams.of(params)
+    keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (phi, visEff, massFlux, dPhidx, TE, Den, CpB,
+                                                                visEffB, phidot, mfdot)]
+    rc = _fn(m, "scalar_tangent_full")(m.ref, ct.c_int(nGradient), *[_P(x) for x in keep], ct.byref(P), _P(sd),
+                                       _P(rd))
+    return rc, sd, rd
+
+
+def scalar_adjoint_full(m, nGradient, phi, visEff, massFlux, dPhidx, TE, Den, CpB, visEffB, subar, rbar, params):
+    """(rc, phibar, mfbar): adjoint of gradient (nGradient sweeps) + scalar face loop w.r.t. (phi, massFlux),
+    on any mesh; dPhidx is the forward gradient the scalar loop read."""
+    assert m.precision == 64
+    pb = np.zeros(m.nCells)
+    mb = np.zeros(m.nFaces)
+    P = ScalarParams.of(params)
+    keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (phi, visEff, massFlux, dPhidx, TE, Den, CpB,
+                                                                visEffB, subar, rbar)]
+    rc = _fn(m, "scalar_adjoint_full")(m.ref, ct.c_int(nGradient), *[_P(x) for x in keep], ct.byref(P), _P(pb),
+                                       _P(mb))
     return rc, pb, mb
