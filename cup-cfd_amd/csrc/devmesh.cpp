@@ -15,8 +15,11 @@ template <class T> Mesh<T>::~Mesh() {
     for (void *p : allocs) (void)hipFree(p);
     for (void *p : ws) (void)hipFree(p);
     for (void *p : ws_retired) (void)hipFree(p);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 4; k++) {
         if (sc_src[k]) (void)hipFree(sc_src[k]);
+        if (sc_rd_off[k]) (void)hipFree(sc_rd_off[k]);
+        if (sc_rd[k]) (void)hipFree(sc_rd[k]);
+    }
     if (rbnd) (void)hipFree(rbnd);
     if (rbnd_off) (void)hipFree(rbnd_off);
     if (cell_order_d) (void)hipFree(cell_order_d);
@@ -498,6 +501,67 @@ template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wa
 
 template int *scalar_carry_src<double>(Mesh<double> &, bool, bool);
 template int *scalar_carry_src<float>(Mesh<float> &, bool, bool);
+
+template <class T>
+int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd) {
+    const int key = (inlet_assigns ? 1 : 0) | (wall_assigns ? 2 : 0);
+    if (!M.sc_rd_off[key]) {
+        const int F = M.d.F, Fi = M.d.Fi, nb = F - Fi;
+        // the sources in mesh face order, as scalar_carry_src walks them
+        std::vector<int> src(F, -1);
+        int last = -1;
+        for (int f = 0; f < F; f++) {
+            const int k = M.face_rank[f];
+            const int nb_ = M.fconn_h[2 * k + 1];
+            if (nb_ >= 0) {
+                src[k] = last;
+                continue;
+            }
+            const int b = ~nb_;
+            const int t = M.btype_h[b];
+            if (t == CUPCFD_RTYPE_INLET) {
+                if (!inlet_assigns) src[k] = last;  // it reads the carried value
+                else last = k;
+            } else if (t == CUPCFD_RTYPE_OUTLET) {
+                last = k;
+            } else if (t == CUPCFD_RTYPE_WALL) {
+                if (wall_assigns && !M.radiab_h[M.bregion_h[b]]) last = k;
+            }
+        }
+        std::vector<int> cnt(nb + 1, 0), list;
+        for (int k = 0; k < F; k++)
+            if (src[k] >= 0) cnt[src[k] - Fi + 1]++;
+        for (int k = 0; k < nb; k++) cnt[k + 1] += cnt[k];
+        list.assign(cnt[nb], 0);
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int f = 0; f < F; f++) {  // ascending mesh face id within each source
+            const int k = M.face_rank[f];
+            if (src[k] >= 0) list[fill[src[k] - Fi]++] = k;
+        }
+        void *po = nullptr, *pr = nullptr;
+        if (hipMalloc(&po, (nb + 1) * sizeof(int)) != hipSuccess) return CUPCFD_E_ERROR;
+        if (hipMalloc(&pr, (list.empty() ? 1 : list.size()) * sizeof(int)) != hipSuccess) {
+            (void)hipFree(po);
+            return CUPCFD_E_ERROR;
+        }
+        bool ok = hipMemcpy(po, cnt.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+        if (ok && !list.empty())
+            ok = hipMemcpy(pr, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) {
+            (void)hipFree(po);
+            (void)hipFree(pr);
+            return CUPCFD_E_ERROR;
+        }
+        M.sc_rd_off[key] = (int *)po;
+        M.sc_rd[key] = (int *)pr;
+    }
+    *off = M.sc_rd_off[key];
+    *rd = M.sc_rd[key];
+    return CUPCFD_E_SUCCESS;
+}
+
+template int scalar_carry_readers<double>(Mesh<double> &, bool, bool, const int **, const int **);
+template int scalar_carry_readers<float>(Mesh<float> &, bool, bool, const int **, const int **);
 
 // A stable counting sort of the boundary ids by region: each region's
 // boundaries in ascending id, the order the reference's boundary loops add

@@ -116,6 +116,11 @@ template <class T> struct Mesh {
     int schedule_k[3] = {-1, -1, -1};
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    // per carry source (boundary face k = g - Fi): the faces that read the
+    // PhiFace it leaves, ascending mesh face id (sc_rd[key][sc_rd_off[key][k]
+    // .. sc_rd_off[key][k + 1])), built on first use by the adjoint
+    int *sc_rd_off[4] = {nullptr, nullptr, nullptr, nullptr};
+    int *sc_rd[4] = {nullptr, nullptr, nullptr, nullptr};
     // boundaries grouped by region, ascending id within each region (region r:
     // rbnd[rbnd_off[r] .. rbnd_off[r + 1])), built on first use by the
     // per-region ordered sums
@@ -200,6 +205,12 @@ struct DeviceChunks {
 
 // Build M's face colouring on the device from its CSR (no-op once built).
 template <class T> int build_face_colours(Mesh<T> &M);
+
+// The readers of each scalar carry source (Mesh::sc_rd_off / sc_rd) for one
+// (INLET assigns, WALL assigns) combination: interior faces, and INLET faces
+// when INLET does not assign; built once per mesh and combination.
+template <class T>
+int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd);
 
 // The same repack as the host build, run on the device: every array of the
 // DevMesh and every host-side mirror of Mesh comes out bit-identical. With

@@ -2353,14 +2353,34 @@ template <class T> struct AdjArgs {
     ScalarArgs<T> s;  // phi, visEff, massFlux and the diffusivity parameters
     const T *subar, *rbar;
     T *h, *dir, *phibar, *mfbar;
+    // typed meshes: each boundary face's pass-A terms (gradient cotangent x3,
+    // phi term) at bterm[4 (g - Fi)] (k_adj_bnd); null on all-DEFAULT meshes
+    const T *bterm;
+    // pass B: each cell's starting value (dir, or PhiBar of the sweep after),
+    // and for a sweep before the last, the h it hands to that sweep
+    const T *binit;
+    T *hnext;
 };
+
+// PhiFace as the scalar loop carries it into face f (the carry-init value on
+// all-DEFAULT meshes)
+template <class T> __device__ __forceinline__ T adj_phiface(const DevMesh<T> &m, const AdjArgs<T> &a, int f) {
+    if (!a.s.cval) return m.carry_init;
+    const int g = a.s.src[f];
+    return g < 0 ? m.carry_init : a.s.cval[g - m.Fi];
+}
 
 // Pass-A terms of CSR entry en of cell c: t[0..2] = (D*fac)*w, t[3] = the direct phi term
 template <class T>
 __device__ __forceinline__ void adj_a_terms(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en, T t[4]) {
     if (en.y < 0) {
-        t[0] = t[1] = t[2] = t[3] = nzero<T>();
-        a.mfbar[en.x >> 1] = 0.0;  // a boundary face has no mass-flux cotangent here
+        if (a.bterm) {  // evaluated ahead by k_adj_bnd, which also wrote the face's MassFluxBar
+            const V4<T> bt = *(const V4<T> *)(a.bterm + 4 * (size_t)((en.x >> 1) - m.Fi));
+            t[0] = bt.x, t[1] = bt.y, t[2] = bt.z, t[3] = bt.w;
+        } else {
+            t[0] = t[1] = t[2] = t[3] = nzero<T>();
+            a.mfbar[en.x >> 1] = 0.0;  // a DEFAULT boundary face has no mass-flux cotangent
+        }
         return;
     }
     const int f = en.x >> 1;
@@ -2385,15 +2405,21 @@ __device__ __forceinline__ void adj_a_terms(const DevMesh<T> &m, const AdjArgs<T
     if (!in) {  // the face's cell1 entry (one per face) also writes its mass-flux cotangent
         const T dfci = mf < 0 ? a.s.phi[nn] : (mf > 0 ? a.s.phi[ip] : (T)0.0);
         const T rterm = mf < 0 ? a.rbar[2 * (size_t)f + 1] : (mf > 0 ? -a.rbar[2 * (size_t)f] : (T)0.0);
-        const T PhiFace = m.carry_init;
+        const T PhiFace = adj_phiface(m, a, f);
         a.mfbar[f] = -(Dg * (PhiFace - dfci)) + rterm;
     }
 }
 
-// Pass-B term of CSR entry en of cell c
-template <class T>
-__device__ __forceinline__ T adj_b_term(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en) {
-    if (en.y < 0) return nzero<T>();
+// Pass-B term t = fac*dot(h_P + h_N, S) of CSR entry en of cell c, and with
+// CORR the previous sweep's gradient cotangent t*corr_f, corr_f = Xf -
+// (Cc[N]*facn + Cc[P]*facp) (GradientKernels.ipp:88-96)
+template <class T, bool CORR = false>
+__device__ __forceinline__ T adj_b_term(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int2 en,
+                                        v3<T> *g = nullptr) {
+    if (en.y < 0) {
+        if (CORR) *g = v3<T>{nzero<T>(), nzero<T>(), nzero<T>()};
+        return nzero<T>();
+    }
     const int f = en.x >> 1;
     const bool in = (en.x & 1) != 0;
     const int ip = in ? en.y : c, nn = in ? c : en.y;
@@ -2404,7 +2430,13 @@ __device__ __forceinline__ T adj_b_term(const DevMesh<T> &m, const AdjArgs<T> &a
     const v3<T> hp = ld3(a.h, ip), hn = ld3(a.h, nn);
     const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
     const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
-    return fac * pF;
+    const T t = fac * pF;
+    if (CORR) {
+        const v3<T> xac = lerp3(xyz(m.cgeo[nn]), facn, xyz(m.cgeo[ip]), facp);
+        const v3<T> corr = sub3(xyz(m.fgeoX[f]), xac);
+        *g = v3<T>{t * corr.x, t * corr.y, t * corr.z};
+    }
+    return t;
 }
 
 template <class T> __device__ __forceinline__ void adj_a_store(const DevMesh<T> &m, const AdjArgs<T> &a, int c, int k,
@@ -2458,35 +2490,133 @@ __global__ __launch_bounds__(LPC * 32) void k_adj_a_vec(DevMesh<T> m, AdjArgs<T>
     }
 }
 
-template <class T>
+// Pass B: PhiBar_c = binit_c + sum of the row's terms; with CORR also
+// hnext_c = (sum of the row's t*corr_f) * (1/vol_c)
+template <class T, bool CORR>
 __global__ __launch_bounds__(kBlock) void k_adj_b(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
     const int lb = xcd_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int c = lb * kBlock + threadIdx.x;
     if (c >= m.C) return;
-    T acc = a.dir[c];
+    T acc = a.binit[c];
+    T gx = 0, gy = 0, gz = 0;
     int e0, n;
     row_range(m, c, e0, n);
-    for (int e = e0; e < e0 + n; e++) acc = acc + adj_b_term(m, a, c, m.ent[e]);
+    for (int e = e0; e < e0 + n; e++) {
+        v3<T> g;
+        acc = acc + adj_b_term<T, CORR>(m, a, c, m.ent[e], &g);
+        if (CORR) {
+            gx = gx + g.x;
+            gy = gy + g.y;
+            gz = gz + g.z;
+        }
+    }
     a.phibar[c] = acc;
+    if (CORR) {
+        const T fact = 1.0 / m.cvol[c];
+        st3(a.hnext, c, v3<T>{gx * fact, gy * fact, gz * fact});
+    }
 }
 
-template <class T, int LPC>
+template <class T, int LPC, bool CORR>
 __global__ __launch_bounds__(LPC * 32) void k_adj_b_vec(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
-    constexpr int CPB = kCellsPerBlock, P = LPC + 1;
-    __shared__ T lds[CPB * P];
+    constexpr int CPB = kCellsPerBlock, P = LPC + 1, NT = CORR ? 4 : 1;
+    static_assert(!CORR || LPC >= 4, "a lane per folded row");
+    __shared__ T lds[NT * CPB * P];
     const int lb = xcd_block(blockIdx.x, per);
     if (lb >= nblk) return;
     const int cl = threadIdx.x / LPC, j = threadIdx.x % LPC;
     const int c = lb * CPB + cl;
     int e0 = 0, n = 0;
     if (c < m.C) row_range(m, c, e0, n);
-    if (j < n) lds[cl * P + j] = adj_b_term(m, a, c, m.ent[e0 + j]);
+    if (j < n) {
+        v3<T> g;
+        lds[cl * P + j] = adj_b_term<T, CORR>(m, a, c, m.ent[e0 + j], &g);
+        if (CORR) {
+            lds[(1 * CPB + cl) * P + j] = g.x;
+            lds[(2 * CPB + cl) * P + j] = g.y;
+            lds[(3 * CPB + cl) * P + j] = g.z;
+        }
+    }
     __syncthreads();
-    if (c >= m.C || j != 0) return;
-    T acc = a.dir[c];
-    for (int i = 0; i < n; i++) acc = acc + lds[cl * P + i];
-    a.phibar[c] = acc;
+    if (c >= m.C || j >= NT) return;
+    T acc = j == 0 ? a.binit[c] : (T)0;
+    for (int i = 0; i < n; i++) acc = acc + lds[(j * CPB + cl) * P + i];
+    if (j == 0) a.phibar[c] = acc;
+    else a.hnext[3 * (size_t)c + (j - 1)] = acc * (1.0 / m.cvol[c]);
+}
+
+// Typed meshes: the cotangent of the PhiFace each boundary face k = g - Fi
+// leaves, summed over the faces that read it in ascending mesh face id
+// (readers rd[rd_off[k] .. rd_off[k + 1])): an interior reader adds
+// -((D*GammaBlend)*mf), an INLET that carries PhiFace -(SuBar[P]*f).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_carry(DevMesh<T> m, AdjArgs<T> a, const int *rd_off, const int *rd,
+                                                      T *pfbar) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.F - m.Fi) return;
+    T acc = 0.0;
+    for (int e = rd_off[k]; e < rd_off[k + 1]; e++) {
+        const int r = rd[e];
+        const int2 cn = m.fconn[r];
+        const T mf = a.s.massFlux[r];
+        T c;
+        if (cn.y >= 0) {
+            const T D = a.subar[cn.x] - a.subar[cn.y];
+            const T Dg = D * a.s.gammaBlend;
+            c = -(Dg * mf);
+        } else {
+            const T Visac = scalar_visac(a.s, a.s.visEffB[~cn.y]);
+            const T VisFace = Visac * m.frl[r];
+            const T f = -VisFace + (T)fmin((double)mf, 0.0);
+            c = -(a.subar[cn.x] * f);
+        }
+        acc = acc + c;
+    }
+    pfbar[k] = acc;
+}
+
+// Typed meshes: pass-A terms of boundary face g = Fi + k of cell P
+// (ScalarKernels.ipp:239-447) into bterm[4k..4k+3], and its MassFluxBar:
+//   INLET:  gradient cotangent SuBar[P]*w_b; MassFluxBar -(SuBar[P]*PhiFace) if mf < 0
+//   OUTLET: (SuBar[P]*w_b) + PFbar*Xpn; phi term PFbar
+//   WALL (enthalpy, flux region): phi term (Hcoef*SuBar[P]) + PFbar
+// with w_b = Visac*S - VisFace*Xpn, Xpn = Xf - Cc[P]; -0.0 where a face adds nothing.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_bnd(DevMesh<T> m, AdjArgs<T> a, const T *pfbar, T *bterm) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m.F - m.Fi) return;
+    const int g = m.Fi + k;
+    const int ip = m.fconn[g].x, ib = ~m.fconn[g].y;
+    const int ir = m.bregion[ib];
+    const int it = m.rtype[ir];
+    const T z = nzero<T>();
+    V4<T> t{z, z, z, z};
+    T mfb = 0.0;
+    const T sb = a.subar[ip];
+    if (it == CUPCFD_RTYPE_INLET || it == CUPCFD_RTYPE_OUTLET) {
+        const V4<T> s = m.fgeoS[g];
+        T Visac = it == CUPCFD_RTYPE_INLET ? a.s.visEffB[ib] : a.s.visEff[ip];
+        Visac = scalar_visac(a.s, Visac);
+        const v3<T> Xpn = sub3(xyz(m.fgeoX[g]), xyz(m.cgeo[ip]));
+        const T VisFace = Visac * m.frl[g];
+        const v3<T> w{Visac * s.x - VisFace * Xpn.x, Visac * s.y - VisFace * Xpn.y, Visac * s.z - VisFace * Xpn.z};
+        if (it == CUPCFD_RTYPE_INLET) {
+            t = V4<T>{sb * w.x, sb * w.y, sb * w.z, z};
+            const T mf = a.s.massFlux[g];
+            const T PhiFace = a.s.inletAssigns ? (T)0 : adj_phiface(m, a, g);
+            if (mf < 0) mfb = -(sb * PhiFace);
+        } else {
+            const T pb = pfbar[k];
+            t = V4<T>{sb * w.x + pb * Xpn.x, sb * w.y + pb * Xpn.y, sb * w.z + pb * Xpn.z, pb};
+        }
+    } else if (it == CUPCFD_RTYPE_WALL && a.s.wallActive && !m.radiab[ir] && m.rflux[ir]) {
+        T Hcoef, VisFace;
+        (void)scalar_wall_phiface(m, a.s, g, ib, ir, ip, Hcoef, VisFace);
+        t.w = Hcoef * sb + pfbar[k];
+    }
+    *(V4<T> *)(bterm + 4 * (size_t)k) = t;
+    a.mfbar[g] = mfb;
 }
 
 // ---------------------------------------------------------------------------
@@ -3445,6 +3575,39 @@ template <class T> int adj_lanes(const Mesh<T> &M) {
     return l == 4 || l == 6 || l == 8 ? l : 0;
 }
 
+template <class T, bool CORR>
+void adj_launch_b(int lpc, const Grid &gr, int nthr, hipStream_t st, const DevMesh<T> &d, const AdjArgs<T> &a) {
+    if (lpc == 8) hipLaunchKernelGGL((k_adj_b_vec<T, 8, CORR>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else if (lpc == 6) hipLaunchKernelGGL((k_adj_b_vec<T, 6, CORR>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else if (lpc == 4) hipLaunchKernelGGL((k_adj_b_vec<T, 4, CORR>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else hipLaunchKernelGGL((k_adj_b<T, CORR>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+}
+
+// Pass A, then pass B once per gradient sweep, last sweep first, on the
+// mesh's lane schedule. a.h receives the last sweep's h; hb is a second
+// buffer of 3C for nGradient > 1 (the sweeps alternate between the two).
+template <class T> int adj_passes(const Mesh<T> &M, AdjArgs<T> a, int nGradient, T *hb, hipStream_t st) {
+    const DevMesh<T> &d = M.d;
+    const int lpc = adj_lanes(M);
+    const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
+    const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
+    if (lpc == 8) hipLaunchKernelGGL((k_adj_a_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else if (lpc == 6) hipLaunchKernelGGL((k_adj_a_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else if (lpc == 4) hipLaunchKernelGGL((k_adj_a_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    else hipLaunchKernelGGL(k_adj_a<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
+    a.binit = a.dir;
+    T *const hbuf[2] = {a.h, hb};
+    for (int s = nGradient; s >= 1; s--) {
+        const int cur = (nGradient - s) & 1;
+        a.h = hbuf[cur];
+        a.hnext = hbuf[cur ^ 1];
+        if (s > 1) adj_launch_b<T, true>(lpc, gr, nthr, st, d, a);
+        else adj_launch_b<T, false>(lpc, gr, nthr, st, d, a);
+        a.binit = a.phibar;
+    }
+    return launch_ok();
+}
+
 template <class T>
 int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const T *MF, const T *SuBar,
                        const T *RFaceBar, int SolveTurb, T vislam, int ivar, int VarT, T Sigma_T, T Prandtl, int VarTE,
@@ -3463,31 +3626,65 @@ int dev_scalar_adjoint(cupcfd_hip_mesh *h, const T *Phi, const T *VisEff, const 
     a.rbar = RFaceBar;
     a.phibar = PhiBar;
     a.mfbar = MFBar;
-    hipStream_t st = (hipStream_t)stream;
     if (d.C) {
         // h and dir are written for every cell by pass A before pass B reads them
         a.h = scratch_h ? scratch_h : (T *)M->ws_get(47, 3 * (size_t)d.C * sizeof(T));
         a.dir = scratch_dir ? scratch_dir : (T *)M->ws_get(48, (size_t)d.C * sizeof(T));
         if (!a.h || !a.dir) return CUPCFD_E_ERROR;
-        const int lpc = adj_lanes(*M);
-        const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
-        const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
-        if (lpc == 8) {
-            hipLaunchKernelGGL((k_adj_a_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-        } else if (lpc == 6) {
-            hipLaunchKernelGGL((k_adj_a_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-        } else if (lpc == 4) {
-            hipLaunchKernelGGL((k_adj_a_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-            hipLaunchKernelGGL((k_adj_b_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
-        } else {
-            hipLaunchKernelGGL(k_adj_a<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
-            hipLaunchKernelGGL(k_adj_b<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
-        }
+        return adj_passes(*M, a, 1, (T *)nullptr, (hipStream_t)stream);
     }
-
     return launch_ok();
+}
+
+// The same chain with nGradient gradient sweeps, on any mesh (typed
+// boundaries, the carried PhiFace and the enthalpy wall included); dPhidx
+// is the forward gradient the scalar loop read.
+template <class T>
+int dev_scalar_adjoint_full(cupcfd_hip_mesh *h, int nGradient, const T *Phi, const T *VisEff, const T *MF,
+                            const T *dPhidx, const T *TE, const T *Den, const T *CpB, const T *VisEffB,
+                            const T *SuBar, const T *RFaceBar, int SolveTurb, int SolveEnthalpy, T vislam, int ivar,
+                            int VarT, T Sigma_T, T Prandtl, int VarTE, T Sigma_k, int VarED, T Sigma_e, T Sigma_s,
+                            T Schmidt, T GammaBlend, T Small, T TMCmu, T *PhiBar, T *MFBar, void *stream) {
+    Mesh<T> *M = get<T>(h);
+    if (!M) return CUPCFD_E_NULL_PTR;
+    const DevMesh<T> &d = M->d;
+    if (nGradient < 1) return CUPCFD_E_CONFIG_INVALID_VALUE;
+    if (!Phi || !VisEff || !MF || !SuBar || !RFaceBar || !PhiBar || !MFBar) return CUPCFD_E_NULL_PTR;
+    if (d.typed && (!dPhidx || !TE || !Den || !CpB || !VisEffB)) return CUPCFD_E_NULL_PTR;
+    AdjArgs<T> a{};
+    const int rc = scalar_args<T, T>(*M, a.s, Phi, nullptr, VisEff, nullptr, nullptr, Den, MF, TE, CpB, VisEffB,
+                                     nullptr, dPhidx, SolveTurb, SolveEnthalpy, vislam, ivar, VarT, Sigma_T, Prandtl,
+                                     VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend, Small, TMCmu);
+    if (rc != CUPCFD_E_SUCCESS) return rc;
+    a.subar = SuBar;
+    a.rbar = RFaceBar;
+    a.phibar = PhiBar;
+    a.mfbar = MFBar;
+    hipStream_t st = (hipStream_t)stream;
+    if (!d.C) return launch_ok();
+    a.h = (T *)M->ws_get(47, 3 * (size_t)d.C * sizeof(T));
+    a.dir = (T *)M->ws_get(48, (size_t)d.C * sizeof(T));
+    T *hb = nGradient > 1 ? (T *)M->ws_get(57, 3 * (size_t)d.C * sizeof(T)) : nullptr;
+    if (!a.h || !a.dir || (nGradient > 1 && !hb)) return CUPCFD_E_ERROR;
+    const int nb = d.F - d.Fi;
+    if (d.typed && nb > 0) {
+        // the carried PhiFace each source face leaves, the readers' cotangent
+        // of it per source, then each boundary face's terms
+        T *cval = (T *)M->ws_get(61, (size_t)nb * sizeof(T));
+        T *pfbar = (T *)M->ws_get(58, (size_t)nb * sizeof(T));
+        T *bterm = (T *)M->ws_get(59, 4 * (size_t)nb * sizeof(T));
+        const int *rd_off = nullptr, *rd = nullptr;
+        if (!cval || !pfbar || !bterm ||
+            scalar_carry_readers<T>(*M, a.s.inletAssigns != 0, a.s.wallActive != 0, &rd_off, &rd) !=
+                CUPCFD_E_SUCCESS)
+            return CUPCFD_E_ERROR;
+        hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a.s, cval);
+        a.s.cval = cval;
+        hipLaunchKernelGGL(k_adj_carry<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, rd_off, rd, pfbar);
+        hipLaunchKernelGGL(k_adj_bnd<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, (const T *)pfbar, bterm);
+        a.bterm = bterm;
+    }
+    return adj_passes(*M, a, nGradient, hb, st);
 }
 
 template <class T, class TS = T>
@@ -4074,6 +4271,21 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     return dev_scalar_adjoint<double>(mesh, PhiCell, VisEff, MassFlux, SuBar, RFaceBar, SolveTurb, vislam, ivar, VarT,
                                       Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt, GammaBlend,
                                       PhiBar, MassFluxBar, stream);
+}
+
+int cupcfd_dev_scalar_adjoint_full_f64(cupcfd_hip_mesh *mesh, int32_t nGradient, const double *PhiCell,
+                                       const double *VisEff, const double *MassFlux, const double *dPhidx,
+                                       const double *TE, const double *Den, const double *CpBoundary,
+                                       const double *visEffBoundary, const double *SuBar, const double *RFaceBar,
+                                       int32_t SolveTurb, int32_t SolveEnthalpy, double vislam, int32_t ivar,
+                                       int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE, double Sigma_k,
+                                       int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt,
+                                       double GammaBlend, double Small, double TMCmu, double *PhiBar,
+                                       double *MassFluxBar, void *stream) {
+    return dev_scalar_adjoint_full<double>(mesh, nGradient, PhiCell, VisEff, MassFlux, dPhidx, TE, Den, CpBoundary,
+                                           visEffBoundary, SuBar, RFaceBar, SolveTurb, SolveEnthalpy, vislam, ivar,
+                                           VarT, Sigma_T, Prandtl, VarTE, Sigma_k, VarED, Sigma_e, Sigma_s, Schmidt,
+                                           GammaBlend, Small, TMCmu, PhiBar, MassFluxBar, stream);
 }
 
 int cupcfd_dev_scalar_adjoint_noalloc_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, const double *VisEff,

@@ -128,7 +128,7 @@ HOST_SYMBOLS = [
 BATCHED = ["gradient_batched", "flux_scalar_batched"]
 # adjoint entry points (FP64 only)
 ADJOINT = ["cupcfd_dev_scalar_adjoint_f64", "cupcfd_dev_scalar_adjoint_batched_f64",
-           "cupcfd_dev_scalar_adjoint_noalloc_f64"]
+           "cupcfd_dev_scalar_adjoint_noalloc_f64", "cupcfd_dev_scalar_adjoint_full_f64"]
 # FP32-storage entry points on 64-bit meshes
 F32S = ["cupcfd_dev_gradient_batched_f32s", "cupcfd_dev_flux_scalar_batched_f32s",
         "cupcfd_dev_scalar_adjoint_batched_f32s"]

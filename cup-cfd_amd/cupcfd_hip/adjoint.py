@@ -1,9 +1,11 @@
 """Reverse mode of the gradient + scalar-flux chain on the GPU, for torch.
 
 ``ScalarFlux`` is a ``torch.autograd.Function``: its forward runs
-GradientPhiGaussDolfyn (nGradient = 1) and FluxScalarDolfynFaceLoop through
-the resident C ABI, from Su = 0, and returns (Su, RFace); its backward calls
-``cupcfd_dev_scalar_adjoint_f64`` for the cotangents of phi and MassFlux.
+GradientPhiGaussDolfyn (``nGradient`` sweeps) and FluxScalarDolfynFaceLoop
+through the resident C ABI, from Su = 0, and returns (Su, RFace); its
+backward calls ``cupcfd_dev_scalar_adjoint_f64`` (one sweep, all-DEFAULT
+mesh) or ``cupcfd_dev_scalar_adjoint_full_f64`` (any mesh, any number of
+sweeps) for the cotangents of phi and MassFlux.
 ``march`` and ``march_adjoint`` run explicit pseudo-time steps
 phi <- phi + dtau * Su(phi, mf) / vol and their reverse with checkpoints:
 the states at every ``every``-th step are kept, the states inside a segment
@@ -11,7 +13,7 @@ are recomputed from its checkpoint when the reverse sweep reaches it.
 
 Tensors are FP64 CUDA tensors in the device mesh's internal numbering (cell
 arrays (C,), face arrays (F,), boundary arrays in mesh numbering), as for
-``cupcfd_hip.fvm.resident``. Meshes whose regions are all DEFAULT only.
+``cupcfd_hip.fvm.resident``.
 """
 from __future__ import annotations
 
@@ -24,8 +26,11 @@ class Chain:
     """What the chain needs besides (phi, mf): the mesh, phiB, VisEff and the scalar loop's parameters.
     The inputs the chain does not differentiate (Den, TE, CpB, visEffB, PhiB) are fixed tensors."""
 
-    def __init__(self, mesh, phiB, visEff, params, den=None, te=None, cpB=None, visEffB=None):
+    def __init__(self, mesh, phiB, visEff, params, den=None, te=None, cpB=None, visEffB=None, nGradient=1):
         dev = visEff.device
+        self.nGradient = nGradient
+        # the one-sweep adjoint of all-DEFAULT meshes needs no forward gradient
+        self.plain = nGradient == 1 and bool((mesh.host.a["regType"] == 0).all())
         C = visEff.shape[0]
         B = phiB.shape[0]
         self.mesh, self.phiB, self.visEff, self.params = mesh, phiB, visEff, dict(params)
@@ -36,12 +41,16 @@ class Chain:
         self.visEffB = visEffB if visEffB is not None else one(B)
         self.C, self.B = C, B
 
+    def gradient(self, phi):
+        g = torch.empty(3 * self.C, dtype=torch.float64, device=phi.device)
+        go = torch.empty_like(g)
+        R.gradient(self.mesh, self.nGradient, phi, self.phiB, g, go)
+        return g
+
     def forward(self, phi, mf):
         """(Su, RFace) from Su = 0."""
         dev = phi.device
-        g = torch.empty(3 * self.C, dtype=torch.float64, device=dev)
-        go = torch.empty_like(g)
-        R.gradient(self.mesh, 1, phi, self.phiB, g, go)
+        g = self.gradient(phi)
         su = torch.zeros(self.C, dtype=torch.float64, device=dev)
         au = torch.zeros(self.C, dtype=torch.float64, device=dev)
         rface = torch.zeros(2 * mf.shape[0], dtype=torch.float64, device=dev)
@@ -54,8 +63,13 @@ class Chain:
         """(phibar, mfbar) = J^T (subar, rbar)."""
         phibar = torch.empty_like(phi)
         mfbar = torch.empty_like(mf)
-        R.scalar_adjoint(self.mesh, phi, self.visEff, mf, subar.contiguous(), rbar.contiguous(), self.params, phibar,
-                         mfbar)
+        if self.plain:
+            R.scalar_adjoint(self.mesh, phi, self.visEff, mf, subar.contiguous(), rbar.contiguous(), self.params,
+                             phibar, mfbar)
+        else:
+            R.scalar_adjoint_full(self.mesh, self.nGradient, phi, self.visEff, mf, self.gradient(phi), self.te,
+                                  self.den, self.cpB, self.visEffB, subar.contiguous(), rbar.contiguous(),
+                                  self.params, phibar, mfbar)
         return phibar, mfbar
 
 

@@ -332,6 +332,25 @@ class resident:
         check(fn, getattr(lib(), fn)(mesh.handle, *a[:5], *cls._adjoint_params(params), *a[5:], cls._s(stream)))
 
     @classmethod
+    def scalar_adjoint_full(cls, mesh, nGradient, phi, visEff, mf, dphidx, te, den, cpB, visEffB, subar, rbar,
+                            params, phibar, mfbar, stream=None):
+        """Adjoint of gradient (nGradient sweeps) + scalar face loop w.r.t. (phi, mf) on any FP64 mesh
+        (cupcfd_dev_scalar_adjoint_full_f64): dphidx is the forward gradient the scalar loop read, te / den /
+        cpB / visEffB its other inputs (None allowed on all-DEFAULT meshes)."""
+        from . import lib, check
+        a = cls._args(mesh, [(phi, "c", "phi"), (visEff, "c", "visEff"), (mf, "f", "massFlux"),
+                             (dphidx, "c3", "dPhidx"), (te, "c", "TE"), (den, "c", "Den"), (cpB, "b", "CpBoundary"),
+                             (visEffB, "b", "visEffBoundary"), (subar, "c", "subar"), (rbar, "f2", "rbar"),
+                             (phibar, "c", "phibar"), (mfbar, "f", "mfbar")])
+        D = ct.c_double
+        p = params
+        check("cupcfd_dev_scalar_adjoint_full_f64", lib().cupcfd_dev_scalar_adjoint_full_f64(
+            mesh.handle, _I(nGradient), *a[:10], _I(p["SolveTurb"]), _I(p["SolveEnthalpy"]), D(p["vislam"]),
+            _I(p["ivar"]), _I(p["VarT"]), D(p["Sigma_T"]), D(p["Prandtl"]), _I(p["VarTE"]), D(p["Sigma_k"]),
+            _I(p["VarED"]), D(p["Sigma_e"]), D(p["Sigma_s"]), D(p["Schmidt"]), D(p["GammaBlend"]), D(p["Small"]),
+            D(p["TMCmu"]), *a[10:], cls._s(stream)))
+
+    @classmethod
     def gradient_batched_f32s(cls, mesh, nGradient, K, phi, phiB, g, go, stream=None):
         """Batched gradient with float32 storage (phi, phiB, g, go float32 tensors) on a 64-bit mesh: double
         geometry and arithmetic, the final gradient rounded to float."""

@@ -506,6 +506,27 @@ int cupcfd_dev_scalar_adjoint_f64(cupcfd_hip_mesh *mesh, const double *PhiCell, 
     double vislam, int32_t ivar, int32_t VarT, double Sigma_T, double Prandtl, int32_t VarTE,
     double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s, double Schmidt, double GammaBlend,
     double *PhiBar, double *MassFluxBar, void *stream);
+/* The adjoint of GradientPhiGaussDolfyn with nGradient >= 1 sweeps
+ * (GradientKernels.ipp:38-136, its correction sweeps included) followed by
+ * FluxScalarDolfynFaceLoop (ScalarKernels.ipp:36-463) w.r.t. (PhiCell,
+ * MassFlux), on any mesh: the INLET and OUTLET source terms, the PhiFace that
+ * OUTLET, INLET and enthalpy WALL faces carry to the faces after them, and the
+ * WALL's Hcoef term are transposed. No counterpart in the reference. dPhidx
+ * is the forward gradient the scalar loop read; TE, Den, CpBoundary and
+ * visEffBoundary are its other inputs (all four, and dPhidx, may be NULL on
+ * an all-DEFAULT mesh, where nothing reads them). Cotangents are on Su (SuBar,
+ * nCells) and RFace (RFaceBar, 2 nFaces); PhiBoundary, Au and the mesh state
+ * are not differentiated. Writes PhiBar (nCells) and MassFluxBar (nFaces).
+ * Device arrays in internal numbering (boundary arrays in mesh numbering), as
+ * for the other resident calls. With nGradient = 1 on an all-DEFAULT mesh the
+ * result is cupcfd_dev_scalar_adjoint_f64's, bit for bit. */
+int cupcfd_dev_scalar_adjoint_full_f64(cupcfd_hip_mesh *mesh, int32_t nGradient, const double *PhiCell,
+    const double *VisEff, const double *MassFlux, const double *dPhidx, const double *TE, const double *Den,
+    const double *CpBoundary, const double *visEffBoundary, const double *SuBar, const double *RFaceBar,
+    int32_t SolveTurb, int32_t SolveEnthalpy, double vislam, int32_t ivar, int32_t VarT, double Sigma_T,
+    double Prandtl, int32_t VarTE, double Sigma_k, int32_t VarED, double Sigma_e, double Sigma_s,
+    double Schmidt, double GammaBlend, double Small, double TMCmu, double *PhiBar, double *MassFluxBar,
+    void *stream);
 /* cupcfd_dev_scalar_adjoint_f64 with no device allocation: no counterpart in
  * the reference; for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Its pass-A
