@@ -778,9 +778,10 @@ def bench_batched(torch, host, dev, K, steps, warmup, label, oracle_sample=None)
     }
 
 
-def bench_adjoint(torch, host, dev, steps, warmup, label):
-    """Forward (gradient nGradient=1 + scalar flux) against forward + adjoint w.r.t. (phi, massFlux) on
-    ``host`` (all regions DEFAULT) in FP64, seeded N(0,1) cotangents on Su and RFace."""
+def bench_adjoint(torch, host, dev, steps, warmup, label, nG=1, full=False, params=None):
+    """Forward (gradient with nG sweeps + scalar flux) against forward + adjoint w.r.t. (phi, massFlux) on
+    ``host`` in FP64, seeded N(0,1) cotangents on Su and RFace. ``full`` times cupcfd_dev_scalar_adjoint_full_f64
+    (any mesh, nG sweeps), else cupcfd_dev_scalar_adjoint_f64 (nG = 1, all-DEFAULT mesh)."""
     from cupcfd_hip import workload as W
     from cupcfd_hip.fvm import resident as R
 
@@ -809,13 +810,17 @@ def bench_adjoint(torch, host, dev, steps, warmup, label):
     pbar = torch.empty(C, dtype=dt, device=cuda)
     mbar = torch.empty(F, dtype=dt, device=cuda)
     stream = torch.cuda.current_stream()
+    P = params or W.SCALAR
 
     def fwd():
-        R.gradient(dev, 1, phi, phiB, g, go, stream)
-        R.scalar(dev, phi, pB, visEff, au, su, den, mf, te, cpB, visEffB, rface, g, W.SCALAR, stream)
+        R.gradient(dev, nG, phi, phiB, g, go, stream)
+        R.scalar(dev, phi, pB, visEff, au, su, den, mf, te, cpB, visEffB, rface, g, P, stream)
 
     def adj():
-        R.scalar_adjoint(dev, phi, visEff, mf, sb, rb, W.SCALAR, pbar, mbar, stream)
+        if full:
+            R.scalar_adjoint_full(dev, nG, phi, visEff, mf, g, te, den, cpB, visEffB, sb, rb, P, pbar, mbar, stream)
+        else:
+            R.scalar_adjoint(dev, phi, visEff, mf, sb, rb, P, pbar, mbar, stream)
 
     def timed(fn):
         quiesce()
@@ -833,8 +838,8 @@ def bench_adjoint(torch, host, dev, steps, warmup, label):
     tf = timed(fwd)
     tfa = timed(lambda: (fwd(), adj()))
     return {
-        "workload": f"gradient (nGradient=1) + scalar flux, forward and reverse w.r.t. (phi, massFlux), {label}, "
-                    f"FP64, one DEFAULT region",
+        "workload": f"gradient (nGradient={nG}) + scalar flux, forward and reverse w.r.t. (phi, massFlux), {label}, "
+                    f"FP64" + (", typed regions, enthalpy wall" if full else ", one DEFAULT region"),
         "cells": C, "forward_ms": tf * 1e3, "forward_adjoint_ms": tfa * 1e3, "ratio": tfa / tf,
     }
 
@@ -1151,6 +1156,17 @@ def main():
         ha, da, la = structgen(args.adjoint_n, 12345)
         out["adjoint"] = bench_adjoint(torch, ha, da, steps=max(5, min(args.steps, 20)), warmup=2, label=la)
         del ha, da
+        # any mesh, several sweeps: typed relabelled mesh, enthalpy wall, nGradient = 2
+        import cupcfd_hip as ch
+        from cupcfd_hip import workload as W
+        ht = ch.HostMesh(args.adjoint_n, seed=12345, permute_cells=True, permute_faces=True, typed=True,
+                         face_extras=True)
+        dt_ = ch.DeviceMesh(ht, 64, reorder=True)
+        pw = dict(W.SCALAR, ivar=W.SCALAR["VarT"], SolveEnthalpy=1)
+        out["adjoint"]["typed_full"] = bench_adjoint(torch, ht, dt_, steps=max(5, min(args.steps, 20)), warmup=2,
+                                                     label=f"StructGen {args.adjoint_n}^3 relabelled", nG=2,
+                                                     full=True, params=pw)
+        del ht, dt_
     if not args.no_partitions:
         out["partitions"] = bench_partitions(torch, 48, 64, steps=max(3, min(args.steps, 10)), warmup=2)
     if not args.no_tets:

@@ -503,7 +503,8 @@ template int *scalar_carry_src<double>(Mesh<double> &, bool, bool);
 template int *scalar_carry_src<float>(Mesh<float> &, bool, bool);
 
 template <class T>
-int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd) {
+int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd,
+                         int *nrd) {
     const int key = (inlet_assigns ? 1 : 0) | (wall_assigns ? 2 : 0);
     if (!M.sc_rd_off[key]) {
         const int F = M.d.F, Fi = M.d.Fi, nb = F - Fi;
@@ -554,14 +555,16 @@ int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, cons
         }
         M.sc_rd_off[key] = (int *)po;
         M.sc_rd[key] = (int *)pr;
+        M.sc_nrd[key] = (int)list.size();
     }
     *off = M.sc_rd_off[key];
     *rd = M.sc_rd[key];
+    *nrd = M.sc_nrd[key];
     return CUPCFD_E_SUCCESS;
 }
 
-template int scalar_carry_readers<double>(Mesh<double> &, bool, bool, const int **, const int **);
-template int scalar_carry_readers<float>(Mesh<float> &, bool, bool, const int **, const int **);
+template int scalar_carry_readers<double>(Mesh<double> &, bool, bool, const int **, const int **, int *);
+template int scalar_carry_readers<float>(Mesh<float> &, bool, bool, const int **, const int **, int *);
 
 // A stable counting sort of the boundary ids by region: each region's
 // boundaries in ascending id, the order the reference's boundary loops add

@@ -121,6 +121,7 @@ template <class T> struct Mesh {
     // .. sc_rd_off[key][k + 1])), built on first use by the adjoint
     int *sc_rd_off[4] = {nullptr, nullptr, nullptr, nullptr};
     int *sc_rd[4] = {nullptr, nullptr, nullptr, nullptr};
+    int sc_nrd[4] = {0, 0, 0, 0};
     // boundaries grouped by region, ascending id within each region (region r:
     // rbnd[rbnd_off[r] .. rbnd_off[r + 1])), built on first use by the
     // per-region ordered sums
@@ -210,7 +211,8 @@ template <class T> int build_face_colours(Mesh<T> &M);
 // (INLET assigns, WALL assigns) combination: interior faces, and INLET faces
 // when INLET does not assign; built once per mesh and combination.
 template <class T>
-int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd);
+int scalar_carry_readers(Mesh<T> &M, bool inlet_assigns, bool wall_assigns, const int **off, const int **rd,
+                         int *nrd);
 
 // The same repack as the host build, run on the device: every array of the
 // DevMesh and every host-side mirror of Mesh comes out bit-identical. With

@@ -2547,32 +2547,48 @@ __global__ __launch_bounds__(LPC * 32) void k_adj_b_vec(DevMesh<T> m, AdjArgs<T>
 }
 
 // Typed meshes: the cotangent of the PhiFace each boundary face k = g - Fi
-// leaves, summed over the faces that read it in ascending mesh face id
-// (readers rd[rd_off[k] .. rd_off[k + 1])): an interior reader adds
-// -((D*GammaBlend)*mf), an INLET that carries PhiFace -(SuBar[P]*f).
+// leaves is the sum, in ascending mesh face id, of its readers' terms
+// (readers rd[rd_off[k] .. rd_off[k + 1])): an interior reader's
+// -((D*GammaBlend)*mf), an INLET's that carries PhiFace -(SuBar[P]*f).
+// k_adj_carry_terms evaluates every reader's term into coef[e] (one thread
+// per reader entry); k_adj_carry_sum adds each source's run of coef in order.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_carry(DevMesh<T> m, AdjArgs<T> a, const int *rd_off, const int *rd,
-                                                      T *pfbar) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= m.F - m.Fi) return;
-    T acc = 0.0;
-    for (int e = rd_off[k]; e < rd_off[k + 1]; e++) {
-        const int r = rd[e];
-        const int2 cn = m.fconn[r];
-        const T mf = a.s.massFlux[r];
-        T c;
-        if (cn.y >= 0) {
-            const T D = a.subar[cn.x] - a.subar[cn.y];
-            const T Dg = D * a.s.gammaBlend;
-            c = -(Dg * mf);
-        } else {
-            const T Visac = scalar_visac(a.s, a.s.visEffB[~cn.y]);
-            const T VisFace = Visac * m.frl[r];
-            const T f = -VisFace + (T)fmin((double)mf, 0.0);
-            c = -(a.subar[cn.x] * f);
-        }
-        acc = acc + c;
+__global__ __launch_bounds__(kBlock) void k_adj_carry_terms(DevMesh<T> m, AdjArgs<T> a, const int *rd, int n,
+                                                            T *coef) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n) return;
+    const int r = rd[e];
+    const int2 cn = m.fconn[r];
+    const T mf = a.s.massFlux[r];
+    T c;
+    if (cn.y >= 0) {
+        const T D = a.subar[cn.x] - a.subar[cn.y];
+        const T Dg = D * a.s.gammaBlend;
+        c = -(Dg * mf);
+    } else {
+        const T Visac = scalar_visac(a.s, a.s.visEffB[~cn.y]);
+        const T VisFace = Visac * m.frl[r];
+        const T f = -VisFace + (T)fmin((double)mf, 0.0);
+        c = -(a.subar[cn.x] * f);
     }
+    coef[e] = c;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_carry_sum(int nb, const int *rd_off, const T *coef, T *pfbar) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nb) return;
+    const int e1 = rd_off[k + 1];
+    T acc = 0.0;
+    int e = rd_off[k];
+    for (; e + 4 <= e1; e += 4) {  // four independent loads, then the adds in order
+        const T c0 = coef[e], c1 = coef[e + 1], c2 = coef[e + 2], c3 = coef[e + 3];
+        acc = acc + c0;
+        acc = acc + c1;
+        acc = acc + c2;
+        acc = acc + c3;
+    }
+    for (; e < e1; e++) acc = acc + coef[e];
     pfbar[k] = acc;
 }
 
@@ -3674,13 +3690,18 @@ int dev_scalar_adjoint_full(cupcfd_hip_mesh *h, int nGradient, const T *Phi, con
         T *pfbar = (T *)M->ws_get(58, (size_t)nb * sizeof(T));
         T *bterm = (T *)M->ws_get(59, 4 * (size_t)nb * sizeof(T));
         const int *rd_off = nullptr, *rd = nullptr;
+        int nrd = 0;
         if (!cval || !pfbar || !bterm ||
-            scalar_carry_readers<T>(*M, a.s.inletAssigns != 0, a.s.wallActive != 0, &rd_off, &rd) !=
+            scalar_carry_readers<T>(*M, a.s.inletAssigns != 0, a.s.wallActive != 0, &rd_off, &rd, &nrd) !=
                 CUPCFD_E_SUCCESS)
             return CUPCFD_E_ERROR;
+        T *coef = (T *)M->ws_get(62, (size_t)(nrd ? nrd : 1) * sizeof(T));
+        if (!coef) return CUPCFD_E_ERROR;
         hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a.s, cval);
         a.s.cval = cval;
-        hipLaunchKernelGGL(k_adj_carry<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, rd_off, rd, pfbar);
+        if (nrd) hipLaunchKernelGGL(k_adj_carry_terms<T>, dim3(blocks(nrd)), dim3(kBlock), 0, st, d, a, rd, nrd, coef);
+        hipLaunchKernelGGL(k_adj_carry_sum<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, nb, rd_off, (const T *)coef,
+                           pfbar);
         hipLaunchKernelGGL(k_adj_bnd<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, (const T *)pfbar, bterm);
         a.bterm = bterm;
     }
