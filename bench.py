@@ -1037,7 +1037,7 @@ def main():
     ap.add_argument("--tets-only", action="store_true", help="run only the tetrahedral stress mesh")
     ap.add_argument("--tets-n", type=int, default=100)
     ap.add_argument("--tets-k", type=int, default=16)
-    ap.add_argument("--adjoint-n", type=int, default=64)
+    ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-k", type=int, default=64)
     ap.add_argument("--schedule", type=int, default=-1,
