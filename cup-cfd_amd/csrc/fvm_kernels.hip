@@ -2546,49 +2546,264 @@ __global__ __launch_bounds__(LPC * 32) void k_adj_b_vec(DevMesh<T> m, AdjArgs<T>
     else a.hnext[3 * (size_t)c + (j - 1)] = acc * (1.0 / m.cvol[c]);
 }
 
+// ---------------------------------------------------------------------------
+// Tile schedule of the adjoint passes (the forward tiles' three phases): each
+// slot's terms are evaluated once for both of its cells (the two sides'
+// terms differ by fac, and in pass A by max / min(mf, 0), not only by sign),
+// then each (cell, component) folds its row in face order, so the sums are
+// the lane schedule's bit for bit.
+// ---------------------------------------------------------------------------
+// cell values pass A reads at a cell
+template <class T> struct AdjCellA {
+    T subar, vis, phi;
+    v3<T> cc;
+};
+
+// Pass A: h_c = (sum of the row's gradient-cotangent terms) / vol_c and dir_c;
+// MassFluxBar of each owned interior face (and of owned DEFAULT boundary
+// faces) is written by its slot.
+template <class T>
+__global__ __launch_bounds__(kTileBlock) void k_adj_a_tile(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TC = kTileCells, NF = 6, NT = 8;  // cells: SuBar, VisEff, phi, centre xyz; slots: 4 terms per side
+    const int S = m.tslots;
+    T *lc = (T *)smem;                         // [NF][TC]
+    T *lt = lc + NF * TC;                      // [NT][S]
+    uint16_t *le = (uint16_t *)(lt + NT * S);  // [tnnz]
+    const int t = xcd_block(blockIdx.x, per);
+    if (t >= nblk) return;
+    const int tid = threadIdx.x;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    stage1(lc, a.subar, c0, nc);
+    stage1(lc + TC, a.s.visEff, c0, nc);
+    stage1(lc + 2 * TC, a.s.phi, c0, nc);
+    stage_centre(lc + 3 * TC, m.cgeo, c0, nc);
+    const int eb = stage_eslots(m, le, c0, nc);
+    int f = 0, out = -1;
+    int2 cn = make_int2(0, 0);
+    bool owned;
+    V4<T> s{};
+    T mf = 0, rl = 0, pf = m.carry_init, rb0 = 0, rb1 = 0;
+    AdjCellA<T> o{};
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        out = -1;
+        if (cn.y < 0) return;
+        s = m.fgeoS[f];
+        mf = a.s.massFlux[f];
+        rl = m.frl[f];
+        if (owned) {
+            pf = adj_phiface(m, a, f);
+            rb0 = a.rbar[2 * (size_t)f];
+            rb1 = a.rbar[2 * (size_t)f + 1];
+        }
+        out = outside_cell(cn, c0, nc);
+        if (out >= 0) o = AdjCellA<T>{a.subar[out], a.s.visEff[out], a.s.phi[out], xyz(m.cgeo[out])};
+    };
+    if (tid < ts.nslot) load(tid);
+    const int k = tid / TC, cl = tid % TC;  // fold item: component k (h xyz, dir) of cell cl
+    const bool fold = cl < nc;
+    int e0 = 0, n = 0;
+    if (fold) row_range(m, c0 + cl, e0, n);
+    __syncthreads();
+    auto cell = [&](int c) -> AdjCellA<T> {
+        const bool o_ = c == out;
+        const int l = o_ ? 0 : c - c0;
+        return AdjCellA<T>{o_ ? o.subar : lc[l], o_ ? o.vis : lc[TC + l], o_ ? o.phi : lc[2 * TC + l],
+                           v3<T>{o_ ? o.cc.x : lc[3 * TC + l], o_ ? o.cc.y : lc[4 * TC + l],
+                                 o_ ? o.cc.z : lc[5 * TC + l]}};
+    };
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
+        if (sl != tid) load(sl);
+        T *q = lt + sl;
+        const T z = nzero<T>();
+        if (cn.y < 0) {  // P side only
+            if (a.bterm) {
+                const V4<T> bt = *(const V4<T> *)(a.bterm + 4 * (size_t)(f - m.Fi));
+                q[0] = bt.x, q[S] = bt.y, q[2 * S] = bt.z, q[3 * S] = bt.w;
+            } else {
+                q[0] = q[S] = q[2 * S] = q[3 * S] = z;
+                if (owned) a.mfbar[f] = 0.0;
+            }
+            continue;
+        }
+        const AdjCellA<T> P = cell(cn.x), N = cell(cn.y);
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        T Visac = N.vis * facn + P.vis * facp;
+        Visac = scalar_visac(a.s, Visac);
+        const T VisFace = Visac * rl;
+        const v3<T> Xpn = sub3(N.cc, P.cc);
+        const T D = P.subar - N.subar;
+        const T Dg = D * a.s.gammaBlend;
+        const T wx = Visac * s.x - VisFace * Xpn.x, wy = Visac * s.y - VisFace * Xpn.y,
+                wz = Visac * s.z - VisFace * Xpn.z;
+        const T tp = D * facp, tn = D * facn;
+        q[0] = tp * wx, q[S] = tp * wy, q[2 * S] = tp * wz;
+        q[3 * S] = Dg * (T)fmax((double)mf, 0.0);
+        q[4 * S] = tn * wx, q[5 * S] = tn * wy, q[6 * S] = tn * wz;
+        q[7 * S] = Dg * (T)fmin((double)mf, 0.0);
+        if (owned) {
+            const T dfci = mf < 0 ? N.phi : (mf > 0 ? P.phi : (T)0.0);
+            const T rterm = mf < 0 ? rb1 : (mf > 0 ? -rb0 : (T)0.0);
+            a.mfbar[f] = -(Dg * (pf - dfci)) + rterm;
+        }
+    }
+    __syncthreads();
+    if (fold) {
+        T acc = 0;
+        for (int e = e0; e < e0 + n; e++) {
+            const int es = le[e - eb];
+            const int sl = es & 0x7fff;
+            acc = acc + lt[(k + ((es & 0x8000) ? 4 : 0)) * S + sl];
+        }
+        adj_a_store(m, a, c0 + cl, k, acc);
+    }
+}
+
+// Pass B: PhiBar_c = binit_c + sum of the row's t = fac*dot(h_P + h_N, S); with
+// CORR also hnext_c = (sum of t*corr_f) / vol_c.
+template <class T, bool CORR>
+__global__ __launch_bounds__(kTileBlock) void k_adj_b_tile(DevMesh<T> m, AdjArgs<T> a, int per, int nblk) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int TC = kTileCells, NF = CORR ? 6 : 3, NT = CORR ? 8 : 2;  // slot rows: t per side (+ t*corr per side)
+    const int S = m.tslots;
+    T *lc = (T *)smem;                         // [NF][TC]: h xyz (+ centre xyz)
+    T *lt = lc + NF * TC;                      // [NT][S]
+    uint16_t *le = (uint16_t *)(lt + NT * S);  // [tnnz]
+    const int t = xcd_block(blockIdx.x, per);
+    if (t >= nblk) return;
+    const int tid = threadIdx.x;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    stage3(lc, a.h, c0, nc);
+    if (CORR) stage_centre(lc + 3 * TC, m.cgeo, c0, nc);
+    const int eb = stage_eslots(m, le, c0, nc);
+    int f = 0, out = -1;
+    int2 cn = make_int2(0, 0);
+    bool owned;
+    V4<T> s{}, xf{};
+    v3<T> oh{}, occ{};
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        out = -1;
+        if (cn.y < 0) return;
+        s = m.fgeoS[f];
+        if (CORR) xf = m.fgeoX[f];
+        out = outside_cell(cn, c0, nc);
+        if (out >= 0) {
+            oh = ld3(a.h, out);
+            if (CORR) occ = xyz(m.cgeo[out]);
+        }
+    };
+    if (tid < ts.nslot) load(tid);
+    const int k = tid / TC, cl = tid % TC;  // fold item: PhiBar (k = 0) or component k - 1 of hnext
+    const bool fold = k < (CORR ? 4 : 1) && cl < nc;
+    int e0 = 0, n = 0;
+    T acc = 0;
+    if (fold) {
+        row_range(m, c0 + cl, e0, n);
+        if (k == 0) acc = a.binit[c0 + cl];
+    }
+    __syncthreads();
+    auto hof = [&](int c) -> v3<T> {
+        if (c == out) return oh;
+        const int l = c - c0;
+        return v3<T>{lc[l], lc[TC + l], lc[2 * TC + l]};
+    };
+    auto ccof = [&](int c) -> v3<T> {
+        if (c == out) return occ;
+        const int l = c - c0;
+        return v3<T>{lc[3 * TC + l], lc[4 * TC + l], lc[5 * TC + l]};
+    };
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
+        if (sl != tid) load(sl);
+        T *q = lt + sl;
+        const T z = nzero<T>();
+        if (cn.y < 0) {
+            for (int j = 0; j < NT; j++) q[j * S] = z;
+            continue;
+        }
+        const v3<T> hp = hof(cn.x), hn = hof(cn.y);
+        const v3<T> hs{hp.x + hn.x, hp.y + hn.y, hp.z + hn.z};
+        const T pF = dot3(hs, v3<T>{s.x, s.y, s.z});
+        const T facn = s.w;
+        const T facp = 1.0 - facn;
+        const T tp = facp * pF, tn = facn * pF;
+        q[0] = tp;
+        q[S] = tn;
+        if (CORR) {
+            const v3<T> xac = lerp3(ccof(cn.y), facn, ccof(cn.x), facp);
+            const v3<T> corr = sub3(xyz(xf), xac);
+            q[2 * S] = tp * corr.x, q[3 * S] = tp * corr.y, q[4 * S] = tp * corr.z;
+            q[5 * S] = tn * corr.x, q[6 * S] = tn * corr.y, q[7 * S] = tn * corr.z;
+        }
+    }
+    __syncthreads();
+    if (fold) {
+        for (int e = e0; e < e0 + n; e++) {
+            const int es = le[e - eb];
+            const int sl = es & 0x7fff;
+            const bool side2 = (es & 0x8000) != 0;
+            const int row = k == 0 ? (side2 ? 1 : 0) : (side2 ? 4 : 1) + k;
+            acc = acc + lt[row * S + sl];
+        }
+        const int c = c0 + cl;
+        if (k == 0) a.phibar[c] = acc;
+        else a.hnext[3 * (size_t)c + (k - 1)] = acc * (1.0 / m.cvol[c]);
+    }
+}
+
 // Typed meshes: the cotangent of the PhiFace each boundary face k = g - Fi
 // leaves is the sum, in ascending mesh face id, of its readers' terms
 // (readers rd[rd_off[k] .. rd_off[k + 1])): an interior reader's
 // -((D*GammaBlend)*mf), an INLET's that carries PhiFace -(SuBar[P]*f).
-// k_adj_carry_terms evaluates every reader's term into coef[e] (one thread
-// per reader entry); k_adj_carry_sum adds each source's run of coef in order.
+// k_adj_carry_terms evaluates every face's term in face order (coalesced;
+// 0 for faces that read no carried value), k_adj_carry_gather puts them in
+// reader-list order, and k_adj_carry_sum adds each source's contiguous run of
+// them in order.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_carry_terms(DevMesh<T> m, AdjArgs<T> a, const int *rd, int n,
-                                                            T *coef) {
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n) return;
-    const int r = rd[e];
+__global__ __launch_bounds__(kBlock) void k_adj_carry_terms(DevMesh<T> m, AdjArgs<T> a, T *coef) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= m.F) return;
     const int2 cn = m.fconn[r];
     const T mf = a.s.massFlux[r];
-    T c;
+    T c = 0.0;
     if (cn.y >= 0) {
         const T D = a.subar[cn.x] - a.subar[cn.y];
         const T Dg = D * a.s.gammaBlend;
         c = -(Dg * mf);
-    } else {
+    } else if (!a.s.inletAssigns && m.rtype[m.bregion[~cn.y]] == CUPCFD_RTYPE_INLET) {
         const T Visac = scalar_visac(a.s, a.s.visEffB[~cn.y]);
         const T VisFace = Visac * m.frl[r];
         const T f = -VisFace + (T)fmin((double)mf, 0.0);
         c = -(a.subar[cn.x] * f);
     }
-    coef[e] = c;
+    coef[r] = c;
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_carry_sum(int nb, const int *rd_off, const T *coef, T *pfbar) {
+__global__ __launch_bounds__(kBlock) void k_adj_carry_gather(int n, const int *rd, const T *coef, T *coefr) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e < n) coefr[e] = coef[rd[e]];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_carry_sum(int nb, const int *rd_off, const T *coefr, T *pfbar) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nb) return;
     const int e1 = rd_off[k + 1];
     T acc = 0.0;
     int e = rd_off[k];
-    for (; e + 4 <= e1; e += 4) {  // four independent loads, then the adds in order
-        const T c0 = coef[e], c1 = coef[e + 1], c2 = coef[e + 2], c3 = coef[e + 3];
+    for (; e + 4 <= e1; e += 4) {  // four loads issued, then the adds in order
+        const T c0 = coefr[e], c1 = coefr[e + 1], c2 = coefr[e + 2], c3 = coefr[e + 3];
         acc = acc + c0;
         acc = acc + c1;
         acc = acc + c2;
         acc = acc + c3;
     }
-    for (; e < e1; e++) acc = acc + coef[e];
+    for (; e < e1; e++) acc = acc + coefr[e];
     pfbar[k] = acc;
 }
 
@@ -3599,15 +3814,22 @@ void adj_launch_b(int lpc, const Grid &gr, int nthr, hipStream_t st, const DevMe
     else hipLaunchKernelGGL((k_adj_b<T, CORR>), gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
 }
 
-// Pass A, then pass B once per gradient sweep, last sweep first, on the
-// mesh's lane schedule. a.h receives the last sweep's h; hb is a second
-// buffer of 3C for nGradient > 1 (the sweeps alternate between the two).
+// Pass A, then pass B once per gradient sweep, last sweep first: on the cell
+// tiles when the mesh has them and its schedule is automatic or tiles, else
+// on the mesh's lane schedule. a.h receives the last sweep's h; hb is a
+// second buffer of 3C for nGradient > 1 (the sweeps alternate between the two).
 template <class T> int adj_passes(const Mesh<T> &M, AdjArgs<T> a, int nGradient, T *hb, hipStream_t st) {
     const DevMesh<T> &d = M.d;
+    const bool tiles = (M.schedule_lpc == 0 || M.schedule_lpc == kTile) && d.ntiles > 0 &&
+                       tile_lds(d, 6, 8, false) > 0;
     const int lpc = adj_lanes(M);
     const Grid gr = xcd_grid(d.C, lpc ? kCellsPerBlock : kBlock);
     const int nthr = lpc ? lpc * kCellsPerBlock : kBlock;
-    if (lpc == 8) hipLaunchKernelGGL((k_adj_a_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
+    const Grid tg = xcd_grid(d.ntiles, 1);
+    if (tiles)
+        hipLaunchKernelGGL(k_adj_a_tile<T>, tg.blocks, kTileBlock, tile_lds(d, 6, 8, false), st, d, a, tg.per,
+                           tg.nblk);
+    else if (lpc == 8) hipLaunchKernelGGL((k_adj_a_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
     else if (lpc == 6) hipLaunchKernelGGL((k_adj_a_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
     else if (lpc == 4) hipLaunchKernelGGL((k_adj_a_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
     else hipLaunchKernelGGL(k_adj_a<T>, gr.blocks, kBlock, 0, st, d, a, gr.per, gr.nblk);
@@ -3617,7 +3839,13 @@ template <class T> int adj_passes(const Mesh<T> &M, AdjArgs<T> a, int nGradient,
         const int cur = (nGradient - s) & 1;
         a.h = hbuf[cur];
         a.hnext = hbuf[cur ^ 1];
-        if (s > 1) adj_launch_b<T, true>(lpc, gr, nthr, st, d, a);
+        if (tiles && s > 1)
+            hipLaunchKernelGGL((k_adj_b_tile<T, true>), tg.blocks, kTileBlock, tile_lds(d, 6, 8, false), st, d, a,
+                               tg.per, tg.nblk);
+        else if (tiles)
+            hipLaunchKernelGGL((k_adj_b_tile<T, false>), tg.blocks, kTileBlock, tile_lds(d, 3, 2, false), st, d, a,
+                               tg.per, tg.nblk);
+        else if (s > 1) adj_launch_b<T, true>(lpc, gr, nthr, st, d, a);
         else adj_launch_b<T, false>(lpc, gr, nthr, st, d, a);
         a.binit = a.phibar;
     }
@@ -3695,12 +3923,17 @@ int dev_scalar_adjoint_full(cupcfd_hip_mesh *h, int nGradient, const T *Phi, con
             scalar_carry_readers<T>(*M, a.s.inletAssigns != 0, a.s.wallActive != 0, &rd_off, &rd, &nrd) !=
                 CUPCFD_E_SUCCESS)
             return CUPCFD_E_ERROR;
-        T *coef = (T *)M->ws_get(62, (size_t)(nrd ? nrd : 1) * sizeof(T));
-        if (!coef) return CUPCFD_E_ERROR;
+        T *coef = (T *)M->ws_get(62, (size_t)d.F * sizeof(T));
+        T *coefr = (T *)M->ws_get(63, (size_t)(nrd ? nrd : 1) * sizeof(T));
+        if (!coef || !coefr) return CUPCFD_E_ERROR;
         hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a.s, cval);
         a.s.cval = cval;
-        if (nrd) hipLaunchKernelGGL(k_adj_carry_terms<T>, dim3(blocks(nrd)), dim3(kBlock), 0, st, d, a, rd, nrd, coef);
-        hipLaunchKernelGGL(k_adj_carry_sum<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, nb, rd_off, (const T *)coef,
+        if (nrd) {
+            hipLaunchKernelGGL(k_adj_carry_terms<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, coef);
+            hipLaunchKernelGGL(k_adj_carry_gather<T>, dim3(blocks(nrd)), dim3(kBlock), 0, st, nrd, rd,
+                               (const T *)coef, coefr);
+        }
+        hipLaunchKernelGGL(k_adj_carry_sum<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, nb, rd_off, (const T *)coefr,
                            pfbar);
         hipLaunchKernelGGL(k_adj_bnd<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, (const T *)pfbar, bterm);
         a.bterm = bterm;
