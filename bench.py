@@ -585,8 +585,8 @@ def bench_all_kernels(torch, n, steps, warmup, schedule):
     stream = torch.cuda.current_stream()
     quiesce()
     out = {}
-    for name in order:
-        fn = calls[name]
+
+    def timed(fn):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -596,13 +596,27 @@ def bench_all_kernels(torch, n, steps, warmup, schedule):
             fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        t = e0.elapsed_time(e1) / 1e3 / steps
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    for name in order:
+        t = timed(calls[name])
         gbs = nbytes[name] / t / 1e9
         out[name] = {"us": t * 1e6, "algorithmic_bytes": nbytes[name], "algorithmic_GBps": gbs,
                      "hbm_frac": gbs / HBM_PEAK_GBS, "bound": "hbm" if nbytes[name] > (32 << 20) else "latency"}
+    # the loops with ordered sums again, their sums taken as a fixed-shape parallel tree (not the reference's
+    # bits; the tests bound the difference)
+    tree = {}
+    dev.set_ordered_sums(False)
+    try:
+        for name in ("mass_bl1", "mass_bl2", "mass_bl3", "mass_bl4", "mass_bl5", "uvw_bnds1"):
+            tree[name] = {"us": timed(calls[name]) * 1e6, "sequential_us": out[name]["us"]}
+    finally:
+        dev.set_ordered_sums(True)
     return {"workload": f"the 16 BenchmarkKernels kernels, resident, one at a time, StructGen {n}^3 relabelled "
                         f"with INLET/OUTLET/WALL/SYMP regions, FP64", "cells": C, "boundaries": B,
-            "repetition_us": sum(v["us"] for v in out.values()), "kernels": out}
+            "repetition_us": sum(v["us"] for v in out.values()), "kernels": out,
+            "tree_sums": {"repetition_us": sum(v["us"] for v in out.values())
+                          + sum(v["us"] - v["sequential_us"] for v in tree.values()), "kernels": tree}}
 
 
 def bench_batched(torch, host, dev, K, steps, warmup, label, oracle_sample=None):

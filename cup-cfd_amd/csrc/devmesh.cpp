@@ -835,6 +835,13 @@ extern "C" int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double valu
     return CUPCFD_E_SUCCESS;
 }
 
+extern "C" int cupcfd_hip_mesh_set_ordered_sums(cupcfd_hip_mesh *mesh, int32_t sequential) {
+    if (!mesh) return CUPCFD_E_NULL_PTR;
+    if (mesh->precision == 64) mesh->m64->d.tree_sums = sequential ? 0 : 1;
+    else mesh->m32->d.tree_sums = sequential ? 0 : 1;
+    return CUPCFD_E_SUCCESS;
+}
+
 extern "C" int cupcfd_hip_mesh_push_state(cupcfd_hip_mesh *mesh) {
     if (!mesh) return CUPCFD_E_NULL_PTR;
     return sync_state_any(mesh, true);

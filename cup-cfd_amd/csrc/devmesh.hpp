@@ -71,6 +71,9 @@ template <class T> struct DevMesh {
     const int *carry_src;
     int typed;
     T carry_init;
+    // boundary sums in the reference's sequential order (0, the default) or as
+    // a fixed-shape parallel tree (1; cupcfd_hip_mesh_set_ordered_sums)
+    int tree_sums;
     // cells that own at least one boundary face
     const int *bcells;
     int nbc;

@@ -151,6 +151,39 @@ template <class T, int NV, class Get> __device__ T block_ordered_sum(T init, int
     return acc;
 }
 
+// The same sums as a fixed-shape parallel tree (the mesh's tree_sums mode):
+// thread i adds values i, i + kSeqBlock, ... in order, each wave combines its
+// threads with a fixed xor tree, and one thread per component adds the waves
+// in order; init is added last. The same bits on every call, not the
+// sequential sum's.
+template <class T, int NV, class Get> __device__ T block_tree_sum(T init, int n, Get get, T *buf) {
+    constexpr int NW = kSeqBlock / 64;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    T p[NV];
+    for (int c = 0; c < NV; c++) p[c] = 0;
+    for (int i = tid; i < n; i += kSeqBlock) {
+        T v[NV];
+        get(i, v);
+        for (int c = 0; c < NV; c++) p[c] = p[c] + v[c];
+    }
+    for (int c = 0; c < NV; c++) {
+        for (int o = 32; o >= 1; o >>= 1) p[c] = p[c] + __shfl_xor(p[c], o, 64);
+        if (lane == 0) buf[c * NW + wave] = p[c];
+    }
+    __syncthreads();
+    if (tid < NV) {
+        T s = buf[tid * NW];
+        for (int w = 1; w < NW; w++) s = s + buf[tid * NW + w];
+        buf[NV * NW + tid] = s;
+    }
+    __syncthreads();
+    return wave < NV ? init + buf[NV * NW + wave] : init;
+}
+
+template <class T, int NV, class Get> __device__ T block_sum(int tree, T init, int n, Get get, T *buf) {
+    return tree ? block_tree_sum<T, NV>(init, n, get, buf) : block_ordered_sum<T, NV>(init, n, get, buf);
+}
+
 // ===========================================================================
 // Per-cell gathers: the face loops that scatter into both cells of a face
 // (gradient, scalar, UVW) are evaluated per cell over its CSR row.
@@ -642,8 +675,8 @@ template <class T>
 __global__ __launch_bounds__(kSeqBlock) void k_mass_bl1(DevMesh<T> m, const T *__restrict__ mf, T *flowin) {
     __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_INLET];
-    const T s = block_ordered_sum<T, 1>(
-        T(0.0), m.nblist[CUPCFD_RTYPE_INLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
+    const T s = block_sum<T, 1>(m.tree_sums, T(0.0), m.nblist[CUPCFD_RTYPE_INLET],
+                                [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
     if (threadIdx.x == 0) *flowin = s;
 }
 
@@ -657,16 +690,16 @@ __global__ __launch_bounds__(kSeqBlock) void k_mass_bl2(DevMesh<T> m, const T *_
     __shared__ T buf[2 * kSeqLds];
     if (blockIdx.x == 0) {
         const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
-        const T s = block_ordered_sum<T, 1>(
-            T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
+        const T s = block_sum<T, 1>(m.tree_sums, T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET],
+                                    [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
         if (threadIdx.x == 0) *flowout = s;
         return;
     }
     const int r = blockIdx.x - 1;
     if (m.rtype[r] != CUPCFD_RTYPE_OUTLET) return;
     const int b0 = roff[r];
-    const T s = block_ordered_sum<T, 1>(
-        flowRegion[r], roff[r + 1] - b0, [&](int i, T *v) { v[0] = mf[m.bface[rbnd[b0 + i]]]; }, buf);
+    const T s = block_sum<T, 1>(m.tree_sums, flowRegion[r], roff[r + 1] - b0,
+                                [&](int i, T *v) { v[0] = mf[m.bface[rbnd[b0 + i]]]; }, buf);
     if (threadIdx.x == 0) flowRegion[r] = s;
 }
 
@@ -674,8 +707,8 @@ __global__ __launch_bounds__(kSeqBlock) void k_mass_bl2(DevMesh<T> m, const T *_
 template <class T> __global__ __launch_bounds__(kSeqBlock) void k_mass_bl3(DevMesh<T> m, T flowin, T *ratearea) {
     __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
-    const T areaout = block_ordered_sum<T, 1>(
-        T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = m.fgeoX[m.bface[lst[i]]].w; }, buf);
+    const T areaout = block_sum<T, 1>(m.tree_sums, T(0.0), m.nblist[CUPCFD_RTYPE_OUTLET],
+                                      [&](int i, T *v) { v[0] = m.fgeoX[m.bface[lst[i]]].w; }, buf);
     if (threadIdx.x == 0) *ratearea = -flowin / areaout;
 }
 
@@ -708,8 +741,8 @@ __global__ __launch_bounds__(kSeqBlock) void k_outlet_sum(DevMesh<T> m, const T 
     __shared__ T buf[2 * kSeqLds];
     const int *lst = m.blist[CUPCFD_RTYPE_OUTLET];
     const T a0 = from_zero ? T(0.0) : *acc;
-    const T s = block_ordered_sum<T, 1>(
-        a0, m.nblist[CUPCFD_RTYPE_OUTLET], [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
+    const T s = block_sum<T, 1>(m.tree_sums, a0, m.nblist[CUPCFD_RTYPE_OUTLET],
+                                [&](int i, T *v) { v[0] = mf[m.bface[lst[i]]]; }, buf);
     if (threadIdx.x == 0) *acc = s;
 }
 
@@ -1654,8 +1687,8 @@ __global__ __launch_bounds__(kSeqBlock) void k_uvw_bnds1(DevMesh<T> m, const int
     const int r = blockIdx.x;
     if (m.rtype[r] != CUPCFD_RTYPE_WALL) return;
     const int b0 = roff[r], c = min((int)threadIdx.x >> 6, 2);
-    const T s = block_ordered_sum<T, 3>(
-        m.rforcet[3 * r + c], roff[r + 1] - b0,
+    const T s = block_sum<T, 3>(
+        m.tree_sums, m.rforcet[3 * r + c], roff[r + 1] - b0,
         [&](int i, T *v) {
             const v3<T> x = ld3((const T *)m.bshear, rbnd[b0 + i]);
             v[0] = x.x, v[1] = x.y, v[2] = x.z;

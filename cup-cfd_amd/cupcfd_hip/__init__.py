@@ -114,7 +114,8 @@ HOST_SYMBOLS = [
     "cupcfd_structgen_create", "cupcfd_host_mesh_view_f64", "cupcfd_host_mesh_view_f32",
     "cupcfd_host_mesh_destroy", "cupcfd_hdf5_mesh_create", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
-    "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_schedule",
+    "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_ordered_sums",
+    "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
     "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field", "cupcfd_hip_mesh_generate_f64",
     "cupcfd_hip_mesh_generate_f32", "cupcfd_hip_mesh_face_colours", "cupcfd_hip_autotune_forget",
@@ -176,6 +177,7 @@ def _declare(L):
     L.cupcfd_hip_mesh_face_order.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_set_carry_init.argtypes = [ct.c_void_p, ct.c_double]
     L.cupcfd_hip_mesh_set_schedule.argtypes = [ct.c_void_p, ct.c_int32]
+    L.cupcfd_hip_mesh_set_ordered_sums.argtypes = [ct.c_void_p, ct.c_int32]
     L.cupcfd_hip_mesh_autotune.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_push_state.argtypes = [ct.c_void_p]
     L.cupcfd_hip_mesh_pull_state.argtypes = [ct.c_void_p]
@@ -476,6 +478,11 @@ class DeviceMesh:
 
     def set_carry_init(self, v):
         check("set_carry_init", lib().cupcfd_hip_mesh_set_carry_init(self._h, float(v)))
+
+    def set_ordered_sums(self, sequential=True):
+        """Boundary-loop sums in the reference's sequential order (True, the default: its bits) or as a
+        fixed-shape parallel tree (False: faster, the same bits every call, not the sequential sum's)."""
+        check("set_ordered_sums", lib().cupcfd_hip_mesh_set_ordered_sums(self._h, 1 if sequential else 0))
 
     def autotune(self):
         """Time the gather schedules of gradient, scalar and UVW on this mesh and keep the fastest of each;

@@ -207,6 +207,16 @@ int cupcfd_hip_mesh_counts(const cupcfd_hip_mesh *mesh, int32_t *nCells, int32_t
 int cupcfd_hip_mesh_cell_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_face_order(const cupcfd_hip_mesh *mesh, int32_t *order);
 int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
+/* How the boundary loops add their per-boundary values (FluxMassDolfyn
+ * BoundaryLoop1-5's flow and area sums, FluxUVWDolfynBndsLoop1's force
+ * tangents; no counterpart in the reference, whose loops are sequential):
+ * sequential != 0 (the default) adds them one after another in boundary
+ * order, as the reference does, so the sums are its bits; sequential == 0
+ * adds them in a fixed-shape parallel tree (one workgroup: a strided partial
+ * sum per thread, then a fixed reduction tree), which is much faster for long
+ * boundary lists and gives the same bits on every call, within a few ulps of
+ * the sum's magnitude of the sequential result rather than equal to it. */
+int cupcfd_hip_mesh_set_ordered_sums(cupcfd_hip_mesh *mesh, int32_t sequential);
 /* Gather schedule of the per-cell face loops (gradient, scalar, UVW): 64 for
  * tiles of 64 cells whose values are staged in LDS and whose faces are each
  * evaluated once; lanes evaluating one cell's faces in parallel (4, 6 or 8; a
