@@ -2088,11 +2088,13 @@ template <class T> struct MeshSetArgs {
 // Batched gradient (nGradient = 1) over every mesh of a set in one persistent
 // launch: logical block lb (XCD-aware, as xcd_block) takes the contiguous
 // unit range [lb*U/P, (lb+1)*U/P) of the U units of all meshes and leaves
-// after its last unit. phi[i] .. out2[i] are mesh i's arrays.
+// after its last unit. phi[i] .. out2[i] are mesh i's arrays. The mesh
+// descriptor and the walk's state want about 106 SGPRs, which would allow 7
+// waves per SIMD; waves_per_eu(8) has the compiler spill the excess into VGPR
+// lanes instead (no scratch), for the 8 waves the per-mesh kernel runs.
 template <class T, class TI, class TO, int G, int RL>
-__global__ __launch_bounds__(kBlock) void k_gradient_batched_set(MeshSetArgs<T> s, const TI *const *phi,
-                                                                 const TI *const *phiB, TO *const *out1,
-                                                                 TO *const *out2, int K) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_gradient_batched_set(
+    MeshSetArgs<T> s, const TI *const *phi, const TI *const *phiB, TO *const *out1, TO *const *out2, int K) {
     const int P = gridDim.x;
     const int lb = xcd_block(blockIdx.x, P / kXcd);
     const int u0 = (int)((int64_t)lb * s.nunits / P), u1 = (int)((int64_t)(lb + 1) * s.nunits / P);
