@@ -55,6 +55,13 @@ template <class T> __device__ __forceinline__ v3<T> ld3s(const T *p, int i, int 
     const size_t b = (size_t)3 * i * ks;
     return v3<T>{p[b], p[b + ks], p[b + 2 * (size_t)ks]};
 }
+// a 3-vector of storage type S widened to T. Batched kernels keep gathered
+// values in their storage type until first use: a conversion right after a
+// load inside a branch makes the wave wait for that load before it issues the
+// next, so a row's gathers would run one memory round trip at a time.
+template <class T, class S> __device__ __forceinline__ v3<T> widen3(v3<S> v) {
+    return v3<T>{(T)v.x, (T)v.y, (T)v.z};
+}
 template <class T> __device__ __forceinline__ void st3(T *__restrict__ p, int i, v3<T> v) {
     p[3 * (size_t)i] = v.x;
     p[3 * (size_t)i + 1] = v.y;
@@ -2032,14 +2039,14 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched_staged(DevMesh<T> m
     __syncthreads();
     const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
     const bool live = c < m.C && kk < K;
-    T phiC = 0, ov[RL];
+    TI phiCs = 0, ov[RL];  // storage type until first use
     if (live) {
         const TI *phi = a.phi + kk, *phiB = a.phiB + kk;
-        phiC = (T)phi[(size_t)c * K];
+        phiCs = phi[(size_t)c * K];
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             const int2 en = sen[cl * RL + i];
-            ov[i] = en.y >= 0 ? (T)phi[(size_t)en.y * K] : (T)phiB[(size_t)(~en.y) * K];
+            ov[i] = *(en.y >= 0 ? phi + (size_t)en.y * K : phiB + (size_t)(~en.y) * K);
         }
     }
     for (int p = tid; p < NP; p += kBlock)
@@ -2048,7 +2055,7 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched_staged(DevMesh<T> m
         if (cb + p < m.C) svol[p] = m.cvol[cb + p];
     __syncthreads();
     if (!live) return;
-    const GradCell<T> own{phiC, {}, {}};
+    const GradCell<T> own{(T)phiCs, {}, {}};
     T acc[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 0; i < RL; i++) {
@@ -2056,13 +2063,14 @@ __global__ __launch_bounds__(kBlock) void k_gradient_batched_staged(DevMesh<T> m
         const V4<T> s = ss[cl * RL + i];
         T t[3];
         if (en.y >= 0) {
-            const GradCell<T> o{ov[i], {}, {}};
+            const GradCell<T> o{(T)ov[i], {}, {}};
             const bool in = (en.x & 1) != 0;
             gradient_interior<T, true>(s, s, in ? o : own, in ? own : o, t);
         } else {
-            t[0] = ov[i] * s.x;
-            t[1] = ov[i] * s.y;
-            t[2] = ov[i] * s.z;
+            const T ob = (T)ov[i];
+            t[0] = ob * s.x;
+            t[1] = ob * s.y;
+            t[2] = ob * s.z;
         }
         acc[0] = acc[0] + t[0];
         acc[1] = acc[1] + t[1];
@@ -2178,22 +2186,24 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_plain(DevMesh<T> m, S
     const int c = batch_uniform<G>(L.c);
     const TS *phi = a.phi + L.k, *dphi = a.dPhidx + L.k;
     TS *sup = a.su + L.k;
-    const T phiC = (T)phi[(size_t)c * K];
-    const v3<T> gC = ld3s_as<T>(dphi, c, K);
-    T su = (T)sup[(size_t)c * K];
+    const TS phiCs = phi[(size_t)c * K];
+    const v3<TS> gCs = ld3s(dphi, c, K);
+    const TS sus = sup[(size_t)c * K];
     const int e0 = c * RL;
     int2 en[RL];
-    T ophi[RL];
-    v3<T> og[RL];
+    TS ophi[RL];  // storage type until first use; a boundary entry gathers the cell's own row
+    v3<TS> og[RL];
 #pragma unroll
     for (int i = 0; i < RL; i++) en[i] = m.ent[e0 + i];
 #pragma unroll
     for (int i = 0; i < RL; i++) {
-        if (en[i].y >= 0) {
-            ophi[i] = (T)phi[(size_t)en[i].y * K];
-            og[i] = ld3s_as<T>(dphi, en[i].y, K);
-        }
+        const int o = en[i].y >= 0 ? en[i].y : c;
+        ophi[i] = phi[(size_t)o * K];
+        og[i] = ld3s(dphi, o, K);
     }
+    const T phiC = (T)phiCs;
+    const v3<T> gC = widen3<T>(gCs);
+    T su = (T)sus;
 #pragma unroll
     for (int i = 0; i < RL; i++) {
         if (en[i].y < 0) continue;
@@ -2201,8 +2211,10 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_plain(DevMesh<T> m, S
         const V4<T> s = m.fgeoS[f];
         const ScalarFaceCoef<T> q = coef[f];
         const bool in = (en[i].x & 1) != 0;
-        const T phiP = in ? ophi[i] : phiC, phiN = in ? phiC : ophi[i];
-        const v3<T> gP = in ? og[i] : gC, gN = in ? gC : og[i];
+        const T oph = (T)ophi[i];
+        const v3<T> ogi = widen3<T>(og[i]);
+        const T phiP = in ? oph : phiC, phiN = in ? phiC : oph;
+        const v3<T> gP = in ? ogi : gC, gN = in ? gC : ogi;
         const v3<T> S{s.x, s.y, s.z};
         const T facn = s.w;
         const T facp = 1.0 - facn;
@@ -2249,19 +2261,18 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_staged(DevMesh<T> m, 
     const bool live = c < m.C && kk < K;
     const TS *phi = a.phi + kk, *dphi = a.dPhidx + kk;
     TS *sup = a.su + kk;
-    T phiC = 0, su = 0, ophi[RL];
-    v3<T> gC{}, og[RL];
+    TS phiCs = 0, sus = 0, ophi[RL];  // storage type until first use
+    v3<TS> gCs{}, og[RL];
     if (live) {
-        phiC = (T)phi[(size_t)c * K];
-        gC = ld3s_as<T>(dphi, c, K);
-        su = (T)sup[(size_t)c * K];
+        phiCs = phi[(size_t)c * K];
+        gCs = ld3s(dphi, c, K);
+        sus = sup[(size_t)c * K];
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             const int2 en = sen[cl * RL + i];
-            if (en.y >= 0) {
-                ophi[i] = (T)phi[(size_t)en.y * K];
-                og[i] = ld3s_as<T>(dphi, en.y, K);
-            }
+            const int o = en.y >= 0 ? en.y : c;  // a boundary entry gathers the cell's own row
+            ophi[i] = phi[(size_t)o * K];
+            og[i] = ld3s(dphi, o, K);
         }
     }
     for (int p = tid; p < NP; p += kBlock) {
@@ -2274,6 +2285,9 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_staged(DevMesh<T> m, 
     }
     __syncthreads();
     if (!live) return;
+    const T phiC = (T)phiCs;
+    const v3<T> gC = widen3<T>(gCs);
+    T su = (T)sus;
 #pragma unroll
     for (int i = 0; i < RL; i++) {
         const int2 en = sen[cl * RL + i];
@@ -2281,8 +2295,10 @@ __global__ __launch_bounds__(kBlock) void k_scalar_batched_staged(DevMesh<T> m, 
         const V4<T> s = ss[cl * RL + i];
         const ScalarFaceCoef<T> q = sq[cl * RL + i];
         const bool in = (en.x & 1) != 0;
-        const T phiP = in ? ophi[i] : phiC, phiN = in ? phiC : ophi[i];
-        const v3<T> gP = in ? og[i] : gC, gN = in ? gC : og[i];
+        const T oph = (T)ophi[i];
+        const v3<T> ogi = widen3<T>(og[i]);
+        const T phiP = in ? oph : phiC, phiN = in ? phiC : oph;
+        const v3<T> gP = in ? ogi : gC, gN = in ? gC : ogi;
         const v3<T> S{s.x, s.y, s.z};
         const T facn = s.w;
         const T facp = 1.0 - facn;
@@ -2955,19 +2971,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched(DevMesh<T> m, AdjBatch
         acc[3] = acc[3] + Dg * (in ? q.fmn : q.fmx);
     };
     if (RL) {
-        const T sc = (T)a.subar[(size_t)c * K + kk];
+        const TS scs = a.subar[(size_t)c * K + kk];
         int2 en[RL > 0 ? RL : 1];
-        T so[RL > 0 ? RL : 1];
+        TS so[RL > 0 ? RL : 1];  // storage type until first use; a boundary entry gathers the cell's own row
 #pragma unroll
         for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
 #pragma unroll
-        for (int i = 0; i < RL; i++)
-            if (en[i].y >= 0) so[i] = (T)a.subar[(size_t)en[i].y * K + kk];
+        for (int i = 0; i < RL; i++) so[i] = a.subar[(size_t)(en[i].y >= 0 ? en[i].y : c) * K + kk];
+        const T sc = (T)scs;
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             if (en[i].y < 0) continue;
             const bool in = (en[i].x & 1) != 0;
-            term(en[i], in ? so[i] : sc, in ? sc : so[i]);
+            const T sv = (T)so[i];
+            term(en[i], in ? sv : sc, in ? sc : sv);
         }
     } else {
         int e0, n;
@@ -3052,20 +3069,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBa
     if (c < m.C) {
         const bool live = kk < K;
         const T PhiFace = m.carry_init;
-        T sc = 0, pc = 0, so[RL], po[RL];
+        TS scs = 0, pcs = 0, so[RL], po[RL];  // storage type until first use
         int2 en[RL];
 #pragma unroll
         for (int i = 0; i < RL; i++) en[i] = m.ent[c * RL + i];
         if (live) {
-            sc = (T)a.subar[(size_t)c * K + kk];
-            pc = (T)a.s.phi[(size_t)c * K + kk];
+            scs = a.subar[(size_t)c * K + kk];
+            pcs = a.s.phi[(size_t)c * K + kk];
 #pragma unroll
             for (int i = 0; i < RL; i++) {
-                if (en[i].y < 0) continue;
-                so[i] = (T)a.subar[(size_t)en[i].y * K + kk];
-                if (!(en[i].x & 1)) po[i] = (T)a.s.phi[(size_t)en[i].y * K + kk];
+                const int o = en[i].y >= 0 ? en[i].y : c;  // a boundary entry gathers the cell's own row
+                so[i] = a.subar[(size_t)o * K + kk];
+                if (!(en[i].x & 1)) po[i] = a.s.phi[(size_t)o * K + kk];
             }
         }
+        const T sc = (T)scs, pc = (T)pcs;
         T acc[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < RL; i++) {
@@ -3076,7 +3094,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBa
                 const AdjFaceCoef<T> q = a.coef[f];
                 const T facn = m.fgeoS[f].w;
                 const T facp = 1.0 - facn;
-                const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
+                const T sv = (T)so[i];
+                const T D = (in ? sv : sc) - (in ? sc : sv);
                 const T Dg = D * a.s.gammaBlend;
                 const T fac = in ? facn : facp;
                 const T tt = D * fac;
@@ -3087,7 +3106,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_mf(DevMesh<T> m, AdjBa
                 if (!in) {
                     const T mf = a.s.massFlux[f];
                     const bool up = mf < 0 || mf > 0;
-                    const T dfci = up ? (mf < 0 ? po[i] : pc) : (T)0.0;
+                    const T dfci = up ? (mf < 0 ? (T)po[i] : pc) : (T)0.0;
                     t = -(Dg * (PhiFace - dfci));
                 }
             }
@@ -3147,17 +3166,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
     __syncthreads();
     const int cl = tid / G, c = cb + cl, kk = blockIdx.y * G + tid % G;
     const bool live = c < m.C && kk < K;
-    T sc = 0, pc = 0, so[RL], po[RL];
+    TS scs = 0, pcs = 0, so[RL], po[RL];  // storage type until first use
     if (live) {
-        sc = (T)a.subar[(size_t)c * K + kk];
-        if (MF) pc = (T)a.s.phi[(size_t)c * K + kk];
+        scs = a.subar[(size_t)c * K + kk];
+        if (MF) pcs = a.s.phi[(size_t)c * K + kk];
 #pragma unroll
         for (int i = 0; i < RL; i++) {
             const int2 en = sen[cl * RL + i];
-            if (en.y >= 0) {
-                so[i] = (T)a.subar[(size_t)en.y * K + kk];
-                if (MF && !(en.x & 1)) po[i] = (T)a.s.phi[(size_t)en.y * K + kk];
-            }
+            const int o = en.y >= 0 ? en.y : c;  // a boundary entry gathers the cell's own row
+            so[i] = a.subar[(size_t)o * K + kk];
+            if (MF && !(en.x & 1)) po[i] = a.s.phi[(size_t)o * K + kk];
         }
     }
     for (int p = tid; p < NP; p += kBlock) {
@@ -3173,6 +3191,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
     __syncthreads();
     if (live) {
         const T PhiFace = m.carry_init;
+        const T sc = (T)scs, pc = (T)pcs;
         T acc[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < RL; i++) {
@@ -3182,7 +3201,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
             const AdjFaceCoef<T> q = sq[cl * RL + i];
             const T facn = sfn[cl * RL + i];
             const T facp = 1.0 - facn;
-            const T D = (in ? so[i] : sc) - (in ? sc : so[i]);
+            const T sv = (T)so[i];
+            const T D = (in ? sv : sc) - (in ? sc : sv);
             const T Dg = D * a.s.gammaBlend;
             const T fac = in ? facn : facp;
             const T tt = D * fac;
@@ -3193,7 +3213,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_a_batched_staged(DevMesh<T> m, A
             if (MF && !in) {
                 const T mf = a.s.massFlux[en.x >> 1];
                 const bool up = mf < 0 || mf > 0;
-                const T dfci = up ? (mf < 0 ? po[i] : pc) : (T)0.0;
+                const T dfci = up ? (mf < 0 ? (T)po[i] : pc) : (T)0.0;
                 lt[(cl * RL + i) * LS + tid % G] = -(Dg * (PhiFace - dfci));
             }
         }
