@@ -789,6 +789,7 @@ int generate_entry(const cupcfd_structgen_params *params, int32_t flags, cupcfd_
         delete hm;
         return rc == CUPCFD_E_SUCCESS ? CUPCFD_E_ERROR : rc;
     }
+    tile_split_build(*M);
     if (host_out) *host_out = hm;
     *out = h;
     return CUPCFD_E_SUCCESS;

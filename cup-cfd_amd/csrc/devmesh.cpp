@@ -24,6 +24,8 @@ template <class T> Mesh<T>::~Mesh() {
     if (rbnd_off) (void)hipFree(rbnd_off);
     if (cell_order_d) (void)hipFree(cell_order_d);
     if (face_order_d) (void)hipFree(face_order_d);
+    if (uvw_col) (void)hipFree(uvw_col);
+    if (uvw_eslot) (void)hipFree(uvw_eslot);
 }
 
 template <class T> void *Mesh<T>::ws_get(int slot, size_t bytes) {
@@ -635,6 +637,7 @@ int create_mesh(const View *view, int32_t flags, cupcfd_hip_mesh **out, bool on_
         M->*keep = *view;
         rc = sync_state(*M, *view, true);
     }
+    if (rc == CUPCFD_E_SUCCESS) tile_split_build(*M);
     auto *h = rc == CUPCFD_E_SUCCESS ? new (std::nothrow) cupcfd_hip_mesh{} : nullptr;
     if (!h) {
         delete M;
@@ -839,6 +842,20 @@ extern "C" int cupcfd_hip_mesh_set_ordered_sums(cupcfd_hip_mesh *mesh, int32_t s
     if (!mesh) return CUPCFD_E_NULL_PTR;
     if (mesh->precision == 64) mesh->m64->d.tree_sums = sequential ? 0 : 1;
     else mesh->m32->d.tree_sums = sequential ? 0 : 1;
+    return CUPCFD_E_SUCCESS;
+}
+
+namespace {
+template <class T> void tile_layout(const Mesh<T> &M, int32_t *info) {
+    const int32_t v[6] = {M.d.ntiles, M.d.tslots, M.d.tnnz, M.split_na, M.split_nb, M.uvw_split};
+    for (int i = 0; i < 6; i++) info[i] = v[i];
+}
+}  // namespace
+
+extern "C" int cupcfd_hip_mesh_tile_layout(const cupcfd_hip_mesh *mesh, int32_t info[6]) {
+    if (!mesh || !info) return CUPCFD_E_NULL_PTR;
+    if (mesh->precision == 64) tile_layout(*mesh->m64, info);
+    else tile_layout(*mesh->m32, info);
     return CUPCFD_E_SUCCESS;
 }
 

@@ -140,6 +140,17 @@ template <class T> struct Mesh {
     int ncolours = -1;
     const int *colour_of = nullptr, *colour_faces = nullptr;
     std::vector<int> colour_off;
+    // The UVW tile's split LDS layout (tile_split_build, at creation): slots
+    // whose face has a cell outside the tile, or is a boundary face, take LDS
+    // columns 0 .. split_na - 1 of a 16-row table (the outside cell, then the
+    // slot's ten terms); faces between two cells of the tile take columns
+    // kSplitA + rank of a 9-row table. uvw_col[t * tslots + sl] is slot sl's
+    // column, uvw_eslot[e] is eslot[e] with the slot replaced by its column.
+    // split_na / split_nb: the most slots of each kind in any tile; uvw_split
+    // is 1 when every tile fits the layout (FP64 meshes only).
+    uint8_t *uvw_col = nullptr;
+    uint16_t *uvw_eslot = nullptr;
+    int split_na = 0, split_nb = 0, uvw_split = 0;
     // workspace pool for drop-in staging and multi-sweep scratch. A slot that
     // must grow gets a new buffer; the old one is retired, not freed, until
     // the mesh is destroyed, so a graph captured with the old buffer never
@@ -152,6 +163,9 @@ template <class T> struct Mesh {
 };
 
 template <class T> int *scalar_carry_src(Mesh<T> &M, bool inlet_assigns, bool wall_assigns);
+// M.uvw_col / M.uvw_eslot and M.uvw_split (on the null stream, synchronous);
+// a mesh whose tiles do not fit the split layout keeps uvw_split = 0
+template <class T> void tile_split_build(Mesh<T> &M);
 // M.rbnd / M.rbnd_off, built on first call; false if they cannot be allocated
 template <class T> bool region_boundaries(Mesh<T> &M);
 // Device copy of M.cell_order (faces = false) or M.face_order, uploaded on first call; null on failure

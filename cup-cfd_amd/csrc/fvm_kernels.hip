@@ -458,8 +458,10 @@ __device__ __forceinline__ void stage_centre(T *l, const V4<T> *g, int c0, int n
     const int i = threadIdx.x;
     if (i < 4 * nc && (i & 3) != 3) l[(i & 3) * ld + (i >> 2)] = ((const T *)(g + c0))[i];
 }
-// the CSR slots of the tile's rows into LDS; returns the tile's first entry
-template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> &m, uint16_t *le, int c0, int nc) {
+// the CSR slots of the tile's rows (src: m.eslot, or a column map of it) into
+// LDS; returns the tile's first entry
+template <class T>
+__device__ __forceinline__ int stage_eslots(const DevMesh<T> &m, const uint16_t *src, uint16_t *le, int c0, int nc) {
     int e0, e1;
     if (m.rowlen) {
         e0 = c0 * m.rowlen;
@@ -468,8 +470,11 @@ template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> 
         e0 = m.row[c0];
         e1 = m.row[c0 + nc];
     }
-    for (int i = threadIdx.x; i < e1 - e0; i += kTileBlock) le[i] = m.eslot[e0 + i];
+    for (int i = threadIdx.x; i < e1 - e0; i += kTileBlock) le[i] = src[e0 + i];
     return e0;
+}
+template <class T> __device__ __forceinline__ int stage_eslots(const DevMesh<T> &m, uint16_t *le, int c0, int nc) {
+    return stage_eslots(m, m.eslot, le, c0, nc);
 }
 
 template <class T, bool SWEEP0>
@@ -1546,22 +1551,35 @@ __global__ __launch_bounds__(LPC * 32) void k_uvw_vec(DevMesh<T> m, UvwArgs<T> a
     }
 }
 
-template <class T, int WC>
-__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_uvw_tile(DevMesh<T> m,
-                                                                                                 UvwArgs<T> a,
-                                                                                                 int per, int nblk) {
+// The UVW tile. Layout (NA = 0): u v w VisEff, dudx dvdx dwdx centre (xyz
+// each) in NF rows of W columns, columns [0, TC) the tile's cells, column
+// TC + sl the cell of slot sl's face outside the tile. Phase B overwrites rows
+// 0..9 of column TC + sl with the slot's terms once the thread of slot sl has
+// read that cell. W is WC when the mesh's tiles fit it (a compile-time row
+// stride, so LDS addresses become immediate offsets), else TC + tslots.
+// Split layout (NA > 0, the mesh's uvw_col / uvw_eslot): a slot with a cell
+// outside the tile or a boundary face takes column TC + col (col < NA) of the
+// NF-row table A, which holds that cell and then the slot's ten terms; a face
+// between two cells of the tile needs no cell column, and its nine terms go
+// to column col - NA of the 9-row table B. On 4x4x4 hex tiles (96 + 144
+// slots) that is 31.7 KB of LDS instead of 39.9 KB: five workgroups per CU
+// instead of four, with 96 VGPRs (waves_per_eu(5)) for five waves per SIMD.
+// The wider variants keep waves_per_eu(4): at 96 VGPRs they would spill to
+// scratch.
+template <class T, int WC, int NA>
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(WC == kTileCells + 240 ? 5 : 4))) void k_uvw_tile(
+    DevMesh<T> m, UvwArgs<T> a, int per, int nblk, const uint8_t *__restrict__ ucol,
+    const uint16_t *__restrict__ ueslot) {
     extern __shared__ __align__(16) unsigned char smem[];
-    // u v w VisEff, dudx dvdx dwdx centre (xyz each): NF rows of W columns,
-    // columns [0, TC) the tile's cells, column TC + sl the cell of slot sl's
-    // face outside the tile. Phase B overwrites rows 0..9 of column TC + sl
-    // with the slot's terms once the thread of slot sl has read that cell. W
-    // is WC when the mesh's tiles fit it (a compile-time row stride, so LDS
-    // addresses become immediate offsets), else TC + tslots.
     constexpr int TC = kTileCells, NF = 16;
-    const int S = m.tslots, W = WC ? WC : TC + S;
-    T *lx = (T *)smem;                         // [NF][W]
-    uint16_t *le = (uint16_t *)(lx + NF * W);  // [tnnz]
-    uint8_t *lf = (uint8_t *)(le + m.tnnz);    // [S]
+    static_assert(NA == 0 || (WC != 0 && NA < WC - TC), "split layout on a compile-time width");
+    constexpr int WB = NA ? WC - TC - NA : 1;    // table B's row stride
+    const int S = m.tslots;
+    const int W = NA ? TC + NA : (WC ? WC : TC + S);  // table A's row stride
+    T *lx = (T *)smem;                              // A: [NF][W]
+    T *lb = lx + NF * W;                            // B: [9][WB]
+    uint16_t *le = (uint16_t *)(lb + (NA ? 9 * WB : 0));  // [tnnz]
+    uint8_t *lf = (uint8_t *)(le + m.tnnz);         // [NA or S]
     const int t = xcd_block(blockIdx.x, per);
     if (t >= nblk) return;
     const int tid = threadIdx.x;
@@ -1576,21 +1594,21 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
     stage3(lx + 7 * W, a.dvdx, c0, nc, W);
     stage3(lx + 10 * W, a.dwdx, c0, nc, W);
     stage_centre(lx + 13 * W, m.cgeo, c0, nc, W);
-    const int eb = stage_eslots(m, le, c0, nc);
-    // phase C's fold item tid < 3 * TC: component k of cell cl, and its starting value
+    const int eb = stage_eslots(m, NA ? ueslot : m.eslot, le, c0, nc);
+    // phase C's fold item tid < 3 * TC: component k of cell cl
     const int k = tid / TC, cl = tid % TC;
     const bool fold = tid < 3 * TC && cl < nc;
     T *const dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
-    const T acc0 = fold ? dst[c0 + cl] : (T)0;
-    int f = 0, out = -1;
+    int f = 0, out = -1, col = 0;
     int2 cn = make_int2(0, 0);
     bool owned;
     V4<T> s{};
     T mf = 0, rl = 0;
     v3<T> uf{m.carry_init, m.carry_init, m.carry_init};
     // slot sl's face and, for an interior face, its carried uFace/vFace/wFace
-    // and its cell outside the tile into column TC + sl
+    // and its cell outside the tile into table A's column TC + col
     auto load = [&](int sl) {
+        col = NA ? (int)ucol[(size_t)t * S + sl] : sl;
         f = slot_conn(m, ts, sl, cn, owned);
         s = m.fgeoS[f];
         out = -1;
@@ -1605,7 +1623,7 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
             out = outside_cell(cn, c0, nc);
             if (out >= 0) {
                 const UvwCell<T> o = uvw_cell(m, a, out);
-                T *q = lx + TC + sl;
+                T *q = lx + TC + col;
                 q[0] = o.u, q[W] = o.v, q[2 * W] = o.w, q[3 * W] = o.vis;
                 q[4 * W] = o.du.x, q[5 * W] = o.du.y, q[6 * W] = o.du.z;
                 q[7 * W] = o.dv.x, q[8 * W] = o.dv.y, q[9 * W] = o.dv.z;
@@ -1616,11 +1634,12 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
     };
     if (tid < ts.nslot) load(tid);
     __syncthreads();
-    // B
+    // B; with WC - TC <= kTileBlock slots (every slot one thread's) a single pass
+    constexpr bool kOnePass = WC != 0 && WC - TC <= kTileBlock;
     for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
-        if (sl != tid) load(sl);  // column TC + sl belongs to this thread alone
+        if (!kOnePass && sl != tid) load(sl);  // the slot's column belongs to this thread alone
         auto cell = [&](int c) -> UvwCell<T> {
-            const T *q = lx + (c == out ? TC + sl : c - c0);
+            const T *q = lx + (c == out ? TC + col : c - c0);
             return UvwCell<T>{q[0],
                               q[W],
                               q[2 * W],
@@ -1638,10 +1657,19 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
         } else {
             fl = uvw_boundary(m, a, f, s, cell(cn.x), ~cn.y, tt);
         }
-        T *q = lx + TC + sl;
-        for (int j = 0; j < 10; j++) q[j * W] = tt[j];
-        lf[sl] = fl ? 1 : 0;
+        if (!NA || col < NA) {
+            T *q = lx + TC + col;
+            for (int j = 0; j < 10; j++) q[j * W] = tt[j];
+            lf[col] = fl ? 1 : 0;
+        } else {  // between two cells of the tile: never an a-updating face
+            T *q = lb + (col - NA);
+            for (int j = 0; j < 9; j++) q[j * WB] = tt[j];
+        }
+        if (kOnePass) break;
     }
+    // the fold's starting value, loaded before the barrier so that the wait
+    // overlaps it, and after phase B so that it holds no register there
+    const T acc0 = fold ? dst[c0 + cl] : (T)0;
     __syncthreads();
     // C: su/sv/sw, then au/av/aw for cells with an a-updating face
     if (fold) {
@@ -1649,19 +1677,25 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
         T acc = acc0;
         int e0, n;
         row_range(m, c, e0, n);
-        const T *l0 = lx + TC + (3 * k) * W, *l1 = l0 + W, *l2 = l1 + W;
+        const T *la = lx + TC + (3 * k) * W;
+        const T *lbk = lb + (3 * k) * WB;
         bool any = false;
         for (int e = e0; e < e0 + n; e++) {
             const int es = le[e - eb];
-            const int sl = es & 0x7fff;
-            T t0 = l0[sl], t1 = l1[sl], t2 = l2[sl];
+            const int sc = es & 0x7fff;
+            T t0, t1, t2;
+            if (!NA || sc < NA) {
+                t0 = la[sc], t1 = la[sc + W], t2 = la[sc + 2 * W];
+                any |= lf[sc] != 0;
+            } else {
+                t0 = lbk[sc - NA], t1 = lbk[sc - NA + WB], t2 = lbk[sc - NA + 2 * WB];
+            }
             if (es & 0x8000) {
                 t0 = -t0;
                 t1 = -t1;
                 t2 = -t2;
             }
             acc = ((acc + t0) + t1) + t2;
-            any |= lf[sl] != 0;
         }
         st_stream(dst + c, acc);
         if (any) {
@@ -1669,8 +1703,8 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
             const T *l9 = lx + TC + 9 * W;
             T aa = dsta[c];
             for (int e = e0; e < e0 + n; e++) {
-                const int sl = le[e - eb] & 0x7fff;
-                if (lf[sl]) aa = aa + l9[sl];
+                const int sc = le[e - eb] & 0x7fff;
+                if ((!NA || sc < NA) && lf[sc]) aa = aa + l9[sc];
             }
             dsta[c] = aa;
         }
@@ -3415,6 +3449,7 @@ template <class T> size_t tile_lds(const DevMesh<T> &d, int nf, int nt, bool fla
 // tile of 4x4x4 hex cells has 240) or kTileSlots2 slots, else 0 (TC +
 // tslots at run time).
 constexpr int kTileSlots1 = 240, kTileSlots2 = 320;
+static_assert(kTileSlots1 == 240, "k_uvw_tile's waves_per_eu names this width");
 template <class T> int tile_width(const DevMesh<T> &d) {
     if (d.tslots <= kTileSlots1) return kTileCells + kTileSlots1;
     if (d.tslots <= kTileSlots2) return kTileCells + kTileSlots2;
@@ -3426,6 +3461,84 @@ template <class T> size_t tile_lds_cols(const DevMesh<T> &d, int nf) {
     const size_t b = (size_t)nf * (w ? w : kTileCells + d.tslots) * sizeof(T) + 2 * (size_t)d.tnnz + d.tslots;
     return b <= 65536 ? b : 0;
 }
+
+// The UVW tile's split layout: table A of 16 rows x (TC + kSplitA) columns,
+// table B of 9 rows x (kTileSlots1 - kSplitA), the CSR column codes, and a
+// flag byte per table-A slot. kSplitA = 96 fits 4x4x4 hex tiles (48 owned
+// faces to the three neighbours above, 48 halo faces from the three below;
+// boundary faces take the place of either at the mesh's edges), which leaves
+// 144 faces between cells of the tile.
+constexpr int kSplitA = 96;
+template <class T> size_t tile_lds_split(const DevMesh<T> &d) {
+    return ((size_t)16 * (kTileCells + kSplitA) + (size_t)9 * (kTileSlots1 - kSplitA)) * sizeof(T) +
+           2 * (size_t)d.tnnz + kSplitA;
+}
+
+// slot sl of tile t has a cell outside the tile, or is a boundary face
+template <class T>
+__device__ __forceinline__ bool slot_kind_a(const DevMesh<T> &m, const TileSlots &ts, int sl, int c0, int nc) {
+    int2 cn;
+    bool owned;
+    (void)slot_conn(m, ts, sl, cn, owned);
+    return cn.y < 0 || outside_cell(cn, c0, nc) >= 0;
+}
+
+// per tile (one workgroup): its counts of table-A and table-B slots, folded
+// into mx[0], mx[1] by atomic maximum
+template <class T> __global__ __launch_bounds__(kTileBlock) void k_split_count(DevMesh<T> m, int *mx) {
+    __shared__ int cnt[2];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int c0 = t * kTileCells, nc = min(kTileCells, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    if (tid < 2) cnt[tid] = 0;
+    __syncthreads();
+    for (int sl = tid; sl < ts.nslot; sl += kTileBlock) atomicAdd(&cnt[slot_kind_a(m, ts, sl, c0, nc) ? 0 : 1], 1);
+    __syncthreads();
+    if (tid < 2) atomicMax(mx + tid, cnt[tid]);
+}
+
+// per tile (one workgroup; every tile has at most kTileBlock slots): the
+// column of each slot, table-A slots in slot order from 0 and table-B slots
+// in slot order from kSplitA, and the tile's CSR entries with the slot
+// replaced by its column
+template <class T>
+__global__ __launch_bounds__(kTileBlock) void k_split_cols(DevMesh<T> m, uint8_t *col, uint16_t *es) {
+    __shared__ int wa[kTileBlock / 64], wb[kTileBlock / 64];
+    __shared__ uint8_t lc[kTileBlock];
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = t * kTileCells, nc = min(kTileCells, m.C - c0);
+    const TileSlots ts = tile_slots(m, t);
+    const bool live = tid < ts.nslot;
+    const bool ka = live && slot_kind_a(m, ts, tid, c0, nc);
+    const uint64_t ba = __ballot(ka), bb = __ballot(live && !ka);
+    const uint64_t below = (1ull << lane) - 1;
+    if (lane == 0) {
+        wa[wave] = __popcll(ba);
+        wb[wave] = __popcll(bb);
+    }
+    __syncthreads();
+    int oa = 0, ob = 0;
+    for (int w = 0; w < wave; w++) oa += wa[w], ob += wb[w];
+    if (live) {
+        const int c = ka ? oa + __popcll(ba & below) : kSplitA + ob + __popcll(bb & below);
+        lc[tid] = (uint8_t)c;
+        col[(size_t)t * m.tslots + tid] = (uint8_t)c;
+    }
+    __syncthreads();
+    int e0, e1;
+    if (m.rowlen) {
+        e0 = c0 * m.rowlen;
+        e1 = (c0 + nc) * m.rowlen;
+    } else {
+        e0 = m.row[c0];
+        e1 = m.row[c0 + nc];
+    }
+    for (int i = tid; i < e1 - e0; i += kTileBlock) {
+        const uint16_t v = m.eslot[e0 + i];
+        es[e0 + i] = (uint16_t)(lc[v & 0x7fff] | (v & 0x8000));
+    }
+}
+
 
 // Gather schedule of gradient, scalar and UVW: the kernel's autotuned schedule
 // if any, else the mesh's. Automatic (schedule 0)
@@ -3834,13 +3947,23 @@ int dev_uvw(cupcfd_hip_mesh *h, T gammaBlend, T small, const T *u, const T *v, c
             hipLaunchKernelGGL(k_uvw_carry_vals<T>, dim3(blocks(d.F - d.Fi)), dim3(kBlock), 0, st, d, a, cuvw);
             a.cuvw = cuvw;
         }
-        if (lpc == kTile && w == kTileCells + kTileSlots1)
-            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots1>), gr.blocks, nthr, lds, st, d, a, gr.per,
-                               gr.nblk);
+        const uint8_t *nc8 = nullptr;
+        const uint16_t *nc16 = nullptr;
+        // the split layout (built for FP64 meshes only, where it lifts the
+        // tile from four to five workgroups per CU; in FP32 the plain layout
+        // already runs seven, and the split one ran 168 us against 160 at 128^3)
+        if (lpc == kTile && w == kTileCells + kTileSlots1 && M->uvw_split)
+            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots1, kSplitA>), gr.blocks, nthr,
+                               tile_lds_split(d), st, d, a, gr.per, gr.nblk, (const uint8_t *)M->uvw_col,
+                               (const uint16_t *)M->uvw_eslot);
+        else if (lpc == kTile && w == kTileCells + kTileSlots1)
+            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots1, 0>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk, nc8, nc16);
         else if (lpc == kTile && w == kTileCells + kTileSlots2)
-            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots2>), gr.blocks, nthr, lds, st, d, a, gr.per,
-                               gr.nblk);
-        else if (lpc == kTile) hipLaunchKernelGGL((k_uvw_tile<T, 0>), gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk);
+            hipLaunchKernelGGL((k_uvw_tile<T, kTileCells + kTileSlots2, 0>), gr.blocks, nthr, lds, st, d, a, gr.per,
+                               gr.nblk, nc8, nc16);
+        else if (lpc == kTile)
+            hipLaunchKernelGGL((k_uvw_tile<T, 0, 0>), gr.blocks, nthr, lds, st, d, a, gr.per, gr.nblk, nc8, nc16);
         else if (lpc == 8) hipLaunchKernelGGL((k_uvw_vec<T, 8>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 6) hipLaunchKernelGGL((k_uvw_vec<T, 6>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
         else if (lpc == 4) hipLaunchKernelGGL((k_uvw_vec<T, 4>), gr.blocks, nthr, 0, st, d, a, gr.per, gr.nblk);
@@ -4193,6 +4316,35 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
 }
 
 }  // namespace
+
+namespace cupcfd_hip {
+
+template <class T> void tile_split_build(Mesh<T> &M) {
+    const DevMesh<T> &d = M.d;
+    M.uvw_split = 0;
+    if (sizeof(T) != 8 || d.ntiles <= 0 || tile_width(d) != kTileCells + kTileSlots1) return;  // FP64 only
+    int *mx = nullptr;
+    if (hipMalloc(&mx, 2 * sizeof(int)) != hipSuccess) return;
+    int h[2] = {0, 0};
+    bool ok = hipMemcpy(mx, h, sizeof h, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_split_count<T>, dim3(d.ntiles), dim3(kTileBlock), 0, 0, d, mx);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(h, mx, sizeof h, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(mx);
+    M.split_na = h[0];
+    M.split_nb = h[1];
+    if (!ok || h[0] > kSplitA || h[1] > kTileSlots1 - kSplitA || tile_lds_split(d) > 65536) return;
+    if (!M.uvw_col && hipMalloc(&M.uvw_col, (size_t)d.ntiles * d.tslots) != hipSuccess) return;
+    if (!M.uvw_eslot && hipMalloc(&M.uvw_eslot, (size_t)d.nnz * sizeof(uint16_t)) != hipSuccess) return;
+    hipLaunchKernelGGL(k_split_cols<T>, dim3(d.ntiles), dim3(kTileBlock), 0, 0, d, M.uvw_col, M.uvw_eslot);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return;
+    M.uvw_split = 1;
+}
+template void tile_split_build<double>(Mesh<double> &);
+template void tile_split_build<float>(Mesh<float> &);
+
+}  // namespace cupcfd_hip
 
 // A set of device meshes of one precision for persistent multi-mesh launches
 struct cupcfd_hip_meshset {

@@ -115,7 +115,7 @@ HOST_SYMBOLS = [
     "cupcfd_host_mesh_destroy", "cupcfd_hdf5_mesh_create", "cupcfd_hip_mesh_create_f64", "cupcfd_hip_mesh_create_f32",
     "cupcfd_hip_mesh_destroy", "cupcfd_hip_mesh_counts", "cupcfd_hip_mesh_cell_order",
     "cupcfd_hip_mesh_face_order", "cupcfd_hip_mesh_set_carry_init", "cupcfd_hip_mesh_set_ordered_sums",
-    "cupcfd_hip_mesh_set_schedule",
+    "cupcfd_hip_mesh_tile_layout", "cupcfd_hip_mesh_set_schedule",
     "cupcfd_hip_mesh_autotune", "cupcfd_hip_mesh_create_device_f64", "cupcfd_hip_mesh_create_device_f32",
     "cupcfd_hip_mesh_digest", "cupcfd_hip_mesh_digest_field", "cupcfd_hip_mesh_generate_f64",
     "cupcfd_hip_mesh_generate_f32", "cupcfd_hip_mesh_face_colours", "cupcfd_hip_autotune_forget",
@@ -178,6 +178,7 @@ def _declare(L):
     L.cupcfd_hip_mesh_set_carry_init.argtypes = [ct.c_void_p, ct.c_double]
     L.cupcfd_hip_mesh_set_schedule.argtypes = [ct.c_void_p, ct.c_int32]
     L.cupcfd_hip_mesh_set_ordered_sums.argtypes = [ct.c_void_p, ct.c_int32]
+    L.cupcfd_hip_mesh_tile_layout.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_autotune.argtypes = [ct.c_void_p, ct.c_void_p]
     L.cupcfd_hip_mesh_push_state.argtypes = [ct.c_void_p]
     L.cupcfd_hip_mesh_pull_state.argtypes = [ct.c_void_p]
@@ -483,6 +484,14 @@ class DeviceMesh:
         """Boundary-loop sums in the reference's sequential order (True, the default: its bits) or as a
         fixed-shape parallel tree (False: faster, the same bits every call, not the sequential sum's)."""
         check("set_ordered_sums", lib().cupcfd_hip_mesh_set_ordered_sums(self._h, 1 if sequential else 0))
+
+    def tile_layout(self):
+        """The cell-tile layout: dict of tiles, slots (most of any tile), entries (most CSR entries of any
+        tile), outside (most slots with a cell outside their tile or a boundary face), inside (most slots
+        between two cells of one tile) and uvw_split (the UVW tile runs its split LDS layout)."""
+        info = (ct.c_int32 * 6)()
+        check("tile_layout", lib().cupcfd_hip_mesh_tile_layout(self._h, info))
+        return dict(zip(("tiles", "slots", "entries", "outside", "inside", "uvw_split"), list(info)))
 
     def autotune(self):
         """Time the gather schedules of gradient, scalar and UVW on this mesh and keep the fastest of each;

@@ -217,6 +217,14 @@ int cupcfd_hip_mesh_set_carry_init(cupcfd_hip_mesh *mesh, double value);
  * boundary lists and gives the same bits on every call, within a few ulps of
  * the sum's magnitude of the sequential result rather than equal to it. */
 int cupcfd_hip_mesh_set_ordered_sums(cupcfd_hip_mesh *mesh, int32_t sequential);
+/* The mesh's cell-tile layout (diagnostic; no counterpart in the reference):
+ * info[0..5] = number of 64-cell tiles, most face slots of any tile, most CSR
+ * entries of any tile, most slots of any tile whose face has a cell outside
+ * the tile or is a boundary face, most slots between two cells of one tile,
+ * and 1 when the UVW tile runs its split LDS layout on this mesh (FP64
+ * meshes whose tiles have at most 96 and 144 slots of the two kinds: every
+ * 4x4x4-cell hex tile), else 0. */
+int cupcfd_hip_mesh_tile_layout(const cupcfd_hip_mesh *mesh, int32_t info[6]);
 /* Gather schedule of the per-cell face loops (gradient, scalar, UVW): 64 for
  * tiles of 64 cells whose values are staged in LDS and whose faces are each
  * evaluated once; lanes evaluating one cell's faces in parallel (4, 6 or 8; a

@@ -281,3 +281,24 @@ def test_autotune_keeps_bits_and_state(meshname):
             outs.append(run(kernel, fvm, d, f, lambda: host.state(64))[1])
         for k in outs[1]:
             assert compare(outs[0][k], outs[1][k])[0], (kernel, k)
+
+
+@pytest.mark.gpu
+def test_uvw_split_layout_in_use_and_bit_identical():
+    """FP64 meshes of 4x4x4-cell hex tiles run the UVW tile's split LDS layout (96 slots with a cell outside the
+    tile or a boundary face, 144 between two cells of the tile); its results equal the one-thread-per-cell
+    schedule's bit for bit, and FP32 meshes and tet tiles keep the plain layout."""
+    host, dev = _setup("typed32", 64)
+    lay = dev.tile_layout()
+    assert lay["uvw_split"] == 1 and lay["outside"] == 96 and lay["inside"] == 144, lay
+    assert _setup("typed32", 32)[1].tile_layout()["uvw_split"] == 0
+    assert _setup("tets_typed", 64)[1].tile_layout()["uvw_split"] == 0
+    outs = []
+    for lpc in (64, 1):
+        d = ch.DeviceMesh(host, 64)
+        d.set_schedule(lpc)
+        f = W.Fields(host, seed=78)
+        host.set_state(seed_state(host))
+        outs.append(run("uvw", fvm, d, f, lambda: host.state(64))[1])
+    for k in outs[0]:
+        assert compare(outs[0][k], outs[1][k])[0], k
