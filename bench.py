@@ -216,20 +216,26 @@ def oracle_outputs(host, f, kernels=("gradient", "mass", "scalar", "uvw")):
 
 def parity_vs_oracle(dev, f, oracle):
     """Each kernel once through the drop-in C ABI on fresh copies of ``f`` against the oracle's outputs on the
-    same inputs: per output, bit identity and the max-norm relative difference."""
+    same inputs: per output, bit identity and the max-norm relative difference. Also the wall time of each
+    drop-in call (host arrays in and out over PCIe, synchronous): the reference's calling convention."""
     from cupcfd_hip import fvm
     from kernel_runs import compare, run
 
-    res, all_bits = {}, True
+    res, all_bits, wall = {}, True, {}
+    run("gradient", fvm, dev, f.copy(), lambda: {})  # first call: the host staging's one-time set-up
     for k, o in oracle.items():
-        _, g = run(k, fvm, dev, f.copy(), lambda: {})
+        fk = f.copy()
+        t0 = time.perf_counter()
+        _, g = run(k, fvm, dev, fk, lambda: {})
+        wall[k] = (time.perf_counter() - t0) * 1e3
         res[k] = {}
         for name, want in o.items():
             a, b = np.ascontiguousarray(g[name]), np.ascontiguousarray(want)
             bits = a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
             res[k][name] = {"bit_identical": bool(bits), "max_rel": compare(a, b)[1]}
             all_bits = all_bits and bits
-    return {"all_bit_identical": bool(all_bits), "kernels": res}
+    return {"all_bit_identical": bool(all_bits), "kernels": res, "dropin_ms": wall,
+            "dropin_sweep_ms": sum(wall.values())}
 
 
 def bench_config0(torch, iters=100):

@@ -175,6 +175,12 @@ template <class T> const int *device_order(Mesh<T> &M, bool faces);
 // + j] = src[k*ncomp + j] (internal -> mesh order), for k < n, j < ncomp
 template <class T> int stage_gather(T *dst, const T *raw, const int *order, int n, int ncomp, int64_t stride);
 template <class T> int stage_scatter(T *raw, const T *src, const int *order, int n, int ncomp);
+// Synchronous copies between pageable host memory and the device (the
+// drop-in entry points' arrays): large ones are pipelined through pinned
+// buffers by a few host threads, small ones take hipMemcpy. The device to host
+// copy first waits for the null stream. CUPCFD_E_SUCCESS or CUPCFD_E_ERROR.
+int staged_h2d(void *dev, const void *host, size_t bytes);
+int staged_d2h(void *host, const void *dev, size_t bytes);
 int sync_state_any(struct ::cupcfd_hip_mesh *h, bool push);
 
 // Counts and pointers a MiniSoA view must carry before either repack reads it.

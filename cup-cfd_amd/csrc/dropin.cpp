@@ -21,9 +21,9 @@ template <> Mesh<float> *mesh_of<float>(cupcfd_hip_mesh *h) { return h && h->pre
 
 // Staging of one drop-in call. Device buffers come from the mesh's workspace
 // pool (slots 0..31 for the kernels' arrays, 64..127 for the raw copies of
-// the caller's arrays). The host makes contiguous copies of the caller's
-// arrays, in the mesh's numbering; the permutation to and from the internal
-// numbering runs on the device (stage_gather / stage_scatter).
+// the caller's arrays). The caller's arrays are copied as they are, in the
+// mesh's numbering (staged_h2d / staged_d2h); the permutation to and from the
+// internal numbering runs on the device (stage_gather / stage_scatter).
 template <class T> struct Stage {
     using V3 = typename V3Of<T>::type;
     Mesh<T> &M;
@@ -71,12 +71,10 @@ template <class T> struct Stage {
         return o;
     }
     void copy_in(void *d, const void *src, size_t bytes) {
-        if (rc == CUPCFD_E_SUCCESS && bytes && hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
-            rc = CUPCFD_E_ERROR;
+        if (rc == CUPCFD_E_SUCCESS) ok(staged_h2d(d, src, bytes));
     }
     void copy_out(void *dst, const void *d, size_t bytes) {
-        if (rc == CUPCFD_E_SUCCESS && bytes && hipMemcpy(dst, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = CUPCFD_E_ERROR;
+        if (rc == CUPCFD_E_SUCCESS) ok(staged_d2h(dst, d, bytes));
     }
     void ok(int r) {
         if (rc == CUPCFD_E_SUCCESS && r != CUPCFD_E_SUCCESS) rc = r;
@@ -99,13 +97,12 @@ template <class T> struct Stage {
     }
     T *upload(size_t n) {
         T *d = dev(n);
-        if (d && n && hipMemcpy(d, h.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) rc = CUPCFD_E_ERROR;
+        if (d) copy_in(d, h.data(), n * sizeof(T));
         return d;
     }
     void download(const T *d, size_t n) {
         h.resize(n);
-        if (rc == CUPCFD_E_SUCCESS && n && hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
-            rc = CUPCFD_E_ERROR;
+        copy_out(h.data(), d, n * sizeof(T));
     }
     static T *at3(const V3 &v, int64_t i, int k) {
         int64_t stride = v.stride ? v.stride : (int64_t)(3 * sizeof(T));
