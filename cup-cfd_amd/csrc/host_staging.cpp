@@ -88,7 +88,7 @@ struct Staging {
     std::mutex m;
     int device = -1;
     pid_t pid = 0;
-    bool ready = false;
+    bool ready = false, failed = false;  // failed: set-up did not succeed; copies take hipMemcpy
     void *pin[kBufs] = {};
     hipStream_t s = nullptr;
     hipEvent_t ev[kBufs] = {};
@@ -97,9 +97,10 @@ struct Staging {
     bool setup() {  // with m held
         int dev = -1;
         if (hipGetDevice(&dev) != hipSuccess) return false;
-        if (ready && pid == getpid()) return dev == device;
-        if (pid != getpid()) ready = false;  // a forked child: its own buffers and threads
-        if (ready) return false;
+        if (pid != getpid()) ready = failed = false;  // a forked child: its own buffers and threads
+        if (ready) return dev == device;
+        if (failed) return false;
+        failed = true;
         for (int b = 0; b < kBufs; b++)
             if (hipHostMalloc(&pin[b], kChunk, hipHostMallocDefault) != hipSuccess) return false;
         if (hipStreamCreate(&s) != hipSuccess) return false;
@@ -107,6 +108,7 @@ struct Staging {
             if (hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess) return false;
             if (hipEventRecord(ev[b], s) != hipSuccess) return false;
         }
+        failed = false;
         const unsigned hw = std::thread::hardware_concurrency();
         pool = new CopyPool((int)std::clamp(hw / 2, 1u, 8u));
         device = dev;
