@@ -2845,9 +2845,10 @@ __global__ __launch_bounds__(kTileBlock) void k_adj_b_tile(DevMesh<T> m, AdjArgs
 // (readers rd[rd_off[k] .. rd_off[k + 1])): an interior reader's
 // -((D*GammaBlend)*mf), an INLET's that carries PhiFace -(SuBar[P]*f).
 // k_adj_carry_terms evaluates every face's term in face order (coalesced;
-// 0 for faces that read no carried value), k_adj_carry_gather puts them in
-// reader-list order, and k_adj_carry_sum adds each source's contiguous run of
-// them in order.
+// 0 for faces that read no carried value), and k_adj_carry_sum gives each
+// source one wave: its lanes gather 64 readers' terms at a time, and the
+// wave adds them in reader order (each term read from its lane with
+// readlane, so the chain of dependent adds is the only serial work).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_adj_carry_terms(DevMesh<T> m, AdjArgs<T> a, T *coef) {
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -2868,28 +2869,32 @@ __global__ __launch_bounds__(kBlock) void k_adj_carry_terms(DevMesh<T> m, AdjArg
     coef[r] = c;
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_carry_gather(int n, const int *rd, const T *coef, T *coefr) {
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e < n) coefr[e] = coef[rd[e]];
+// lane l's value of v, as a wave-uniform value (l wave-uniform)
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_adj_carry_sum(int nb, const int *rd_off, const T *coefr, T *pfbar) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_adj_carry_sum(int nb, const int *rd_off, const int *rd, const T *coef,
+                                                          T *pfbar) {
+    const int k = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);  // one wave per source
     if (k >= nb) return;
-    const int e1 = rd_off[k + 1];
+    const int lane = threadIdx.x & 63;
+    const int e0 = uni(rd_off[k]), e1 = uni(rd_off[k + 1]);
     T acc = 0.0;
-    int e = rd_off[k];
-    for (; e + 4 <= e1; e += 4) {  // four loads issued, then the adds in order
-        const T c0 = coefr[e], c1 = coefr[e + 1], c2 = coefr[e + 2], c3 = coefr[e + 3];
-        acc = acc + c0;
-        acc = acc + c1;
-        acc = acc + c2;
-        acc = acc + c3;
+    for (int b = e0; b < e1; b += 64) {
+        const int e = b + lane;
+        const T c = e < e1 ? coef[rd[e]] : (T)0;
+        const int cnt = min(64, e1 - b);
+        for (int i = 0; i < cnt; i++) acc = acc + lane_value(c, i);
     }
-    for (; e < e1; e++) acc = acc + coefr[e];
-    pfbar[k] = acc;
+    if (lane == 0) pfbar[k] = acc;
 }
 
 // Typed meshes: pass-A terms of boundary face g = Fi + k of cell P
@@ -4102,17 +4107,12 @@ int dev_scalar_adjoint_full(cupcfd_hip_mesh *h, int nGradient, const T *Phi, con
                 CUPCFD_E_SUCCESS)
             return CUPCFD_E_ERROR;
         T *coef = (T *)M->ws_get(62, (size_t)d.F * sizeof(T));
-        T *coefr = (T *)M->ws_get(63, (size_t)(nrd ? nrd : 1) * sizeof(T));
-        if (!coef || !coefr) return CUPCFD_E_ERROR;
+        if (!coef) return CUPCFD_E_ERROR;
         hipLaunchKernelGGL(k_scalar_carry_vals<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a.s, cval);
         a.s.cval = cval;
-        if (nrd) {
-            hipLaunchKernelGGL(k_adj_carry_terms<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, coef);
-            hipLaunchKernelGGL(k_adj_carry_gather<T>, dim3(blocks(nrd)), dim3(kBlock), 0, st, nrd, rd,
-                               (const T *)coef, coefr);
-        }
-        hipLaunchKernelGGL(k_adj_carry_sum<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, nb, rd_off, (const T *)coefr,
-                           pfbar);
+        if (nrd) hipLaunchKernelGGL(k_adj_carry_terms<T>, dim3(blocks(d.F)), dim3(kBlock), 0, st, d, a, coef);
+        hipLaunchKernelGGL(k_adj_carry_sum<T>, dim3((nb + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, st, nb,
+                           rd_off, rd, (const T *)coef, pfbar);
         hipLaunchKernelGGL(k_adj_bnd<T>, dim3(blocks(nb)), dim3(kBlock), 0, st, d, a, (const T *)pfbar, bterm);
         a.bterm = bterm;
     }
