@@ -436,6 +436,19 @@ __device__ __forceinline__ int slot_conn(const DevMesh<T> &m, const TileSlots &t
     cn = m.fconn[f];
     return f;
 }
+// slot_conn without branches: both loads go through selected pointers, so
+// they issue with the other loads of a phase's first round (sl < ts.nslot)
+template <class T>
+__device__ __forceinline__ int slot_conn_flat(const DevMesh<T> &m, const TileSlots &ts, int sl, int2 &cn,
+                                              bool &owned) {
+    const int nown = ts.ti.y + ts.ti.w;
+    owned = sl < nown;
+    const int fo = sl < ts.ti.y ? ts.ti.x + sl : ts.ti.z + (sl - ts.ti.y);
+    const int h = ts.h0 + (sl - nown);
+    cn = *(owned ? m.fconn + fo : m.thc + h);
+    const int fh = *(owned ? m.thp : m.thf + h);  // thp: any valid address when the slot is owned
+    return owned ? fo : fh;
+}
 __device__ __forceinline__ bool in_tile(int c, int c0, int nc) { return (unsigned)(c - c0) < (unsigned)nc; }
 // the cell of a slot's face outside the tile, or -1 (a slot has at most one)
 __device__ __forceinline__ int outside_cell(int2 cn, int c0, int nc) {
@@ -1585,31 +1598,57 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(WC =
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
-    // A
-    stage1(lx, a.u, c0, nc);
-    stage1(lx + W, a.v, c0, nc);
-    stage1(lx + 2 * W, a.w, c0, nc);
-    stage1(lx + 3 * W, a.visEff, c0, nc);
-    stage3(lx + 4 * W, a.dudx, c0, nc, W);
-    stage3(lx + 7 * W, a.dvdx, c0, nc, W);
-    stage3(lx + 10 * W, a.dwdx, c0, nc, W);
-    stage_centre(lx + 13 * W, m.cgeo, c0, nc, W);
-    const int eb = stage_eslots(m, NA ? ueslot : m.eslot, le, c0, nc);
+    // A. First round: every load that depends on no other load, at clamped
+    // indices so that all of them are unconditional and issue before the
+    // first wait; their LDS writes follow. Thread tid stages field tid / TC of
+    // u, v, w, VisEff (cell tid % TC), element tid of the packed gradients and
+    // of the centres, CSR slot codes tid and tid + kTileBlock, and its slot's
+    // column and connectivity.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const T x1 = (tid < TC ? a.u : tid < 2 * TC ? a.v : tid < 3 * TC ? a.w : a.visEff)[c0 + ci];
+    const size_t g0 = 3 * (size_t)c0 + min(tid, 3 * nc - 1);
+    const T gu = a.dudx[g0], gv = a.dvdx[g0], gw = a.dwdx[g0];
+    const T xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
+    int eb, ee;
+    if (m.rowlen) {
+        eb = c0 * m.rowlen;
+        ee = (c0 + nc) * m.rowlen;
+    } else {
+        eb = m.row[c0];
+        ee = m.row[c0 + nc];
+    }
+    const uint16_t *esrc = NA ? ueslot : m.eslot;
+    const int ne = ee - eb;
+    const uint16_t q0 = esrc[eb + min(tid, ne - 1)], q1 = esrc[eb + min(tid + kTileBlock, ne - 1)];
     // phase C's fold item tid < 3 * TC: component k of cell cl
     const int k = tid / TC, cl = tid % TC;
     const bool fold = tid < 3 * TC && cl < nc;
     T *const dst = k == 0 ? a.su : (k == 1 ? a.sv : a.sw);
-    int f = 0, out = -1, col = 0;
-    int2 cn = make_int2(0, 0);
+    const bool live = tid < ts.nslot;
+    const int sl0 = min(tid, ts.nslot - 1);
+    int col = NA ? (int)ucol[(size_t)t * S + sl0] : sl0;
+    int2 cn;
     bool owned;
+    int f = slot_conn_flat(m, ts, sl0, cn, owned);
+    if ((tid & (TC - 1)) < nc) lx[(tid / TC) * W + (tid & (TC - 1))] = x1;
+    if (tid < 3 * nc) {
+        const int r = tid % 3, c = tid / 3;
+        lx[(4 + r) * W + c] = gu;
+        lx[(7 + r) * W + c] = gv;
+        lx[(10 + r) * W + c] = gw;
+    }
+    if (tid < 4 * nc && (tid & 3) != 3) lx[(13 + (tid & 3)) * W + (tid >> 2)] = xc;
+    if (tid < ne) le[tid] = q0;
+    if (tid + kTileBlock < ne) le[tid + kTileBlock] = q1;
+    for (int i = tid + 2 * kTileBlock; i < ne; i += kTileBlock) le[i] = esrc[eb + i];
+    int out = -1;
     V4<T> s{};
     T mf = 0, rl = 0;
     v3<T> uf{m.carry_init, m.carry_init, m.carry_init};
-    // slot sl's face and, for an interior face, its carried uFace/vFace/wFace
-    // and its cell outside the tile into table A's column TC + col
-    auto load = [&](int sl) {
-        col = NA ? (int)ucol[(size_t)t * S + sl] : sl;
-        f = slot_conn(m, ts, sl, cn, owned);
+    // second round: slot sl's face and, for an interior face, its carried
+    // uFace/vFace/wFace and its cell outside the tile into table A's column
+    // TC + col (cn, f and col already set)
+    auto rest = [&]() {
         s = m.fgeoS[f];
         out = -1;
         if (cn.y >= 0) {
@@ -1632,7 +1671,13 @@ __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(WC =
             }
         }
     };
-    if (tid < ts.nslot) load(tid);
+    // a later slot sl of this thread (tiles of more than kTileBlock slots)
+    auto load = [&](int sl) {
+        col = NA ? (int)ucol[(size_t)t * S + sl] : sl;
+        f = slot_conn(m, ts, sl, cn, owned);
+        rest();
+    };
+    if (live) rest();
     __syncthreads();
     // B; with WC - TC <= kTileBlock slots (every slot one thread's) a single pass
     constexpr bool kOnePass = WC != 0 && WC - TC <= kTileBlock;
