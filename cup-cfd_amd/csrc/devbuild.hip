@@ -392,7 +392,7 @@ __global__ void k_tile_stats(int NT, int C, int TC, const int4 *tinfo, const int
         const int c0 = (int)t * TC, c1 = min(C, c0 + TC);
         atomicMax(&stat[S_TSLOTS], nslot);
         atomicMax(&stat[S_TNNZ], row[c1] - row[c0]);
-        bool ok = nslot <= 32767;
+        bool ok = nslot >= 1 && nslot <= 32767;  // as the host repack: 15-bit slots, none empty
         for (int k = ti.x; k < ti.x + ti.y; k++)
             if (fconn[k].x < c0 || fconn[k].x >= c1) ok = false;
         for (int k = ti.z; k < ti.z + ti.w; k++)

@@ -381,7 +381,8 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
             const int4 ti = tinfo[t];
             const int nslot = ti.y + ti.w + (thp[t + 1] - thp[t]);
             tslots = std::max(tslots, nslot);
-            if (nslot > 32767) fits = false;
+            // slot codes are 15 bits; a tile whose cells have no faces has no slots to stage
+            if (nslot > 32767 || nslot == 0) fits = false;
             const int c0 = t * TC, c1 = std::min(C, c0 + TC);
             tnnz = std::max(tnnz, row[c1] - row[c0]);
             // owned faces (whose per-face outputs this tile writes) have their owner in the tile

@@ -504,20 +504,54 @@ __global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, Grad
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
-    // A
-    stage1(lc, a.phi, c0, nc);
+    // A. First round: every load that depends on no other load, at clamped
+    // indices so that all of them are unconditional and issue before the
+    // first wait; their LDS writes follow. Thread tid stages phi of cell
+    // tid % TC (tid < TC) and, after the first sweep, element tid of the
+    // packed previous gradient and of the centres; CSR slot codes tid and
+    // tid + kTileBlock; its slot's connectivity; and its fold item's row and
+    // volume.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const T x1 = a.phi[c0 + ci];
+    T gd = 0, xc = 0;
     if (!SWEEP0) {
-        stage3(lc + TC, a.grado, c0, nc);
-        stage_centre(lc + 4 * TC, m.cgeo, c0, nc);
+        gd = a.grado[3 * (size_t)c0 + min(tid, 3 * nc - 1)];
+        xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
     }
-    const int eb = stage_eslots(m, le, c0, nc);
-    int f = 0, out = -1;
-    int2 cn = make_int2(0, 0);
+    int eb, ee;
+    if (m.rowlen) {
+        eb = c0 * m.rowlen;
+        ee = (c0 + nc) * m.rowlen;
+    } else {
+        eb = m.row[c0];
+        ee = m.row[c0 + nc];
+    }
+    const int ne = ee - eb;  // >= 1: a tile whose cells have no faces takes no tile schedule
+    const uint16_t q0 = m.eslot[eb + min(tid, ne - 1)], q1 = m.eslot[eb + min(tid + kTileBlock, ne - 1)];
+    const int k = tid / TC, cl = tid % TC;
+    const bool fold = tid < 3 * TC && cl < nc;
+    int e0, n;
+    row_range(m, c0 + ci, e0, n);
+    const T vol = m.cvol[c0 + ci];
+    const bool live = tid < ts.nslot;
+    const int sl0 = min(tid, ts.nslot - 1);
+    int2 cn;
     bool owned;
+    int f = slot_conn_flat(m, ts, sl0, cn, owned);
+    if (tid < nc) lc[tid] = x1;
+    if (!SWEEP0) {
+        if (tid < 3 * nc) lc[(1 + tid % 3) * TC + tid / 3] = gd;
+        if (tid < 4 * nc && (tid & 3) != 3) lc[(4 + (tid & 3)) * TC + (tid >> 2)] = xc;
+    }
+    if (tid < ne) le[tid] = q0;
+    if (tid + kTileBlock < ne) le[tid + kTileBlock] = q1;
+    for (int i = tid + 2 * kTileBlock; i < ne; i += kTileBlock) le[i] = m.eslot[eb + i];
+    int out = -1;
     V4<T> s{}, xf{};
     GradCell<T> o{};
-    auto load = [&](int sl) {
-        f = slot_conn(m, ts, sl, cn, owned);
+    // second round: slot sl's face values and its cell outside the tile (cn
+    // and f already set)
+    auto rest = [&]() {
         s = m.fgeoS[f];
         out = -1;
         if (cn.y >= 0) {
@@ -526,15 +560,12 @@ __global__ __launch_bounds__(kTileBlock) void k_gradient_tile(DevMesh<T> m, Grad
             if (out >= 0) o = grad_cell<T, SWEEP0>(m, a, out);
         }
     };
-    if (tid < ts.nslot) load(tid);
-    const int k = tid / TC, cl = tid % TC;
-    const bool fold = tid < 3 * TC && cl < nc;
-    int e0 = 0, n = 0;
-    T vol = 1;
-    if (fold) {
-        row_range(m, c0 + cl, e0, n);
-        vol = m.cvol[c0 + cl];
-    }
+    // a later slot sl of this thread (tiles of more than kTileBlock slots)
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        rest();
+    };
+    if (live) rest();
     __syncthreads();
     // B
     auto cell = [&](int c) -> GradCell<T> {
@@ -1151,24 +1182,49 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
-    // A
-    stage1(lx, a.phi, c0, nc);
-    stage1(lx + W, a.visEff, c0, nc);
-    stage3(lx + 2 * W, a.dPhidx, c0, nc, W);
-    stage_centre(lx + 5 * W, m.cgeo, c0, nc, W);
-    const int eb = stage_eslots(m, le, c0, nc);
-    const T su0 = tid < nc ? a.su[c0 + tid] : (T)0;  // phase C's starting value
-    int f = 0, out = -1;
-    int2 cn = make_int2(0, 0);
+    // A. First round: every load that depends on no other load, at clamped
+    // indices so that all of them are unconditional and issue before the
+    // first wait; their LDS writes follow. Thread tid stages phi (tid < TC)
+    // or VisEff (TC <= tid < 2 TC) of cell tid % TC, element tid of the packed
+    // gradient and of the centres, CSR slot codes tid and tid + kTileBlock,
+    // and its slot's connectivity; threads tid < nc also phase C's starting
+    // value.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const T x1 = (tid < TC ? a.phi : a.visEff)[c0 + ci];
+    const T gd = a.dPhidx[3 * (size_t)c0 + min(tid, 3 * nc - 1)];
+    const T xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
+    int eb, ee;
+    if (m.rowlen) {
+        eb = c0 * m.rowlen;
+        ee = (c0 + nc) * m.rowlen;
+    } else {
+        eb = m.row[c0];
+        ee = m.row[c0 + nc];
+    }
+    const int ne = ee - eb;  // >= 1: a tile whose cells have no faces takes no tile schedule
+    const uint16_t q0 = m.eslot[eb + min(tid, ne - 1)], q1 = m.eslot[eb + min(tid + kTileBlock, ne - 1)];
+    const T su0 = a.su[c0 + ci];
+    const bool live = tid < ts.nslot;
+    const int sl0 = min(tid, ts.nslot - 1);
+    int2 cn;
     bool owned;
+    int f = slot_conn_flat(m, ts, sl0, cn, owned);
+    if (tid < 2 * TC && (tid & (TC - 1)) < nc) lx[(tid / TC) * W + (tid & (TC - 1))] = x1;
+    if (tid < 3 * nc) lx[(2 + tid % 3) * W + tid / 3] = gd;
+    if (tid < 4 * nc && (tid & 3) != 3) lx[(5 + (tid & 3)) * W + (tid >> 2)] = xc;
+    if (tid < ne) le[tid] = q0;
+    if (tid + kTileBlock < ne) le[tid + kTileBlock] = q1;
+    for (int i = tid + 2 * kTileBlock; i < ne; i += kTileBlock) le[i] = m.eslot[eb + i];
+    int out = -1;
     V4<T> s{};
     T mf = 0, rl = 0, pf = m.carry_init;
-    // slot sl's face and, for an interior face, its carried PhiFace and its
-    // cell outside the tile into column TC + sl
-    auto load = [&](int sl) {
-        f = slot_conn(m, ts, sl, cn, owned);
+    // second round: slot sl's face values and, for an interior face, its
+    // carried PhiFace and its cell outside the tile into column TC + sl; for a
+    // boundary face its terms, evaluated ahead, into column TC + sl (cn and f
+    // already set)
+    auto rest = [&](int sl) {
         out = -1;
-        if (cn.y < 0) {  // a boundary face: its terms, evaluated ahead, into column TC + sl
+        if (cn.y < 0) {
             T *q = lx + TC + sl;
             if (a.bterm) {
                 const int k = f - m.Fi;
@@ -1197,7 +1253,12 @@ __global__ __launch_bounds__(kTileBlock) void k_scalar_tile(DevMesh<T> m, Scalar
             q[5 * W] = o.cc.x, q[6 * W] = o.cc.y, q[7 * W] = o.cc.z;
         }
     };
-    if (tid < ts.nslot) load(tid);
+    // a later slot sl of this thread (tiles of more than kTileBlock slots)
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        rest(sl);
+    };
+    if (live) rest(tid);
     __syncthreads();
     // B
     for (int sl = tid; sl < ts.nslot; sl += kTileBlock) {
