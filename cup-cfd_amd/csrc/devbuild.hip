@@ -11,7 +11,9 @@
 //               expression (built without contraction), and min/max of the
 //               bounding box are order-independent.
 //   face order  the host's counting sort by (boundary?, owner) bucket is
-//               stable == radix sort of (bucket, face id).
+//               stable == radix sort of (bucket, face id); its stable sort
+//               of each owner tile's interior run by neighbour group ==
+//               a stable radix sort of the interior run by (tile, group).
 //   CSR rows    slot 2f is face f's cell1 entry, 2f+1 its cell2 entry; a
 //               stable sort of the 2F slots by internal cell keeps ascending
 //               slot order within each row, which is the host's fill order.
@@ -204,6 +206,18 @@ __global__ void k_face_bucket(int F, int C, const int *c1, const int *fb, const 
     GRID_STRIDE(f, F) {
         key[f] = (unsigned)((fb[f] > -1 ? C : 0) + crank[c1[f]]);
         val[f] = (int)f;
+    }
+}
+
+// interior faces regrouped within each owner tile: same-tile faces first,
+// then the faces shared with each other tile, by that tile (key (t1, group))
+__global__ void k_face_group(int Fi, int TC, unsigned gb, const int *forder, const int *c1, const int *c2,
+                             const int *crank, uint64_t *key, int *val) {
+    GRID_STRIDE(k, Fi) {
+        const int f = forder[k];
+        const int t1 = crank[c1[f]] / TC, t2 = crank[c2[f]] / TC;
+        key[k] = ((uint64_t)t1 << gb) | (uint64_t)(t2 == t1 ? 0 : t2 + 1);
+        val[k] = f;
     }
 }
 
@@ -581,7 +595,8 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
     }
     W.launch(C, k_rank, C, (const int *)corder, crank);
 
-    // ---- face order: (boundary?, owner) buckets, mesh order within ----------
+    // ---- face order: (boundary?, owner) buckets, mesh order within, then ----
+    // ---- interior faces grouped by neighbour tile within each owner tile ---
     unsigned *fkey = W.alloc<unsigned>(F, false), *fskey = W.alloc<unsigned>(F, false);
     int *fval = W.alloc<int>(F, false), *forder = W.alloc<int>(F, false), *frank = W.alloc<int>(F, false);
     W.launch(F, k_face_bucket, F, C, (const int *)c1, (const int *)fb, (const int *)crank, fkey, fval);
@@ -590,6 +605,18 @@ template <class T, class View> int build_device(Mesh<T> &M, const View &v, int f
             return rocprim::radix_sort_pairs(tmp, bytes, fkey, fskey, fval, forder, (size_t)F, 0,
                                              bits_for(2 * (int64_t)C), (hipStream_t)0);
         });
+    if (Fi) {
+        const int NTg = (C + kTileCells - 1) / kTileCells;
+        const unsigned gb = bits_for(NTg);
+        uint64_t *gkey = W.alloc<uint64_t>(Fi, false), *gskey = W.alloc<uint64_t>(Fi, false);
+        int *gval = W.alloc<int>(Fi, false);
+        W.launch(Fi, k_face_group, Fi, kTileCells, gb, (const int *)forder, (const int *)c1, (const int *)c2,
+                 (const int *)crank, gkey, gval);
+        W.prim_call([&](void *tmp, size_t &bytes) {
+            return rocprim::radix_sort_pairs(tmp, bytes, gkey, gskey, gval, forder, (size_t)Fi, 0, 2 * gb,
+                                             (hipStream_t)0);
+        });
+    }
     W.launch(F, k_rank, F, (const int *)forder, frank);
 
     // ---- face and cell streams ---------------------------------------------

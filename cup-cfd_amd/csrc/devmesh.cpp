@@ -131,7 +131,8 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     M.cell_rank.assign(C, 0);
     for (int k = 0; k < C; k++) M.cell_rank[M.cell_order[k]] = k;
 
-    // ---- face order: (boundary?, owner) buckets, mesh order within ----------
+    // ---- face order: (boundary?, owner) buckets, mesh order within, then ----
+    // ---- interior faces grouped by neighbour tile within each owner tile ---
     std::vector<int> cnt(2 * (size_t)C + 1, 0);
     auto bucket = [&](int f) {
         int own = M.cell_rank[v.faceCell1ID[f]];
@@ -141,6 +142,30 @@ template <class T, class View> int build(Mesh<T> &M, const View &v, int flags) {
     for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
     M.face_order.assign(F, 0);
     for (int f = 0; f < F; f++) M.face_order[cnt[bucket(f)]++] = f;
+    // Interior faces, stably regrouped within each owner tile (kTileCells
+    // consecutive internal cells): first the faces whose cell2 is in the same
+    // tile, then those shared with each other tile, by that tile. A tile that
+    // reads its halo faces from a neighbour then reads one contiguous run of
+    // the neighbour's face records instead of records spread over all of its
+    // range. Owner order (then mesh order) is kept within each group.
+    {
+        // the bucket order already runs tile by tile: sort each tile's run
+        std::vector<int> grp(F, 0);
+        for (int f = 0; f < F; f++)
+            if (v.faceBoundaryID[f] <= -1) {
+                const int t1 = M.cell_rank[v.faceCell1ID[f]] / kTileCells;
+                const int t2 = M.cell_rank[v.faceCell2ID[f]] / kTileCells;
+                grp[f] = t2 == t1 ? 0 : t2 + 1;
+            }
+        for (int k0 = 0; k0 < Fi;) {
+            const int t = M.cell_rank[v.faceCell1ID[M.face_order[k0]]] / kTileCells;
+            int k1 = k0 + 1;
+            while (k1 < Fi && M.cell_rank[v.faceCell1ID[M.face_order[k1]]] / kTileCells == t) k1++;
+            std::stable_sort(M.face_order.begin() + k0, M.face_order.begin() + k1,
+                             [&](int a, int b) { return grp[a] < grp[b]; });
+            k0 = k1;
+        }
+    }
     M.face_rank.assign(F, 0);
     for (int k = 0; k < F; k++) M.face_rank[M.face_order[k]] = k;
 
