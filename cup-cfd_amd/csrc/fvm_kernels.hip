@@ -2766,19 +2766,45 @@ __global__ __launch_bounds__(kTileBlock) void k_adj_a_tile(DevMesh<T> m, AdjArgs
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
-    stage1(lc, a.subar, c0, nc);
-    stage1(lc + TC, a.s.visEff, c0, nc);
-    stage1(lc + 2 * TC, a.s.phi, c0, nc);
-    stage_centre(lc + 3 * TC, m.cgeo, c0, nc);
-    const int eb = stage_eslots(m, le, c0, nc);
-    int f = 0, out = -1;
-    int2 cn = make_int2(0, 0);
+    // First round: every load that depends on no other load, at clamped
+    // indices so that all of them issue before the first wait; their LDS
+    // writes follow. Thread tid stages SuBar, VisEff or phi (tid / TC = 0, 1,
+    // 2) of cell tid % TC, element tid of the centres, CSR slot codes tid and
+    // tid + kTileBlock, its slot's connectivity and its fold item's row.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const T x1 = (tid < TC ? a.subar : tid < 2 * TC ? a.s.visEff : a.s.phi)[c0 + ci];
+    const T xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
+    int eb, ee;
+    if (m.rowlen) {
+        eb = c0 * m.rowlen;
+        ee = (c0 + nc) * m.rowlen;
+    } else {
+        eb = m.row[c0];
+        ee = m.row[c0 + nc];
+    }
+    const int ne = ee - eb;  // >= 1: a tile whose cells have no faces takes no tile schedule
+    const uint16_t q0 = m.eslot[eb + min(tid, ne - 1)], q1 = m.eslot[eb + min(tid + kTileBlock, ne - 1)];
+    const int k = tid / TC, cl = tid % TC;  // fold item: component k (h xyz, dir) of cell cl
+    const bool fold = cl < nc;
+    int e0, n;
+    row_range(m, c0 + ci, e0, n);
+    const bool live = tid < ts.nslot;
+    const int sl0 = min(tid, ts.nslot - 1);
+    int2 cn;
     bool owned;
+    int f = slot_conn_flat(m, ts, sl0, cn, owned);
+    if (tid < 3 * TC && (tid & (TC - 1)) < nc) lc[(tid / TC) * TC + (tid & (TC - 1))] = x1;
+    if (tid < 4 * nc && (tid & 3) != 3) lc[(3 + (tid & 3)) * TC + (tid >> 2)] = xc;
+    if (tid < ne) le[tid] = q0;
+    if (tid + kTileBlock < ne) le[tid + kTileBlock] = q1;
+    for (int i = tid + 2 * kTileBlock; i < ne; i += kTileBlock) le[i] = m.eslot[eb + i];
+    int out = -1;
     V4<T> s{};
     T mf = 0, rl = 0, pf = m.carry_init, rb0 = 0, rb1 = 0;
     AdjCellA<T> o{};
-    auto load = [&](int sl) {
-        f = slot_conn(m, ts, sl, cn, owned);
+    // second round: slot sl's face values and its cell outside the tile (cn
+    // and f already set)
+    auto rest = [&]() {
         out = -1;
         if (cn.y < 0) return;
         s = m.fgeoS[f];
@@ -2792,11 +2818,12 @@ __global__ __launch_bounds__(kTileBlock) void k_adj_a_tile(DevMesh<T> m, AdjArgs
         out = outside_cell(cn, c0, nc);
         if (out >= 0) o = AdjCellA<T>{a.subar[out], a.s.visEff[out], a.s.phi[out], xyz(m.cgeo[out])};
     };
-    if (tid < ts.nslot) load(tid);
-    const int k = tid / TC, cl = tid % TC;  // fold item: component k (h xyz, dir) of cell cl
-    const bool fold = cl < nc;
-    int e0 = 0, n = 0;
-    if (fold) row_range(m, c0 + cl, e0, n);
+    // a later slot sl of this thread (tiles of more than kTileBlock slots)
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        rest();
+    };
+    if (live) rest();
     __syncthreads();
     auto cell = [&](int c) -> AdjCellA<T> {
         const bool o_ = c == out;
@@ -2868,16 +2895,47 @@ __global__ __launch_bounds__(kTileBlock) void k_adj_b_tile(DevMesh<T> m, AdjArgs
     const int tid = threadIdx.x;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const TileSlots ts = tile_slots(m, t);
-    stage3(lc, a.h, c0, nc);
-    if (CORR) stage_centre(lc + 3 * TC, m.cgeo, c0, nc);
-    const int eb = stage_eslots(m, le, c0, nc);
-    int f = 0, out = -1;
-    int2 cn = make_int2(0, 0);
+    // First round: every load that depends on no other load, at clamped
+    // indices so that all of them issue before the first wait; their LDS
+    // writes follow. Thread tid stages element tid of the packed h (and of
+    // the centres), CSR slot codes tid and tid + kTileBlock, its slot's
+    // connectivity and its fold item's row and starting value.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const T gd = a.h[3 * (size_t)c0 + min(tid, 3 * nc - 1)];
+    T xc = 0;
+    if (CORR) xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
+    int eb, ee;
+    if (m.rowlen) {
+        eb = c0 * m.rowlen;
+        ee = (c0 + nc) * m.rowlen;
+    } else {
+        eb = m.row[c0];
+        ee = m.row[c0 + nc];
+    }
+    const int ne = ee - eb;  // >= 1: a tile whose cells have no faces takes no tile schedule
+    const uint16_t q0 = m.eslot[eb + min(tid, ne - 1)], q1 = m.eslot[eb + min(tid + kTileBlock, ne - 1)];
+    const int k = tid / TC, cl = tid % TC;  // fold item: PhiBar (k = 0) or component k - 1 of hnext
+    const bool fold = k < (CORR ? 4 : 1) && cl < nc;
+    int e0, n;
+    row_range(m, c0 + ci, e0, n);
+    const T b0 = a.binit[c0 + ci];
+    const bool live = tid < ts.nslot;
+    const int sl0 = min(tid, ts.nslot - 1);
+    int2 cn;
     bool owned;
+    int f = slot_conn_flat(m, ts, sl0, cn, owned);
+    if (tid < 3 * nc) lc[(tid % 3) * TC + tid / 3] = gd;
+    if (CORR && tid < 4 * nc && (tid & 3) != 3) lc[(3 + (tid & 3)) * TC + (tid >> 2)] = xc;
+    if (tid < ne) le[tid] = q0;
+    if (tid + kTileBlock < ne) le[tid + kTileBlock] = q1;
+    for (int i = tid + 2 * kTileBlock; i < ne; i += kTileBlock) le[i] = m.eslot[eb + i];
+    T acc = k == 0 ? b0 : (T)0;
+    int out = -1;
     V4<T> s{}, xf{};
     v3<T> oh{}, occ{};
-    auto load = [&](int sl) {
-        f = slot_conn(m, ts, sl, cn, owned);
+    // second round: slot sl's face values and its cell outside the tile (cn
+    // and f already set)
+    auto rest = [&]() {
         out = -1;
         if (cn.y < 0) return;
         s = m.fgeoS[f];
@@ -2888,15 +2946,12 @@ __global__ __launch_bounds__(kTileBlock) void k_adj_b_tile(DevMesh<T> m, AdjArgs
             if (CORR) occ = xyz(m.cgeo[out]);
         }
     };
-    if (tid < ts.nslot) load(tid);
-    const int k = tid / TC, cl = tid % TC;  // fold item: PhiBar (k = 0) or component k - 1 of hnext
-    const bool fold = k < (CORR ? 4 : 1) && cl < nc;
-    int e0 = 0, n = 0;
-    T acc = 0;
-    if (fold) {
-        row_range(m, c0 + cl, e0, n);
-        if (k == 0) acc = a.binit[c0 + cl];
-    }
+    // a later slot sl of this thread (tiles of more than kTileBlock slots)
+    auto load = [&](int sl) {
+        f = slot_conn(m, ts, sl, cn, owned);
+        rest();
+    };
+    if (live) rest();
     __syncthreads();
     auto hof = [&](int c) -> v3<T> {
         if (c == out) return oh;
