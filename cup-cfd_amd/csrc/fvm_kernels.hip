@@ -621,41 +621,50 @@ template <class T> struct MassArgs {
     T *teB, *edB, *viseffB, *tB;
 };
 
-template <class T> __device__ void mass_interior(const DevMesh<T> &m, const MassArgs<T> &a, int f) {
-    const int2 cn = m.fconn[f];
-    const int ip = cn.x, in = cn.y;
-    const V4<T> s = m.fgeoS[f];
+// the cell values an interior face of the mass loop reads
+template <class T> struct MassCell {
+    v3<T> du, dv, dw, dp;
+    T den, u, v, w, p, ar;
+    V4<T> g;  // centre, volume
+};
+template <class T> __device__ __forceinline__ MassCell<T> mass_cell(const DevMesh<T> &m, const MassArgs<T> &a, int c) {
+    return MassCell<T>{ld3(a.dudx, c), ld3(a.dvdx, c), ld3(a.dwdx, c), ld3(a.dpdx, c), a.den[c], a.u[c],
+                       a.v[c],         a.w[c],         a.p[c],         a.ar[c],         m.cgeo[c]};
+}
+
+// Interior face f (MassKernels.ipp:137-195) between ip = cell1 (P) and
+// in = cell2 (N): mass flux and the two RFace entries.
+template <class T>
+__device__ __forceinline__ void mass_face(const MassArgs<T> &a, int f, const V4<T> &s, const V4<T> &xf,
+                                          const V4<T> &pa, const V2<T> &na, const MassCell<T> &N,
+                                          const MassCell<T> &P) {
     const T facn = s.w;
     const T facp = 1.0 - facn;
-    const v3<T> dudxac = lerp3(ld3(a.dudx, in), facn, ld3(a.dudx, ip), facp);
-    const v3<T> dvdxac = lerp3(ld3(a.dvdx, in), facn, ld3(a.dvdx, ip), facp);
-    const v3<T> dwdxac = lerp3(ld3(a.dwdx, in), facn, ld3(a.dwdx, ip), facp);
-    const T denf = a.den[in] * facn + a.den[ip] * facp;
-    const V4<T> gin = m.cgeo[in], gip = m.cgeo[ip];
-    const v3<T> cin = xyz(gin), cip = xyz(gip);
+    const v3<T> dudxac = lerp3(N.du, facn, P.du, facp);
+    const v3<T> dvdxac = lerp3(N.dv, facn, P.dv, facp);
+    const v3<T> dwdxac = lerp3(N.dw, facn, P.dw, facp);
+    const T denf = N.den * facn + P.den * facp;
+    const v3<T> cin = xyz(N.g), cip = xyz(P.g);
     const v3<T> xac = lerp3(cin, facn, cip, facp);
-    const V4<T> xf = m.fgeoX[f];
     const v3<T> delta = sub3(xyz(xf), xac);
-    const T uFace = a.u[in] * facn + a.u[ip] * facp + dot3(dudxac, delta);
-    const T vFace = a.v[in] * facn + a.v[ip] * facp + dot3(dvdxac, delta);
-    const T wFace = a.w[in] * facn + a.w[ip] * facp + dot3(dwdxac, delta);
+    const T uFace = N.u * facn + P.u * facp + dot3(dudxac, delta);
+    const T vFace = N.v * facn + P.v * facp + dot3(dvdxac, delta);
+    const T wFace = N.w * facn + P.w * facp + dot3(dwdxac, delta);
     const T mf = denf * (uFace * s.x + vFace * s.y + wFace * s.z);
     const v3<T> xnorm = normalise3(v3<T>{s.x, s.y, s.z});
-    const V4<T> pa = m.fgeoP[f];
-    const V2<T> na = m.fgeoN[f];
     const v3<T> xpac = xyz(pa);
     const v3<T> xnac{pa.w, na.x, na.y};
-    const v3<T> gp_ip = ld3(a.dpdx, ip);
-    const T pip = a.p[ip] + dot3(gp_ip, sub3(xpac, cip));
-    const T pin = a.p[in] + dot3(gp_ip, sub3(xpac, cin));  // dpdx[ip] (MassKernels.ipp:169)
+    const v3<T> gp_ip = P.dp;
+    const T pip = P.p + dot3(gp_ip, sub3(xpac, cip));
+    const T pin = N.p + dot3(gp_ip, sub3(xpac, cin));  // dpdx[ip] (MassKernels.ipp:169)
     const v3<T> xpn = sub3(xnac, xpac);
     const v3<T> xpn2 = sub3(cin, cip);
-    const T apv1 = a.den[ip] * a.ar[ip];
-    const T apv2 = a.den[in] * a.ar[in];
+    const T apv1 = P.den * P.ar;
+    const T apv2 = N.den * N.ar;
     T apv = apv2 * facn + apv1 * facp;
-    const T factv = gin.w * facn + gip.w * facp;
+    const T factv = N.g.w * facn + P.g.w * facp;
     apv *= xf.w * factv / dot3(xpn2, xnorm);
-    const v3<T> gp_in = ld3(a.dpdx, in);
+    const v3<T> gp_in = N.dp;
     const T dpx = (gp_in.x * facn + gp_ip.x * facp) * xpn.x;
     const T dpy = (gp_in.y * facn + gp_ip.y * facp) * xpn.y;
     const T dpz = (gp_in.z * facn + gp_ip.z * facp) * xpn.z;
@@ -663,6 +672,13 @@ template <class T> __device__ void mass_interior(const DevMesh<T> &m, const Mass
     st_stream(a.rface + 2 * (size_t)f, -fact);
     st_stream(a.rface + 2 * (size_t)f + 1, -fact);
     st_stream(a.massFlux + f, mf - fact * ((pin - pip) - dpx - dpy - dpz));
+}
+
+template <class T> __device__ void mass_interior(const DevMesh<T> &m, const MassArgs<T> &a, int f) {
+    const int2 cn = m.fconn[f];
+    const V4<T> s = m.fgeoS[f];
+    const MassCell<T> N = mass_cell(m, a, cn.y), P = mass_cell(m, a, cn.x);
+    mass_face(a, f, s, m.fgeoX[f], m.fgeoP[f], m.fgeoN[f], N, P);
 }
 
 // boundary faces of one owning cell, in face order (MassKernels.ipp:197-305)
@@ -723,6 +739,85 @@ __global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, in
     } else {
         const int k = (b - kXcd * per) * kBlock + threadIdx.x;
         if (k < m.nbc) mass_boundary_cell(m, a, m.bcells[k]);
+    }
+}
+
+// a2 on cell tiles: a workgroup stages the values of its tile's cells in LDS
+// with coalesced loads and evaluates the interior faces the tile owns (one
+// contiguous face range, every owner in the tile); a face's cell2 outside
+// the tile is gathered into registers. mass_face on the same values as
+// k_mass, so the same bits, with a fraction of k_mass's scattered gathers.
+// Blocks from kXcd * per on: the boundary cells, as in k_mass.
+template <class T>
+__global__ __launch_bounds__(kTileBlock) void k_mass_tile(DevMesh<T> m, MassArgs<T> a, int per, int nblk) {
+    constexpr int TC = kTileCells;
+    // rows: dudx, dvdx, dwdx, dpdx (x, y, z each), den, u, v, w, p, ar, centre x, y, z, volume
+    __shared__ T lc[22 * TC];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (b >= kXcd * per) {
+        const int k = (b - kXcd * per) * kTileBlock + tid;
+        if (k < m.nbc) mass_boundary_cell(m, a, m.bcells[k]);
+        return;
+    }
+    const int t = xcd_block(b, per);
+    if (t >= nblk) return;
+    const int c0 = t * TC, nc = min(TC, m.C - c0);
+    const int4 ti = m.tinfo[t];
+    // First round: every load that depends on no other load, at clamped
+    // indices so that all of them issue before the first wait. Thread tid
+    // stages element tid of the four packed gradients, den/u/v/w (tid / TC)
+    // and p/ar of cell tid % TC, element tid of the centres, and loads its
+    // first face's connectivity and geometry.
+    const int ci = min(tid & (TC - 1), nc - 1);
+    const size_t g0 = 3 * (size_t)c0 + min(tid, 3 * nc - 1);
+    const T gu = a.dudx[g0], gv = a.dvdx[g0], gw = a.dwdx[g0], gp = a.dpdx[g0];
+    const T x1 = (tid < TC ? a.den : tid < 2 * TC ? a.u : tid < 3 * TC ? a.v : a.w)[c0 + ci];
+    const T x2 = (tid < TC ? a.p : a.ar)[c0 + ci];
+    const T xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
+    const bool live = tid < ti.y;
+    const int f0 = ti.y > 0 ? ti.x + min(tid, ti.y - 1) : 0;
+    const int2 cn = m.fconn[f0];
+    const V4<T> s = m.fgeoS[f0], xf = m.fgeoX[f0], pa = m.fgeoP[f0];
+    const V2<T> na = m.fgeoN[f0];
+    if (tid < 3 * nc) {
+        const int r = tid % 3, c = tid / 3;
+        lc[r * TC + c] = gu;
+        lc[(3 + r) * TC + c] = gv;
+        lc[(6 + r) * TC + c] = gw;
+        lc[(9 + r) * TC + c] = gp;
+    }
+    if ((tid & (TC - 1)) < nc) {
+        lc[(12 + tid / TC) * TC + (tid & (TC - 1))] = x1;
+        if (tid < 2 * TC) lc[(16 + tid / TC) * TC + (tid & (TC - 1))] = x2;
+    }
+    if (tid < 4 * nc) lc[(18 + (tid & 3)) * TC + (tid >> 2)] = xc;
+    // second round: the face's cell2 when it lies outside the tile
+    const bool outside = live && !in_tile(cn.y, c0, nc);
+    MassCell<T> o{};
+    if (outside) o = mass_cell(m, a, cn.y);
+    __syncthreads();
+    auto cell = [&](int c) -> MassCell<T> {
+        const T *q = lc + (c - c0);
+        return MassCell<T>{v3<T>{q[0], q[TC], q[2 * TC]},
+                           v3<T>{q[3 * TC], q[4 * TC], q[5 * TC]},
+                           v3<T>{q[6 * TC], q[7 * TC], q[8 * TC]},
+                           v3<T>{q[9 * TC], q[10 * TC], q[11 * TC]},
+                           q[12 * TC],
+                           q[13 * TC],
+                           q[14 * TC],
+                           q[15 * TC],
+                           q[16 * TC],
+                           q[17 * TC],
+                           V4<T>{q[18 * TC], q[19 * TC], q[20 * TC], q[21 * TC]}};
+    };
+    if (live) mass_face(a, f0, s, xf, pa, na, outside ? o : cell(cn.y), cell(cn.x));
+    // tiles that own more faces than the workgroup has threads
+    for (int j = tid + kTileBlock; j < ti.y; j += kTileBlock) {
+        const int f = ti.x + j;
+        const int2 cj = m.fconn[f];
+        const MassCell<T> N = in_tile(cj.y, c0, nc) ? cell(cj.y) : mass_cell(m, a, cj.y);
+        mass_face(a, f, m.fgeoS[f], m.fgeoX[f], m.fgeoP[f], m.fgeoN[f], N, cell(cj.x));
     }
 }
 
@@ -3823,11 +3918,18 @@ int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, co
     const DevMesh<T> &d = M->d;
     MassArgs<T> a{dudx, dvdx, dwdx, dpdx, den, u, v, w, p, ar, denB, massFlux, su, rface, small, sTE, sTD, sV, sE,
                   teC, edC, vC, tC, teB, edB, vB, tB};
-    const Grid gi = xcd_grid(d.Fi, kBlock);
     const int nbb = blocks(d.nbc);
-    if (gi.blocks + nbb > 0)
-        hipLaunchKernelGGL(k_mass<T>, dim3(gi.blocks + nbb), dim3(kBlock), 0, (hipStream_t)stream, d, a, gi.per,
-                           gi.nblk);
+    if (d.ntiles) {  // cell tiles: every interior face is owned by the tile of its cell1
+        const Grid gt = xcd_grid(d.ntiles, 1);
+        static_assert(kTileBlock == kBlock, "boundary-cell blocks of k_mass_tile count kBlock cells");
+        hipLaunchKernelGGL(k_mass_tile<T>, dim3(gt.blocks + nbb), dim3(kTileBlock), 0, (hipStream_t)stream, d, a,
+                           gt.per, gt.nblk);
+    } else {
+        const Grid gi = xcd_grid(d.Fi, kBlock);
+        if (gi.blocks + nbb > 0)
+            hipLaunchKernelGGL(k_mass<T>, dim3(gi.blocks + nbb), dim3(kBlock), 0, (hipStream_t)stream, d, a, gi.per,
+                               gi.nblk);
+    }
     // the face loop's counters count boundary faces by region type
     if (icinl) *icinl += M->ntype[CUPCFD_RTYPE_INLET];
     if (icout) *icout += M->ntype[CUPCFD_RTYPE_OUTLET];
