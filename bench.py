@@ -975,6 +975,7 @@ def bench_tets(torch, n, K, steps, warmup):
     mesh_build = {"host_generate_s": t_gen, "host_repack_s": t_mesh - t_gen, "device_generate_s": t_dev,
                   "device_generate_identical": dgen.digest() == dev.digest(), "speedup_setup": t_mesh / t_dev}
     dgen = None
+    sched = dev.autotune()
     C, F, B = host.nCells, host.nFaces, host.nBoundaries
     f = W.Fields(host, seed=808)
     kernels = make_sweep(torch, host, dev, f)
@@ -985,6 +986,7 @@ def bench_tets(torch, n, K, steps, warmup):
     out = {"workload": f"plain FP64 gradient+mass+scalar+UVW sweep, {label} ({C} cells, {F} faces), one DEFAULT "
                        f"region", "cells": C, "faces": F, "boundaries": B, "mesh_setup_s": t_mesh,
            "mesh_build": mesh_build, "gcell_sweeps_per_s": C / step_s / 1e9, "ms_per_step": step_s * 1e3,
+           "gather_schedules": dict(zip(("gradient", "scalar", "uvw"), sched)),
            "kernels": kernel_report(C, F, F - B, B, per)}
     out["batched"] = bench_batched(torch, host, dev, K, steps=max(2, min(steps, 5)), warmup=1, label=label,
                                    oracle_sample=(0, K - 1))

@@ -115,8 +115,9 @@ template <class T> struct Mesh {
     int32_t ntype[5] = {0, 0, 0, 0, 0};               // boundary faces per region type
     // gather schedule: lanes per cell (0 = automatic, 1 = one thread per cell, 4, 6, 8; 64 = cell tiles)
     int schedule_lpc = 0;
-    // per-kernel schedules chosen by cupcfd_hip_mesh_autotune (gradient, scalar, UVW; -1 = schedule_lpc)
-    int schedule_k[3] = {-1, -1, -1};
+    // per-kernel schedules chosen by cupcfd_hip_mesh_autotune (gradient, scalar, UVW, mass flux; -1 =
+    // schedule_lpc)
+    int schedule_k[4] = {-1, -1, -1, -1};
     // scalar carry sources per (inlet assigns, wall assigns) combination
     int *sc_src[4] = {nullptr, nullptr, nullptr, nullptr};
     // per carry source (boundary face k = g - Fi): the faces that read the

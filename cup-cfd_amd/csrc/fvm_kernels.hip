@@ -3810,7 +3810,7 @@ __global__ __launch_bounds__(kTileBlock) void k_split_cols(DevMesh<T> m, uint8_t
 // thread per cell. Returns kTile, 4, 6, 8, or 0 for one thread per cell.
 constexpr int kTile = 64;   // the tile schedule's code in cupcfd_hip_mesh_set_schedule
 constexpr int kColour = 2;  // colour groups (gradient; the other gathers take the automatic schedule)
-enum { kGatherGradient = 0, kGatherScalar = 1, kGatherUvw = 2 };
+enum { kGatherGradient = 0, kGatherScalar = 1, kGatherUvw = 2, kMassFaces = 3 };
 template <class T> int gather_schedule(const Mesh<T> &M, int kind, int nf, int nt, bool flags, size_t &lds) {
     int l = M.schedule_k[kind] >= 0 ? M.schedule_k[kind] : M.schedule_lpc;
     lds = 0;
@@ -3919,7 +3919,11 @@ int dev_mass(cupcfd_hip_mesh *h, const T *dudx, const T *dvdx, const T *dwdx, co
     MassArgs<T> a{dudx, dvdx, dwdx, dpdx, den, u, v, w, p, ar, denB, massFlux, su, rface, small, sTE, sTD, sV, sE,
                   teC, edC, vC, tC, teB, edB, vB, tB};
     const int nbb = blocks(d.nbc);
-    if (d.ntiles) {  // cell tiles: every interior face is owned by the tile of its cell1
+    // cell tiles (every interior face is owned by the tile of its cell1) for
+    // the automatic and tile schedules, unless the autotuner timed one thread
+    // per face faster on this mesh; one thread per face otherwise
+    const int sch = M->schedule_k[kMassFaces] >= 0 ? M->schedule_k[kMassFaces] : M->schedule_lpc;
+    if (d.ntiles && (sch == 0 || sch == kTile)) {
         const Grid gt = xcd_grid(d.ntiles, 1);
         static_assert(kTileBlock == kBlock, "boundary-cell blocks of k_mass_tile count kBlock cells");
         hipLaunchKernelGGL(k_mass_tile<T>, dim3(gt.blocks + nbb), dim3(kTileBlock), 0, (hipStream_t)stream, d, a,
@@ -4470,8 +4474,9 @@ int dev_scalar_adjoint_batched(cupcfd_hip_mesh *h, int K, const TS *Phi, const T
 
 // Per-kernel schedule choice by timing: each candidate schedule (cell tiles,
 // the lane-per-face width the mesh allows, one thread per cell) of the
-// gradient, scalar and UVW gathers runs on scratch inputs on a private stream
-// (one warm-up, then the best of three timed calls) and the fastest is kept.
+// gradient, scalar and UVW gathers, and cell tiles or one thread per face for
+// the mass flux, runs on scratch inputs on a private stream (one warm-up, then
+// the best of three timed calls) and the fastest is kept.
 // The mesh's mutable boundary and region state is saved and restored around
 // the runs. Every schedule gives the same bits, so only speed is chosen.
 template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
@@ -4487,9 +4492,9 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
     cand[nc++] = 1;
     cand[nc++] = kColour;  // tried for the gradient only
     if (C == 0) {
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 4; k++) {
             M->schedule_k[k] = -1;
-            if (chosen) chosen[k] = 0;
+            if (chosen && k < 3) chosen[k] = 0;
         }
         return CUPCFD_E_SUCCESS;
     }
@@ -4528,6 +4533,10 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
             return dev_scalar<T>(h, cell(0), bnd(1), cell(7), cell(8), cell(9), cell(10), mf, cell(11), bnd(2),
                                  bnd(3), rface, cell(12), 1, 0, sp[0], 1, 2, sp[1], sp[2], 3, sp[3], 4, sp[4], sp[5],
                                  sp[6], sp[7], sp[8], sp[9], sp[10], st);
+        if (kind == kMassFaces)
+            return dev_mass<T>(h, cell(1), cell(4), cell(12), cell(21), cell(7), bnd(0), cell(0), cell(8), cell(9), mf,
+                               cell(10), cell(11), cell(15), rface, (T)1e-18, nullptr, nullptr, nullptr, nullptr, 0, 0,
+                               0, 0, cell(16), bnd(1), cell(17), bnd(2), cell(18), bnd(3), cell(19), bnd(4), st);
         return dev_uvw<T>(h, (T)0.9, (T)1e-18, cell(0), cell(7), cell(10), bnd(4), bnd(5), bnd(6), cell(11), bnd(7),
                           mf, cell(1), cell(4), cell(12), rface, cell(15), cell(16), cell(17), cell(18), cell(19),
                           cell(20), st);
@@ -4536,11 +4545,12 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     int rc = CUPCFD_E_SUCCESS;
-    for (int kind = 0; kind < 3 && rc == CUPCFD_E_SUCCESS; kind++) {
+    for (int kind = 0; kind < 4 && rc == CUPCFD_E_SUCCESS; kind++) {
         float best = 0;
         int pick = cand[0];
         for (int j = 0; j < nc && rc == CUPCFD_E_SUCCESS; j++) {
             if (cand[j] == kColour && kind != kGatherGradient) continue;
+            if (kind == kMassFaces && cand[j] != kTile && cand[j] != 1) continue;  // tiles or one thread per face
             M->schedule_k[kind] = cand[j];
             rc = run(kind);  // warm-up (and lazily built tables)
             float tmin = 0;
@@ -4559,7 +4569,7 @@ template <class T> int autotune(cupcfd_hip_mesh *h, int32_t *chosen) {
             }
         }
         M->schedule_k[kind] = pick;
-        if (chosen) chosen[kind] = pick;
+        if (chosen && kind < 3) chosen[kind] = pick;
     }
     {
         T *q = state;
@@ -5114,7 +5124,8 @@ namespace {
 
 // Autotuned schedules, cached per (GPU architecture, precision, mesh class)
 // for the process and, when CUPCFD_HIP_TUNE_CACHE names a file, across
-// processes: one "key gradient scalar uvw" line per entry. The mesh class is
+// processes: one "key gradient scalar uvw mass" line per entry (a line
+// without the mass entry leaves the mass flux on its automatic schedule). The mesh class is
 // what the choice depends on: row lengths, whether tiles apply and at which
 // LDS width, and the cell count to the nearest power of two.
 std::mutex tune_mutex;
@@ -5140,9 +5151,12 @@ template <class T> std::string tune_key(const Mesh<T> &M) {
 void tune_load(const char *path) {
     FILE *fp = std::fopen(path, "r");
     if (!fp) return;
-    char key[160];
-    int g, sc, u;
-    while (std::fscanf(fp, "%159s %d %d %d", key, &g, &sc, &u) == 4) tune_cache[key] = {g, sc, u};
+    char line[256], key[160];
+    while (std::fgets(line, sizeof line, fp)) {
+        int g, sc, u, ms = -1;
+        const int n = std::sscanf(line, "%159s %d %d %d %d", key, &g, &sc, &u, &ms);
+        if (n >= 4) tune_cache[key] = {g, sc, u, n == 5 ? ms : -1};
+    }
     std::fclose(fp);
 }
 
@@ -5150,7 +5164,8 @@ void tune_save(const char *path) {
     const std::string tmp = std::string(path) + ".tmp";
     FILE *fp = std::fopen(tmp.c_str(), "w");
     if (!fp) return;
-    for (const auto &e : tune_cache) std::fprintf(fp, "%s %d %d %d\n", e.first.c_str(), e.second[0], e.second[1], e.second[2]);
+    for (const auto &e : tune_cache)
+        std::fprintf(fp, "%s %d %d %d %d\n", e.first.c_str(), e.second[0], e.second[1], e.second[2], e.second[3]);
     if (std::fclose(fp) == 0) (void)std::rename(tmp.c_str(), path);
 }
 
@@ -5164,9 +5179,9 @@ template <class T> int autotune_cached(cupcfd_hip_mesh *h, int32_t *chosen) {
         if (path && *path) tune_load(path);
         auto it = tune_cache.find(key);
         if (it != tune_cache.end()) {
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < 4; k++) {
                 M->schedule_k[k] = it->second[k];
-                if (chosen) chosen[k] = it->second[k];
+                if (chosen && k < 3) chosen[k] = it->second[k];
             }
             return CUPCFD_E_SUCCESS;
         }
@@ -5177,7 +5192,7 @@ template <class T> int autotune_cached(cupcfd_hip_mesh *h, int32_t *chosen) {
     if (chosen)
         for (int k = 0; k < 3; k++) chosen[k] = pick[k];
     std::lock_guard<std::mutex> lock(tune_mutex);
-    tune_cache[key] = {pick[0], pick[1], pick[2]};
+    tune_cache[key] = {pick[0], pick[1], pick[2], M->schedule_k[kMassFaces]};
     if (path && *path) tune_save(path);
     return CUPCFD_E_SUCCESS;
 }

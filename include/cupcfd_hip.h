@@ -235,8 +235,10 @@ int cupcfd_hip_mesh_tile_layout(const cupcfd_hip_mesh *mesh, int32_t info[6]);
  * which each face adds its terms straight to its cells. The colouring is
  * built on the schedule's first call, which allocates and synchronises. 0 (default) takes tiles when the mesh's tiles fit (at most
  * 32767 face slots per tile and 64 KiB of LDS), else the row length of
- * all-tet / all-hex meshes, else 8. Every schedule gives bit-identical
- * results. */
+ * all-tet / all-hex meshes, else 8. The mass-flux face loop takes the same
+ * tiles (each tile evaluating the interior faces whose cell1 it holds) for 0
+ * and 64, and one thread per face for the other schedules. Every schedule
+ * gives bit-identical results. */
 int cupcfd_hip_mesh_set_schedule(cupcfd_hip_mesh *mesh, int32_t lanes_per_cell);
 /* The mesh's order-preserving face colouring, built on the device on first
  * use: colour(f) = 1 + the largest colour among the faces before f (in the
@@ -250,15 +252,18 @@ int cupcfd_hip_mesh_face_colours(cupcfd_hip_mesh *mesh, int32_t *colour, int32_t
 /* Choose the gather schedule of the gradient, scalar and UVW loops separately
  * by timing each candidate (cell tiles, the lane-per-face width the mesh
  * allows, one thread per cell, and colour groups for the gradient) on scratch
- * inputs on a private stream; the
+ * inputs on a private stream, and that of the mass-flux face loop (cell tiles
+ * or one thread per face) the same way; the
  * mesh's boundary and region state is left as it was. schedules (may be NULL)
- * receives the three choices. A later cupcfd_hip_mesh_set_schedule replaces
+ * receives the three gather choices (the mass-flux choice is kept on the mesh
+ * and in the cache file). A later cupcfd_hip_mesh_set_schedule replaces
  * them. The choice is cached per (GPU architecture, precision, mesh class:
  * row lengths, tile applicability and LDS width, typed regions, cell count
  * to the nearest power of two), so a later call on a mesh of the same class
  * takes it without timing; with the environment variable
  * CUPCFD_HIP_TUNE_CACHE naming a file, the cache is read from and written to
- * that file ("key gradient scalar uvw" per line), so it also persists across
+ * that file ("key gradient scalar uvw mass" per line; a line without the
+ * mass entry is read too), so it also persists across
  * processes. Synchronous; no counterpart in the reference. */
 int cupcfd_hip_mesh_autotune(cupcfd_hip_mesh *mesh, int32_t *schedules);
 /* Forget the process's autotune cache (the file, if any, is left alone). */

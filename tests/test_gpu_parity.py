@@ -213,17 +213,18 @@ SCATTERED = {"tets_scattered": dict(nx=6, ny=5, nz=4, tets=True, jitter=0.3, typ
 def test_tiles_of_scattered_cells(meshname):
     """Without the Morton renumbering a tile's 64 cells are scattered, so a tile can have more than 256 face
     slots and threads take a second slot, whose outside cell is staged after the first barrier: still the bits
-    of one thread per cell, in every LDS row width."""
+    of one thread per cell, in every LDS row width. The mass flux on tiles (a tile may own more faces than its
+    workgroup has threads) gives the bits of one thread per face."""
     host = ch.HostMesh(**SCATTERED[meshname]) if meshname in SCATTERED else _setup(meshname, 64)[0]
     outs = {}
     for lpc in (1, 64):
         dev = ch.DeviceMesh(host, 64, reorder=False)
         dev.set_schedule(lpc)
-        for kernel in ("gradient", "scalar", "scalar_wall", "uvw"):
+        for kernel in ("gradient", "mass", "scalar", "scalar_wall", "uvw"):
             f = W.Fields(host, seed=13)
             host.set_state(seed_state(host))
             outs[(lpc, kernel)] = run(kernel, fvm, dev, f, lambda: host.state(64))[1]
-    for kernel in ("gradient", "scalar", "scalar_wall", "uvw"):
+    for kernel in ("gradient", "mass", "scalar", "scalar_wall", "uvw"):
         for k in outs[(1, kernel)]:
             assert compare(outs[(1, kernel)][k], outs[(64, kernel)][k])[0], (kernel, k)
 
@@ -240,11 +241,13 @@ def test_autotune_cache_per_mesh_class(tmp_path, monkeypatch):
         a = ch.DeviceMesh(ch.HostMesh(12, permute_cells=True, seed=1), 64)
         ca = a.autotune()
         lines = [ln.split() for ln in path.read_text().splitlines() if ln.strip()]
-        assert len(lines) == 1 and [int(x) for x in lines[0][1:]] == ca
+        # "key gradient scalar uvw mass": the mass flux's choice is cell tiles (64) or one thread per face (1)
+        assert len(lines) == 1 and [int(x) for x in lines[0][1:4]] == ca and int(lines[0][4]) in (1, 64)
         b = ch.DeviceMesh(ch.HostMesh(12, permute_cells=True, permute_faces=True, seed=2), 64)
         assert b.autotune() == ca
         assert len([ln for ln in path.read_text().splitlines() if ln.strip()]) == 1
-        # a cache entry written by another process is what the next call takes
+        # a cache entry written by another process is what the next call takes (a line without the mass
+        # flux's entry is read too)
         path.write_text(f"{lines[0][0]} 1 1 1\n")
         L.cupcfd_hip_autotune_forget()
         c = ch.DeviceMesh(ch.HostMesh(12, seed=3), 64)
