@@ -231,3 +231,32 @@ def test_generate_rejects_bad_params():
         assert L.cupcfd_hip_mesh_generate_f64(ct.byref(p), 1, ct.byref(hh), ct.byref(dh)) == code, kw
         assert not hh.value and not dh.value
     assert L.cupcfd_hip_mesh_generate_f64(None, 1, None, None) == ch.E_NULL_PTR
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("on_device", [False, True])
+@pytest.mark.parametrize("meshname", ["typed_perm", "typed32", "tets16", "line"])
+def test_face_order_groups_faces_by_neighbour_tile(meshname, on_device):
+    """The internal face order, from either repack: interior faces before boundary faces; interior faces by
+    owner tile (64 consecutive internal cells), and within a tile first the faces whose cell2 is in the same
+    tile, then those shared with each other tile, by that tile, in owner then mesh order within each group.
+    So the faces a tile reads from one neighbour (its halo faces) are one contiguous run of records."""
+    host = ch.HostMesh(**MESHES[meshname])
+    dev = ch.DeviceMesh(host, 64, on_device=on_device)
+    a = host.a
+    fo = dev.face_order
+    rank = np.empty(host.nCells, np.int64)
+    rank[dev.cell_order] = np.arange(host.nCells)
+    bnd = a["faceBnd"][fo] > -1
+    Fi = int((~bnd).sum())
+    assert not bnd[:Fi].any() and bnd[Fi:].all()
+    own = rank[a["faceCell1"][fo[:Fi]]]
+    nb = rank[a["faceCell2"][fo[:Fi]]]
+    t1, t2 = own // 64, nb // 64
+    grp = np.where(t2 == t1, 0, t2 + 1)
+    key = np.stack([fo[:Fi], own, grp, t1])  # lexsort: last row is the primary key
+    assert np.array_equal(np.lexsort(key), np.arange(Fi))
+    # each (owner tile, neighbour tile) pair's faces are one contiguous run
+    pair = t1 * (host.nCells // 64 + 2) + grp
+    starts = np.flatnonzero(np.r_[True, pair[1:] != pair[:-1]])
+    assert len(np.unique(pair[starts])) == len(starts)
