@@ -37,6 +37,12 @@ MESHES = {
     "tets_typed": dict(nx=4, ny=3, nz=3, tets=True, jitter=0.2, typed=True, permute_cells=True,
                        permute_faces=True, face_extras=True, seed=5),
 }
+# meshes at the edges of the tile schedule: one cell (a tile that owns no interior face), and a slab one cell
+# thick (every cell owns boundary faces; the last tile is partial)
+ODD_MESHES = {
+    "one_cell": dict(nx=1, ny=1, nz=1, typed=True),
+    "slab": dict(nx=40, ny=3, nz=1, typed=True, permute_cells=True, permute_faces=True, seed=2),
+}
 # kernels whose FP64 and FP32 results must be bit-identical to the oracle
 BITEXACT = {"gradient", "gradient3", "mass", "mass_bl1", "mass_bl2", "mass_bl3", "mass_bl4", "mass_bl5",
             "mass_region", "scalar", "scalar_t", "uvw", "uvw_region1", "uvw_bnds1", "visc_cell1", "visc_cell2"}
@@ -47,7 +53,7 @@ _cache = {}
 def _setup(meshname, precision):
     key = (meshname, precision)
     if key not in _cache:
-        host = ch.HostMesh(**MESHES[meshname])
+        host = ch.HostMesh(**(MESHES[meshname] if meshname in MESHES else ODD_MESHES[meshname]))
         dev = ch.DeviceMesh(host, precision=precision, reorder=True)
         _cache[key] = (host, dev)
     return _cache[key]
@@ -55,7 +61,7 @@ def _setup(meshname, precision):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("meshname", list(MESHES))
+@pytest.mark.parametrize("meshname", list(MESHES) + list(ODD_MESHES))
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_kernel_parity(kernel, meshname, precision):
     host, dev = _setup(meshname, precision)
