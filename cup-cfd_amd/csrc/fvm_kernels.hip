@@ -744,20 +744,15 @@ __global__ __launch_bounds__(kBlock) void k_mass(DevMesh<T> m, MassArgs<T> a, in
 
 // a2 on cell tiles: a workgroup stages the values of its tile's cells in LDS
 // with coalesced loads and evaluates the interior faces the tile owns (one
-// contiguous face range, every owner in the tile). A face's cell2 outside
-// the tile is gathered before the barrier: for the range's last nh faces
-// (the faces to other tiles come last in a tile's range) by the threads past
-// the range into LDS columns TC..TC + nh (on a 4x4x4 hex tile, 192 faces,
-// the fourth wave gathers all 48 such cells), for earlier ones by the face's
-// own thread into registers. mass_face on the same values as k_mass, so the
-// same bits, with a fraction of k_mass's scattered gathers. Blocks from
-// kXcd * per on: the boundary cells, as in k_mass.
+// contiguous face range, every owner in the tile); a face's cell2 outside
+// the tile is gathered into registers. mass_face on the same values as
+// k_mass, so the same bits, with a fraction of k_mass's scattered gathers.
+// Blocks from kXcd * per on: the boundary cells, as in k_mass.
 template <class T>
 __global__ __launch_bounds__(kTileBlock) void k_mass_tile(DevMesh<T> m, MassArgs<T> a, int per, int nblk) {
-    constexpr int TC = kTileCells, NH = 64, LW = TC + NH;
-    // rows of LW columns: dudx, dvdx, dwdx, dpdx (x, y, z each), den, u, v, w, p, ar, centre x, y, z, volume;
-    // columns [0, TC) the tile's cells, TC + h the outside cell2 of the h-th of the tile's last nh faces
-    __shared__ T lc[22 * LW];
+    constexpr int TC = kTileCells;
+    // rows: dudx, dvdx, dwdx, dpdx (x, y, z each), den, u, v, w, p, ar, centre x, y, z, volume
+    __shared__ T lc[22 * TC];
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     if (b >= kXcd * per) {
@@ -769,73 +764,60 @@ __global__ __launch_bounds__(kTileBlock) void k_mass_tile(DevMesh<T> m, MassArgs
     if (t >= nblk) return;
     const int c0 = t * TC, nc = min(TC, m.C - c0);
     const int4 ti = m.tinfo[t];
-    // threads past the tile's faces gather the outside cells of its last nh
-    // faces (the cross faces, last in each tile's face range) into LDS
-    const int nh = max(0, min(NH, min(ti.y, kTileBlock - ti.y)));
-    const bool live = tid < ti.y;
-    const bool helper = !live && tid - ti.y < nh;
-    const int jj = live ? tid : (helper ? ti.y - nh + (tid - ti.y) : 0);
+    // First round: every load that depends on no other load, at clamped
+    // indices so that all of them issue before the first wait. Thread tid
+    // stages element tid of the four packed gradients, den/u/v/w (tid / TC)
+    // and p/ar of cell tid % TC, element tid of the centres, and loads its
+    // first face's connectivity and geometry.
     const int ci = min(tid & (TC - 1), nc - 1);
     const size_t g0 = 3 * (size_t)c0 + min(tid, 3 * nc - 1);
     const T gu = a.dudx[g0], gv = a.dvdx[g0], gw = a.dwdx[g0], gp = a.dpdx[g0];
     const T x1 = (tid < TC ? a.den : tid < 2 * TC ? a.u : tid < 3 * TC ? a.v : a.w)[c0 + ci];
     const T x2 = (tid < TC ? a.p : a.ar)[c0 + ci];
     const T xc = ((const T *)(m.cgeo + c0))[min(tid, 4 * nc - 1)];
-    const int f0 = ti.y > 0 ? ti.x + min(jj, ti.y - 1) : 0;
+    const bool live = tid < ti.y;
+    const int f0 = ti.y > 0 ? ti.x + min(tid, ti.y - 1) : 0;
     const int2 cn = m.fconn[f0];
     const V4<T> s = m.fgeoS[f0], xf = m.fgeoX[f0], pa = m.fgeoP[f0];
     const V2<T> na = m.fgeoN[f0];
     if (tid < 3 * nc) {
         const int r = tid % 3, c = tid / 3;
-        lc[r * LW + c] = gu;
-        lc[(3 + r) * LW + c] = gv;
-        lc[(6 + r) * LW + c] = gw;
-        lc[(9 + r) * LW + c] = gp;
+        lc[r * TC + c] = gu;
+        lc[(3 + r) * TC + c] = gv;
+        lc[(6 + r) * TC + c] = gw;
+        lc[(9 + r) * TC + c] = gp;
     }
     if ((tid & (TC - 1)) < nc) {
-        lc[(12 + tid / TC) * LW + (tid & (TC - 1))] = x1;
-        if (tid < 2 * TC) lc[(16 + tid / TC) * LW + (tid & (TC - 1))] = x2;
+        lc[(12 + tid / TC) * TC + (tid & (TC - 1))] = x1;
+        if (tid < 2 * TC) lc[(16 + tid / TC) * TC + (tid & (TC - 1))] = x2;
     }
-    if (tid < 4 * nc) lc[(18 + (tid & 3)) * LW + (tid >> 2)] = xc;
-    if (helper && !in_tile(cn.y, c0, nc)) {
-        const MassCell<T> o = mass_cell(m, a, cn.y);
-        T *q = lc + TC + (tid - ti.y);
-        q[0] = o.du.x, q[LW] = o.du.y, q[2 * LW] = o.du.z;
-        q[3 * LW] = o.dv.x, q[4 * LW] = o.dv.y, q[5 * LW] = o.dv.z;
-        q[6 * LW] = o.dw.x, q[7 * LW] = o.dw.y, q[8 * LW] = o.dw.z;
-        q[9 * LW] = o.dp.x, q[10 * LW] = o.dp.y, q[11 * LW] = o.dp.z;
-        q[12 * LW] = o.den, q[13 * LW] = o.u, q[14 * LW] = o.v, q[15 * LW] = o.w, q[16 * LW] = o.p,
-        q[17 * LW] = o.ar;
-        q[18 * LW] = o.g.x, q[19 * LW] = o.g.y, q[20 * LW] = o.g.z, q[21 * LW] = o.g.w;
-    }
+    if (tid < 4 * nc) lc[(18 + (tid & 3)) * TC + (tid >> 2)] = xc;
+    // second round: the face's cell2 when it lies outside the tile
+    const bool outside = live && !in_tile(cn.y, c0, nc);
+    MassCell<T> o{};
+    if (outside) o = mass_cell(m, a, cn.y);
     __syncthreads();
-    auto col = [&](int l) -> MassCell<T> {
-        const T *q = lc + l;
-        return MassCell<T>{v3<T>{q[0], q[LW], q[2 * LW]},
-                           v3<T>{q[3 * LW], q[4 * LW], q[5 * LW]},
-                           v3<T>{q[6 * LW], q[7 * LW], q[8 * LW]},
-                           v3<T>{q[9 * LW], q[10 * LW], q[11 * LW]},
-                           q[12 * LW],
-                           q[13 * LW],
-                           q[14 * LW],
-                           q[15 * LW],
-                           q[16 * LW],
-                           q[17 * LW],
-                           V4<T>{q[18 * LW], q[19 * LW], q[20 * LW], q[21 * LW]}};
+    auto cell = [&](int c) -> MassCell<T> {
+        const T *q = lc + (c - c0);
+        return MassCell<T>{v3<T>{q[0], q[TC], q[2 * TC]},
+                           v3<T>{q[3 * TC], q[4 * TC], q[5 * TC]},
+                           v3<T>{q[6 * TC], q[7 * TC], q[8 * TC]},
+                           v3<T>{q[9 * TC], q[10 * TC], q[11 * TC]},
+                           q[12 * TC],
+                           q[13 * TC],
+                           q[14 * TC],
+                           q[15 * TC],
+                           q[16 * TC],
+                           q[17 * TC],
+                           V4<T>{q[18 * TC], q[19 * TC], q[20 * TC], q[21 * TC]}};
     };
-    if (live) {
-        MassCell<T> N;
-        if (in_tile(cn.y, c0, nc)) N = col(cn.y - c0);
-        else if (tid >= ti.y - nh) N = col(TC + tid - (ti.y - nh));
-        else N = mass_cell(m, a, cn.y);
-        mass_face(a, f0, s, xf, pa, na, N, col(cn.x - c0));
-    }
+    if (live) mass_face(a, f0, s, xf, pa, na, outside ? o : cell(cn.y), cell(cn.x));
     // tiles that own more faces than the workgroup has threads
     for (int j = tid + kTileBlock; j < ti.y; j += kTileBlock) {
         const int f = ti.x + j;
         const int2 cj = m.fconn[f];
-        const MassCell<T> N = in_tile(cj.y, c0, nc) ? col(cj.y - c0) : mass_cell(m, a, cj.y);
-        mass_face(a, f, m.fgeoS[f], m.fgeoX[f], m.fgeoP[f], m.fgeoN[f], N, col(cj.x - c0));
+        const MassCell<T> N = in_tile(cj.y, c0, nc) ? cell(cj.y) : mass_cell(m, a, cj.y);
+        mass_face(a, f, m.fgeoS[f], m.fgeoX[f], m.fgeoP[f], m.fgeoN[f], N, cell(cj.x));
     }
 }
 
